@@ -1,0 +1,181 @@
+"""Benchmark: cell-updates/s of the NaSt2D time step at a fixed Poisson residual.
+
+Default workload: lid-driven cavity, 1024 x 1024 cells on the unit square,
+Re = 1000, lid u = 1, tau = 0.5 (adaptive step), gamma = 0.9, red-black SOR
+with omega = 1.8 to a fluid-masked residual of 1e-6 relative to rhs, FP64,
+zero initial fields.  A step is one full tick (boundary conditions, F/G,
+RHS, pressure solve to the residual, velocity update, next step size).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Prints one JSON line.  `value` = interior cells * K / wall seconds of the K
+timed ticks, inputs resident in HBM.  The roofline entry is for the dominant
+kernel (the red-black SOR pass), timed with HIP events on the library's
+stream inside the timed region; the CPU baseline is the oracle's
+lexicographic SOR (the reference algorithm, one core) timed on a bounded
+sample of the same grid.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "cell-updates/s at fixed Poisson residual: plain GPU SOR vs learned+batched"
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes per interior cell of one red-black sweep: each half-sweep
+# reads own p, other p, own rhs and writes own p at 4 B/cell, plus 1 B flags
+SWEEP_BYTES_PER_CELL = 33.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=1024, help="interior cells per side")
+    ap.add_argument("--eps", type=float, default=1e-6)
+    ap.add_argument("--check-every", type=int, default=16)
+    ap.add_argument("--sample", type=int, default=32, help="time every n-th launch")
+    ap.add_argument("--cpu-sweeps", type=int, default=150)
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def cpu_baseline(n, sweeps_per_step, nsweeps):
+    """Oracle (plain-C port of the reference, one core) on the same grid:
+    one tick's non-SOR stages plus `nsweeps` lexicographic SOR sweeps,
+    extrapolated to the GPU's measured sweeps per step."""
+    import numpy as np
+
+    from oracle import pyoracle as O
+    from stroemung_amd import presets
+
+    ct, vel = presets.lid_cavity(n, n)
+    kw = dict(delx=1.0 / n, dely=1.0 / n, delt=1e-4, gamma=0.9, reynolds=1000.0,
+              sor_absolute_epsilon=0.0, max_iterations=nsweeps, omega=1.8,
+              masked_stop=1)
+    sim = O.Sim(ct, vel, **kw)
+    sim.u[...] = np.random.default_rng(0).uniform(-0.1, 0.1, sim.u.shape)
+    assert sim.call("rebuild_boundary_list") == 0
+    t0 = time.perf_counter()
+    sim.call("set_boundary_u_and_v")
+    sim.call("calculate_f_and_g")
+    sim.call("calculate_rhs")
+    t1 = time.perf_counter()
+    sim.solve()
+    t2 = time.perf_counter()
+    sim.call("set_u_and_v")
+    t3 = time.perf_counter()
+    t_stages = (t1 - t0) + (t3 - t2)
+    t_sweep = (t2 - t1) / nsweeps
+    step_s = t_stages + sweeps_per_step * t_sweep
+    return {
+        "value": n * n / step_s,
+        "unit": "cell-updates/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": ("oracle lexicographic SOR (reference algorithm, masked residual test each "
+                   "sweep) on the %dx%d cavity: %d sweeps timed (%.3f ms/sweep) plus one "
+                   "tick's BC/FG/RHS/ADAP_UV (%.1f ms); extrapolated to %.0f sweeps per "
+                   "step as measured on the GPU" % (n, n, nsweeps, t_sweep * 1e3,
+                                                   t_stages * 1e3, sweeps_per_step)),
+    }
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(--gpus %d requested)" % a.gpus, file=sys.stderr)
+        sys.exit(2)
+    from stroemung_amd import presets
+    from stroemung_amd.simulation import Context, make_params
+
+    n = a.n
+    ct, vel = presets.lid_cavity(n, n)
+    prm = make_params(1.0 / n, 1.0 / n, 0.0, 0.9, 1000.0, a.eps, 10_000_000, 1.8, tau=0.5,
+                      solver="rb", check_every=a.check_every)
+    ctx = Context(0)
+    ch = ctx.add_grid(prm, ct, vel)
+    ctx.construct()
+    for _ in range(a.warmup):
+        ctx.tick()
+    ctx.synchronize()
+    ctx.reset_stats()
+    ctx.profile(a.sample)
+    sweeps = 0
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        its, _ = ctx.tick()
+        sweeps += its[ch]
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    ctx.profile(0)
+    st = ctx.state(ch)
+    elapsed = t1 - t0
+    cells = n * n
+    value = cells * a.steps / elapsed
+
+    kname = "rb_half"
+    ms, timed, launches = ctx.kernel_stats(kname)
+    avg_s = ms / 1e3 / max(timed, 1)
+    bytes_per_launch = SWEEP_BYTES_PER_CELL / 2.0 * cells
+    achieved = bytes_per_launch / avg_s / 1e9 if timed else None
+    kernels = {}
+    for k in ("rb_half", "ghost", "resid", "solve_check", "fg", "rhs", "adap_uv",
+              "bc_velocity", "ranges"):
+        kms, kt, kl = ctx.kernel_stats(k)
+        if kl:
+            kernels[k] = {"launches": kl, "timed": kt,
+                          "avg_us": round(kms * 1e3 / kt, 3) if kt else None}
+    sweeps_per_step = sweeps / a.steps
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": elapsed * 1e3 / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "lid-driven cavity %dx%d, Re=1000, tau=0.5, gamma=0.9, red-black SOR "
+                        "omega=1.8 to relative residual %g, fp64, zero initial fields" % (n, n, a.eps),
+            "solver": "rb",
+            "check_every": a.check_every,
+            "sweeps_per_step": sweeps_per_step,
+            "final_delt": st["delt"],
+            "final_time": st["time"],
+            "final_residual_ratio": (st["residual_squared"] / st["tolerance_squared"]
+                                     if st["tolerance_squared"] else None),
+        },
+        "roofline": {
+            "kernel": kname,
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS if achieved else None,
+            "traffic": None,
+            "bytes_per_launch": bytes_per_launch,
+            "avg_launch_us": avg_s * 1e6,
+            "timed_launches": timed,
+        },
+        "kernels": kernels,
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(n, sweeps_per_step, a.cpu_sweeps)
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

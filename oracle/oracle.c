@@ -425,6 +425,20 @@ int orc_solve_sor(orc_sim *s, uint32_t *iters, double *norm2)
                      + middle * ((((p_i_p1_j + p_i_m1_j) / delx2)
                                   + ((p_i_j_p1 + p_i_j_m1) / dely2)) - s->rhs[i]);
             }
+        if (s->masked_stop) {
+            /* extension: the red-black solver's fluid-masked relative test */
+            double rsum2 = 0.0;
+            double res2 = orc_masked_residual_sum(s, &rsum2);
+            norm_squared = res2 / s->fluid_cells;
+            s->last_res2 = norm_squared;
+            s->last_tol2 = sq(s->sor_absolute_epsilon) * rsum2 / s->fluid_cells;
+            if (res2 <= sq(s->sor_absolute_epsilon) * rsum2) {
+                *iters = (uint32_t)(it + 1);
+                *norm2 = norm_squared;
+                return 0;
+            }
+            continue;
+        }
         double init = orc_get_initial_norm_squared(s);
         norm_squared = orc_calculate_norm_squared(s);
         if ((norm_squared < init) || (norm_squared < epsilon_squared)) {

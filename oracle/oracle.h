@@ -63,6 +63,7 @@ typedef struct {
     double tau;            /* > 0: adaptive time step, book formula */
     int32_t solver;        /* ORC_SOLVER_LEX (reference) or ORC_SOLVER_RB */
     int32_t check_every;   /* RB: sweeps between residual checks */
+    int32_t masked_stop;   /* LEX: stop on the RB criterion instead of the reference's */
     /* fields, nx*ny each */
     double *u, *v, *p, *f, *g, *rhs;
     const uint8_t *cell_type;   /* ORC_* */

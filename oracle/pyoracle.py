@@ -53,6 +53,7 @@ class OrcSim(C.Structure):
         ("tau", C.c_double),
         ("solver", C.c_int32),
         ("check_every", C.c_int32),
+        ("masked_stop", C.c_int32),
         ("u", C.POINTER(C.c_double)), ("v", C.POINTER(C.c_double)),
         ("p", C.POINTER(C.c_double)), ("f", C.POINTER(C.c_double)),
         ("g", C.POINTER(C.c_double)), ("rhs", C.POINTER(C.c_double)),
@@ -150,7 +151,7 @@ class Sim:
     def __init__(self, cell_type, cell_vel=None, *, delx, dely, delt, gamma, reynolds,
                  sor_absolute_epsilon, max_iterations, omega, initial_norm_squared=None,
                  iterations=0, time=0.0, u=None, v=None, p=None, tau=0.0,
-                 solver=SOLVER_LEX, check_every=1):
+                 solver=SOLVER_LEX, check_every=1, masked_stop=0):
         self.cell_type = np.ascontiguousarray(cell_type, dtype=np.uint8)
         nx, ny = self.cell_type.shape
         if cell_vel is None:
@@ -175,6 +176,7 @@ class Sim:
         s.tau = tau
         s.solver = solver
         s.check_every = check_every
+        s.masked_stop = masked_stop
         s.u, s.v, s.p = _dp(self.u), _dp(self.v), _dp(self.p)
         s.f, s.g, s.rhs = _dp(self.f), _dp(self.g), _dp(self.rhs)
         s.cell_type = self.cell_type.ctypes.data_as(C.POINTER(C.c_uint8))

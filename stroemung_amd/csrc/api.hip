@@ -1,0 +1,906 @@
+// Host runtime and C ABI of libstroemung_hip.so.
+//
+// The context owns the device: one HIP stream, packed per-field allocations
+// for all channels, the boundary lists and the per-channel scalar state.  A
+// tick is a fixed sequence of kernel launches on that stream; the only host
+// synchronisation inside a tick is the red-black solve polling its
+// per-channel "still running" flags.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/stroemung_hip.h"
+#include "kernels.h"
+#include "stm_internal.h"
+
+using namespace stm;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct HostChan {
+    stm_params prm;
+    uint32_t nx = 0, ny = 0, pitch = 0;
+    int64_t off = 0;
+    std::vector<uint8_t> type;
+    std::vector<double> vel;       // 2 per cell
+    std::vector<int8_t> edge;
+    std::vector<uint32_t> blist;
+    double fluid_cells = 0.0;
+    int32_t tile0 = 0, ntiles = 0, boff = 0;
+};
+
+struct KStat {
+    uint64_t launches = 0, timed = 0;
+    double ms = 0.0;
+};
+
+struct PendingTiming {
+    KStat* stat;
+    hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct stm_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::vector<HostChan> chans;
+    bool finalized = false;
+    int64_t total = 0;             // elements per packed field
+    int ntiles = 0, nbtot = 0, check_every = 1;
+    bool any_rb = false, any_lex = false;
+    double *u = nullptr, *v = nullptr, *p = nullptr, *f = nullptr, *g = nullptr, *rhs = nullptr;
+    double* scratch = nullptr;
+    uint8_t* flags = nullptr;
+    ChanDesc* d_cd = nullptr;
+    ChanState* d_cs = nullptr;
+    BEntry* d_be = nullptr;
+    double2* d_restore = nullptr;
+    Tile* d_tiles = nullptr;
+    double* d_part = nullptr;
+    int32_t* d_active = nullptr;
+    int32_t* h_active = nullptr;   // pinned
+    ChanState* h_cs = nullptr;     // pinned
+    int be_capacity = 0;
+    // profiling
+    uint32_t sample = 0;
+    std::unordered_map<std::string, KStat> stats;
+    std::vector<PendingTiming> pending;
+    std::vector<hipEvent_t> free_events;
+};
+
+namespace {
+
+int fail(stm_ctx* ctx, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (ctx) ctx->err = buf;
+    g_last_error = buf;
+    return code;
+}
+
+#define HIPCHK(ctx, call)                                                              \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return fail((ctx), STM_ERR_HIP, "%s: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                           \
+    } while (0)
+
+int64_t round_up(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
+
+// src/grid/mod.rs:167-200 and 270-332: the edge kind of boundary cell (x, y)
+// from which of its four neighbours are fluid; -2 if too thin.
+int edge_kind(uint32_t nx, uint32_t ny, const uint8_t* t, uint32_t x, uint32_t y) {
+    const bool up = y > 0 && t[x * ny + (y - 1)] == STM_FLUID;
+    const bool down = y + 1 < ny && t[x * ny + (y + 1)] == STM_FLUID;
+    const bool right = x + 1 < nx && t[(x + 1) * ny + y] == STM_FLUID;
+    const bool left = x > 0 && t[(x - 1) * ny + y] == STM_FLUID;
+    const int code = (left ? 8 : 0) | (right ? 4 : 0) | (up ? 2 : 0) | (down ? 1 : 0);
+    switch (code) {
+    case 0: return E_NONE;
+    case 8: return E_W;
+    case 8 | 2: return E_NW;
+    case 2: return E_N;
+    case 4 | 2: return E_NE;
+    case 4: return E_E;
+    case 4 | 1: return E_SE;
+    case 1: return E_S;
+    case 8 | 1: return E_SW;
+    default: return -2;
+    }
+}
+
+const char* type_name(uint8_t t) {
+    switch (t) {
+    case STM_NOSLIP: return "Boundary(NoSlip)";
+    case STM_INFLOW: return "Boundary(Inflow)";
+    case STM_OUTFLOW: return "Boundary(Outflow)";
+    case STM_MOVING_WALL: return "Boundary(MovingWall)";
+    default: return "Fluid";
+    }
+}
+
+int classify(uint32_t nx, uint32_t ny, const uint8_t* t, int8_t* edge, uint32_t* list,
+             uint32_t* n, double* fluid, char* err, size_t errlen) {
+    uint32_t k = 0;
+    uint64_t nf = 0;
+    for (uint32_t x = 0; x < nx; x++)
+        for (uint32_t y = 0; y < ny; y++) {
+            const size_t i = (size_t)x * ny + y;
+            if (t[i] > STM_MOVING_WALL) {
+                if (err) snprintf(err, errlen, "unknown cell type %u at (%u, %u)", t[i], x, y);
+                return STM_ERR_INVALID_ARG;
+            }
+            if (t[i] == STM_FLUID) {
+                nf++;
+                edge[i] = -1;
+                continue;
+            }
+            const int e = edge_kind(nx, ny, t, x, y);
+            if (e < 0) {
+                if (err)
+                    snprintf(err, errlen,
+                             "BoundaryTooThinError: A cell `%s` at `(%u, %u)` has fluid on "
+                             "opposing sides.", type_name(t[i]), x, y);
+                return STM_ERR_INVALID_GRID;
+            }
+            edge[i] = (int8_t)e;
+            list[k++] = (uint32_t)i;
+        }
+    *n = k;
+    *fluid = (double)nf;
+    return STM_OK;
+}
+
+int classify_chan(stm_ctx* ctx, HostChan& c) {
+    const size_t n = (size_t)c.nx * c.ny;
+    c.edge.assign(n, 0);
+    c.blist.assign(n, 0);
+    uint32_t nb = 0;
+    char err[256] = {0};
+    const int rc = classify(c.nx, c.ny, c.type.data(), c.edge.data(), c.blist.data(), &nb,
+                            &c.fluid_cells, err, sizeof err);
+    if (rc != STM_OK) return fail(ctx, rc, "%s", err);
+    c.blist.resize(nb);
+    return STM_OK;
+}
+
+ChanDesc make_desc(const HostChan& c) {
+    ChanDesc d{};
+    const stm_params& q = c.prm;
+    d.nx = (int32_t)c.nx;
+    d.ny = (int32_t)c.ny;
+    d.pitch = (int32_t)c.pitch;
+    d.solver = q.solver;
+    d.off = c.off;
+    d.boff = c.boff;
+    d.nb = (int32_t)c.blist.size();
+    d.tile0 = c.tile0;
+    d.ntiles = c.ntiles;
+    d.itermax = (int32_t)q.max_iterations;
+    d.check_every = q.check_every > 0 ? q.check_every : 1;
+    d.delx = q.delx;
+    d.dely = q.dely;
+    d.gamma = q.gamma;
+    d.reynolds = q.reynolds;
+    d.omega = q.omega;
+    d.eps = q.sor_absolute_epsilon;
+    d.tau = q.tau;
+    d.fluid_cells = c.fluid_cells;
+    // The same expressions as the oracle (and src/simulation.rs:240-246).
+    d.delx2 = q.delx * q.delx;
+    d.dely2 = q.dely * q.dely;
+    d.omw = 1.0 - q.omega;
+    d.mid = q.omega / ((2.0 / d.delx2) + (2.0 / d.dely2));
+    d.rdx2 = 1.0 / d.delx2;
+    d.rdy2 = 1.0 / d.dely2;
+    d.eps2 = q.sor_absolute_epsilon * q.sor_absolute_epsilon;
+    return d;
+}
+
+void fill_entries(const HostChan& c, int chan, std::vector<BEntry>& out) {
+    const uint32_t ny = c.ny;
+    for (uint32_t i : c.blist) {
+        const uint32_t x = i / ny, y = i % ny;
+        BEntry e{};
+        e.gidx = c.off + (int64_t)x * c.pitch + y;
+        e.chan = chan;
+        e.pitch = (int32_t)c.pitch;
+        e.type = c.type[i];
+        e.edge = (uint8_t)c.edge[i];
+        e.colour = (uint8_t)((x + y) & 1);
+        e.bu = (e.type == STM_INFLOW || e.type == STM_MOVING_WALL) ? c.vel[2 * i] : 0.0;
+        e.bv = (e.type == STM_INFLOW || e.type == STM_MOVING_WALL) ? c.vel[2 * i + 1] : 0.0;
+        // Order rule of the parallel SETBCOND: a West / NorthWest cell reads
+        // v[W] after the wall cell at (x-1, y+1) has stored its bv there.
+        if ((e.edge == E_W || e.edge == E_NW) && y + 1 < ny) {
+            const uint32_t j = (x - 1) * ny + (y + 1);
+            const uint8_t tw = c.type[j];
+            if (tw == STM_NOSLIP || tw == STM_INFLOW || tw == STM_MOVING_WALL) {
+                e.vw_rule = 1;
+                e.vw_const = tw == STM_NOSLIP ? 0.0 : c.vel[2 * j + 1];
+            }
+        }
+        out.push_back(e);
+    }
+}
+
+int upload_boundaries(stm_ctx* ctx) {
+    std::vector<BEntry> be;
+    int boff = 0;
+    for (size_t k = 0; k < ctx->chans.size(); k++) {
+        ctx->chans[k].boff = boff;
+        fill_entries(ctx->chans[k], (int)k, be);
+        boff = (int)be.size();
+    }
+    ctx->nbtot = (int)be.size();
+    if (ctx->nbtot > ctx->be_capacity) {
+        if (ctx->d_be) HIPCHK(ctx, hipFree(ctx->d_be));
+        if (ctx->d_restore) HIPCHK(ctx, hipFree(ctx->d_restore));
+        ctx->d_be = nullptr;
+        ctx->d_restore = nullptr;
+        const int cap = std::max(ctx->nbtot, 1);
+        HIPCHK(ctx, hipMalloc(&ctx->d_be, sizeof(BEntry) * cap));
+        HIPCHK(ctx, hipMalloc(&ctx->d_restore, sizeof(double2) * cap));
+        ctx->be_capacity = cap;
+    }
+    if (ctx->nbtot)
+        HIPCHK(ctx, hipMemcpyAsync(ctx->d_be, be.data(), sizeof(BEntry) * be.size(),
+                                   hipMemcpyHostToDevice, ctx->stream));
+    return STM_OK;
+}
+
+int upload_flags(stm_ctx* ctx, const HostChan& c) {
+    std::vector<uint8_t> fl((size_t)c.nx * c.pitch, 0);
+    for (uint32_t x = 0; x < c.nx; x++)
+        for (uint32_t y = 0; y < c.ny; y++) {
+            const size_t i = (size_t)x * c.ny + y;
+            fl[(size_t)x * c.pitch + y] =
+                make_flag(c.type[i], c.type[i] == STM_FLUID ? 0 : (uint8_t)c.edge[i]);
+        }
+    HIPCHK(ctx, hipMemcpyAsync(ctx->flags + c.off, fl.data(), fl.size(), hipMemcpyHostToDevice,
+                               ctx->stream));
+    return STM_OK;
+}
+
+int upload_descs(stm_ctx* ctx) {
+    std::vector<ChanDesc> cd;
+    for (auto& c : ctx->chans) cd.push_back(make_desc(c));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->d_cd, cd.data(), sizeof(ChanDesc) * cd.size(),
+                               hipMemcpyHostToDevice, ctx->stream));
+    return STM_OK;
+}
+
+int finalize(stm_ctx* ctx) {
+    if (ctx->finalized) return STM_OK;
+    if (ctx->chans.empty()) return fail(ctx, STM_ERR_STATE, "no grid added");
+    int64_t off = 0;
+    int tiles = 0;
+    std::vector<Tile> tl;
+    ctx->any_rb = ctx->any_lex = false;
+    for (size_t k = 0; k < ctx->chans.size(); k++) {
+        HostChan& c = ctx->chans[k];
+        c.pitch = (uint32_t)round_up(c.ny, 16);
+        c.off = off;
+        off += round_up((int64_t)c.nx * c.pitch, 64);
+        c.tile0 = tiles;
+        for (uint32_t x0 = 0; x0 < c.nx; x0 += TILE_X)
+            for (uint32_t y0 = 0; y0 < c.ny; y0 += TILE_Y)
+                tl.push_back(Tile{(int32_t)k, (int32_t)x0, (int32_t)y0, 0});
+        c.ntiles = (int)tl.size() - tiles;
+        tiles = (int)tl.size();
+        if (c.prm.solver == STM_SOLVER_RB) ctx->any_rb = true; else ctx->any_lex = true;
+    }
+    ctx->check_every = std::max(1, ctx->chans[0].prm.check_every);
+    ctx->total = off;
+    ctx->ntiles = tiles;
+    const size_t bytes = sizeof(double) * (size_t)off;
+    double** fields[] = {&ctx->u, &ctx->v, &ctx->p, &ctx->f, &ctx->g, &ctx->rhs, &ctx->scratch};
+    for (double** fp : fields) {
+        HIPCHK(ctx, hipMalloc(fp, bytes));
+        HIPCHK(ctx, hipMemsetAsync(*fp, 0, bytes, ctx->stream));
+    }
+    HIPCHK(ctx, hipMalloc(&ctx->flags, (size_t)off));
+    HIPCHK(ctx, hipMemsetAsync(ctx->flags, 0, (size_t)off, ctx->stream));
+    const size_t nch = ctx->chans.size();
+    HIPCHK(ctx, hipMalloc(&ctx->d_cd, sizeof(ChanDesc) * nch));
+    HIPCHK(ctx, hipMalloc(&ctx->d_cs, sizeof(ChanState) * nch));
+    HIPCHK(ctx, hipMalloc(&ctx->d_tiles, sizeof(Tile) * tl.size()));
+    HIPCHK(ctx, hipMalloc(&ctx->d_part, sizeof(double) * NPART * tl.size()));
+    HIPCHK(ctx, hipMalloc(&ctx->d_active, sizeof(int32_t) * nch));
+    HIPCHK(ctx, hipHostMalloc(&ctx->h_active, sizeof(int32_t) * nch, hipHostMallocDefault));
+    HIPCHK(ctx, hipHostMalloc(&ctx->h_cs, sizeof(ChanState) * nch, hipHostMallocDefault));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->d_tiles, tl.data(), sizeof(Tile) * tl.size(),
+                               hipMemcpyHostToDevice, ctx->stream));
+    int rc = upload_boundaries(ctx);
+    if (rc) return rc;
+    for (auto& c : ctx->chans)
+        if ((rc = upload_flags(ctx, c))) return rc;
+    if ((rc = upload_descs(ctx))) return rc;
+    std::vector<ChanState> cs(nch);
+    for (size_t k = 0; k < nch; k++) {
+        const stm_params& q = ctx->chans[k].prm;
+        ChanState s{};
+        s.delt = q.delt;
+        s.time = q.time;
+        s.iterations = q.iterations;
+        s.has_init = q.has_initial_norm_squared ? 1 : 0;
+        s.init_norm2 = q.has_initial_norm_squared ? q.initial_norm_squared : 0.0;
+        cs[k] = s;
+    }
+    HIPCHK(ctx, hipMemcpyAsync(ctx->d_cs, cs.data(), sizeof(ChanState) * nch,
+                               hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->finalized = true;
+    return STM_OK;
+}
+
+// ------------------------------------------------------------- profiling
+
+int drain(stm_ctx* ctx) {
+    for (auto& t : ctx->pending) {
+        HIPCHK(ctx, hipEventSynchronize(t.b));
+        float ms = 0.f;
+        HIPCHK(ctx, hipEventElapsedTime(&ms, t.a, t.b));
+        t.stat->ms += ms;
+        t.stat->timed += 1;
+        ctx->free_events.push_back(t.a);
+        ctx->free_events.push_back(t.b);
+    }
+    ctx->pending.clear();
+    return STM_OK;
+}
+
+hipEvent_t take_event(stm_ctx* ctx) {
+    if (ctx->free_events.empty()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        return e;
+    }
+    hipEvent_t e = ctx->free_events.back();
+    ctx->free_events.pop_back();
+    return e;
+}
+
+template <class F>
+int launch(stm_ctx* ctx, const char* name, F&& fn) {
+    KStat& k = ctx->stats[name];
+    const bool timed = ctx->sample && (k.launches % ctx->sample) == 0;
+    k.launches++;
+    hipEvent_t a = nullptr, b = nullptr;
+    if (timed) {
+        a = take_event(ctx);
+        b = take_event(ctx);
+        if (!a || !b) return fail(ctx, STM_ERR_HIP, "hipEventCreate failed");
+        HIPCHK(ctx, hipEventRecord(a, ctx->stream));
+    }
+    fn();
+    if (timed) {
+        HIPCHK(ctx, hipEventRecord(b, ctx->stream));
+        ctx->pending.push_back(PendingTiming{&k, a, b});
+        if (ctx->pending.size() >= 2048) return drain(ctx);
+    }
+    return STM_OK;
+}
+
+#define LAUNCH(name, ...)                                      \
+    do {                                                       \
+        int rc_ = launch(ctx, name, [&] { __VA_ARGS__; });     \
+        if (rc_) return rc_;                                   \
+    } while (0)
+
+int check_launch(stm_ctx* ctx) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return fail(ctx, STM_ERR_HIP, "kernel launch: %s", hipGetErrorString(e));
+    return STM_OK;
+}
+
+int ready(stm_ctx* ctx) {
+    if (!ctx) return fail(nullptr, STM_ERR_INVALID_ARG, "null context");
+    int rc = finalize(ctx);
+    if (rc) return rc;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    return STM_OK;
+}
+
+int nch(stm_ctx* ctx) { return (int)ctx->chans.size(); }
+
+// ------------------------------------------------------------- stages
+
+int stage_bc(stm_ctx* ctx) {
+    LAUNCH("bc_velocity", launch_bc_velocity(ctx->stream, nch(ctx), ctx->d_cd, ctx->d_be,
+                                             ctx->d_restore, ctx->u, ctx->v));
+    return STM_OK;
+}
+
+int stage_fg(stm_ctx* ctx) {
+    LAUNCH("fg", launch_fg(ctx->stream, ctx->ntiles, ctx->d_cd, ctx->d_cs, ctx->d_tiles,
+                           ctx->flags, ctx->u, ctx->v, ctx->f, ctx->g));
+    return STM_OK;
+}
+
+int stage_rhs(stm_ctx* ctx) {
+    LAUNCH("rhs", launch_rhs(ctx->stream, ctx->ntiles, ctx->d_cd, ctx->d_cs, ctx->d_tiles,
+                             ctx->flags, ctx->f, ctx->g, ctx->rhs, ctx->d_part));
+    return STM_OK;
+}
+
+int rb_sweep(stm_ctx* ctx, int s) {
+    LAUNCH("ghost", launch_ghost(ctx->stream, ctx->nbtot, ctx->d_be, ctx->d_cs, ctx->p, -1, 1));
+    for (int colour = 0; colour < 2; colour++)
+        LAUNCH("rb_half", launch_rb_half(ctx->stream, ctx->ntiles, ctx->d_cd, ctx->d_cs,
+                                         ctx->d_tiles, ctx->flags, ctx->rhs, ctx->p, colour, s));
+    return STM_OK;
+}
+
+int rb_check(stm_ctx* ctx, int nsweeps) {
+    LAUNCH("ghost", launch_ghost(ctx->stream, ctx->nbtot, ctx->d_be, ctx->d_cs, ctx->p, -1, 1));
+    LAUNCH("resid", launch_resid(ctx->stream, ctx->ntiles, ctx->d_cd, ctx->d_cs, ctx->d_tiles,
+                                 ctx->flags, ctx->rhs, ctx->p, ctx->d_part));
+    LAUNCH("solve_check", launch_solve_check(ctx->stream, nch(ctx), ctx->d_cd, ctx->d_cs,
+                                             ctx->d_part, nsweeps, ctx->d_active));
+    return STM_OK;
+}
+
+// Reads the per-channel "still running" flags; *any = some channel runs.
+int poll_active(stm_ctx* ctx, bool* any) {
+    HIPCHK(ctx, hipMemcpyAsync(ctx->h_active, ctx->d_active, sizeof(int32_t) * nch(ctx),
+                               hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    int rc = check_launch(ctx);
+    if (rc) return rc;
+    *any = false;
+    for (int k = 0; k < nch(ctx); k++) *any = *any || ctx->h_active[k];
+    return STM_OK;
+}
+
+// POISSON: sets up tol2 and the per-channel flags from the rhs^2 partials
+// left by stage_rhs, then runs each channel's solver.
+int stage_solve(stm_ctx* ctx) {
+    LAUNCH("solve_begin", launch_solve_begin(ctx->stream, nch(ctx), ctx->d_cd, ctx->d_cs,
+                                             ctx->d_part, ctx->d_active));
+    if (ctx->any_lex)
+        LAUNCH("lex_solve", launch_lex_solve(ctx->stream, nch(ctx), ctx->d_cd, ctx->d_cs,
+                                             ctx->d_be, ctx->flags, ctx->rhs, ctx->p,
+                                             ctx->scratch));
+    if (!ctx->any_rb) return STM_OK;
+    const int every = ctx->check_every;
+    int batch = 1;     // check intervals queued between polls; grows to 32
+    for (;;) {
+        for (int b = 0; b < batch; b++) {
+            for (int s = 0; s < every; s++) {
+                int rc = rb_sweep(ctx, s);
+                if (rc) return rc;
+            }
+            int rc = rb_check(ctx, every);
+            if (rc) return rc;
+        }
+        bool any = false;
+        int rc = poll_active(ctx, &any);
+        if (rc) return rc;
+        if (!any) break;
+        if (batch < 32) batch *= 2;
+    }
+    return STM_OK;
+}
+
+int stage_adap(stm_ctx* ctx) {
+    LAUNCH("adap_uv", launch_adap_uv(ctx->stream, ctx->ntiles, ctx->d_cd, ctx->d_cs,
+                                     ctx->d_tiles, ctx->f, ctx->g, ctx->p, ctx->u, ctx->v));
+    LAUNCH("restore", launch_restore(ctx->stream, ctx->nbtot, ctx->d_be, ctx->d_restore,
+                                     ctx->u, ctx->v));
+    return STM_OK;
+}
+
+int stage_ranges(stm_ctx* ctx, int want_pressure) {
+    LAUNCH("ranges", launch_ranges(ctx->stream, ctx->ntiles, nch(ctx), ctx->d_cd, ctx->d_cs,
+                                   ctx->d_tiles, ctx->flags, ctx->u, ctx->v, ctx->p,
+                                   ctx->d_part, want_pressure));
+    return STM_OK;
+}
+
+int tick(stm_ctx* ctx) {
+    int rc;
+    LAUNCH("delt", launch_delt(ctx->stream, nch(ctx), ctx->d_cd, ctx->d_cs));
+    if ((rc = stage_bc(ctx))) return rc;
+    if ((rc = stage_fg(ctx))) return rc;
+    if ((rc = stage_rhs(ctx))) return rc;
+    if ((rc = stage_solve(ctx))) return rc;
+    if ((rc = stage_adap(ctx))) return rc;
+    if ((rc = stage_ranges(ctx, 2))) return rc;
+    LAUNCH("advance_time", launch_advance_time(ctx->stream, nch(ctx), ctx->d_cs));
+    return check_launch(ctx);
+}
+
+int read_states(stm_ctx* ctx) {
+    HIPCHK(ctx, hipMemcpyAsync(ctx->h_cs, ctx->d_cs, sizeof(ChanState) * nch(ctx),
+                               hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return check_launch(ctx);
+}
+
+int check_chan(stm_ctx* ctx, uint32_t ch) {
+    if (!ctx) return fail(nullptr, STM_ERR_INVALID_ARG, "null context");
+    if (ch >= ctx->chans.size())
+        return fail(ctx, STM_ERR_INVALID_ARG, "channel %u out of range (%zu channels)", ch,
+                    ctx->chans.size());
+    return STM_OK;
+}
+
+int check_params(stm_ctx* ctx, const stm_params* q) {
+    if (!q) return fail(ctx, STM_ERR_INVALID_ARG, "null params");
+    if (!(q->delx > 0.0) || !(q->dely > 0.0))
+        return fail(ctx, STM_ERR_INVALID_ARG, "cell size must be positive");
+    if (q->solver != STM_SOLVER_LEX && q->solver != STM_SOLVER_RB)
+        return fail(ctx, STM_ERR_INVALID_ARG, "unknown solver %d", q->solver);
+    return STM_OK;
+}
+
+}  // namespace
+
+// =================================================================== C ABI
+
+extern "C" {
+
+int stm_device_count(int* count) {
+    if (!count) return fail(nullptr, STM_ERR_INVALID_ARG, "null count");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0) {
+        *count = 0;
+        return fail(nullptr, STM_ERR_NO_DEVICE, "no HIP device: %s", hipGetErrorString(e));
+    }
+    *count = n;
+    return STM_OK;
+}
+
+int stm_create(int device, stm_ctx** out) {
+    if (!out) return fail(nullptr, STM_ERR_INVALID_ARG, "null out");
+    *out = nullptr;
+    int n = 0;
+    int rc = stm_device_count(&n);
+    if (rc) return rc;
+    if (device < 0 || device >= n)
+        return fail(nullptr, STM_ERR_NO_DEVICE, "device %d not present (%d devices)", device, n);
+    stm_ctx* ctx = new stm_ctx();
+    ctx->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete ctx;
+        return fail(nullptr, STM_ERR_HIP, "stream creation: %s", hipGetErrorString(e));
+    }
+    *out = ctx;
+    return STM_OK;
+}
+
+void stm_destroy(stm_ctx* ctx) {
+    if (!ctx) return;
+    hipSetDevice(ctx->device);
+    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    drain(ctx);
+    for (hipEvent_t e : ctx->free_events) hipEventDestroy(e);
+    void* bufs[] = {ctx->u, ctx->v, ctx->p, ctx->f, ctx->g, ctx->rhs, ctx->scratch, ctx->flags,
+                    ctx->d_cd, ctx->d_cs, ctx->d_be, ctx->d_restore, ctx->d_tiles, ctx->d_part,
+                    ctx->d_active};
+    for (void* b : bufs)
+        if (b) hipFree(b);
+    if (ctx->h_active) hipHostFree(ctx->h_active);
+    if (ctx->h_cs) hipHostFree(ctx->h_cs);
+    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char* stm_last_error(const stm_ctx* ctx) {
+    return ctx ? ctx->err.c_str() : g_last_error.c_str();
+}
+
+int stm_rebuild_boundary_list(uint32_t nx, uint32_t ny, const uint8_t* cell_type,
+                              int8_t* edge_out, uint32_t* list_out, uint32_t* n_out,
+                              double* fluid_cells_out, char* err, size_t errlen) {
+    if (!cell_type || !edge_out || !list_out || !n_out || !fluid_cells_out || nx == 0 || ny == 0)
+        return fail(nullptr, STM_ERR_INVALID_ARG, "invalid arguments");
+    const int rc = classify(nx, ny, cell_type, edge_out, list_out, n_out, fluid_cells_out, err,
+                            errlen);
+    if (rc && err) g_last_error = err;
+    return rc;
+}
+
+int stm_add_grid(stm_ctx* ctx, const stm_params* params, uint32_t nx, uint32_t ny,
+                 const uint8_t* cell_type, const double* cell_velocity, uint32_t* channel_out) {
+    if (!ctx) return fail(nullptr, STM_ERR_INVALID_ARG, "null context");
+    if (ctx->finalized)
+        return fail(ctx, STM_ERR_STATE, "grids must be added before the first upload/construct");
+    int rc = check_params(ctx, params);
+    if (rc) return rc;
+    if (!cell_type || nx < 3 || ny < 3)
+        return fail(ctx, STM_ERR_INVALID_ARG, "grid must be at least 3x3 with cell types");
+    if (!ctx->chans.empty() && params->check_every != ctx->chans[0].prm.check_every)
+        return fail(ctx, STM_ERR_INVALID_ARG, "all channels must share check_every");
+    HostChan c;
+    c.prm = *params;
+    c.nx = nx;
+    c.ny = ny;
+    const size_t n = (size_t)nx * ny;
+    c.type.assign(cell_type, cell_type + n);
+    c.vel.assign(2 * n, 0.0);
+    if (cell_velocity) std::copy(cell_velocity, cell_velocity + 2 * n, c.vel.begin());
+    if ((rc = classify_chan(ctx, c))) return rc;
+    ctx->chans.push_back(std::move(c));
+    if (channel_out) *channel_out = (uint32_t)(ctx->chans.size() - 1);
+    return STM_OK;
+}
+
+int stm_set_cells(stm_ctx* ctx, uint32_t ch, const uint8_t* cell_type,
+                  const double* cell_velocity) {
+    int rc = check_chan(ctx, ch);
+    if (rc) return rc;
+    if (!cell_type) return fail(ctx, STM_ERR_INVALID_ARG, "null cell types");
+    HostChan& c = ctx->chans[ch];
+    HostChan trial = c;
+    const size_t n = (size_t)c.nx * c.ny;
+    trial.type.assign(cell_type, cell_type + n);
+    if (cell_velocity) trial.vel.assign(cell_velocity, cell_velocity + 2 * n);
+    if ((rc = classify_chan(ctx, trial))) return rc;   // the old grid stays on error
+    c = std::move(trial);
+    if (!ctx->finalized) return STM_OK;
+    if ((rc = ready(ctx))) return rc;
+    if ((rc = upload_boundaries(ctx))) return rc;
+    if ((rc = upload_flags(ctx, c))) return rc;
+    if ((rc = upload_descs(ctx))) return rc;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return STM_OK;
+}
+
+int stm_set_params(stm_ctx* ctx, uint32_t ch, const stm_params* params) {
+    int rc = check_chan(ctx, ch);
+    if (rc) return rc;
+    if ((rc = check_params(ctx, params))) return rc;
+    if (params->check_every != ctx->chans[0].prm.check_every && ctx->chans.size() > 1)
+        return fail(ctx, STM_ERR_INVALID_ARG, "all channels must share check_every");
+    if (ctx->finalized && (params->solver == STM_SOLVER_RB) != (ctx->chans[ch].prm.solver == STM_SOLVER_RB))
+        ctx->any_rb = ctx->any_lex = true;
+    ctx->chans[ch].prm = *params;
+    if (ch == 0) ctx->check_every = std::max(1, params->check_every);
+    if (!ctx->finalized) return STM_OK;
+    if ((rc = ready(ctx))) return rc;
+    if ((rc = upload_descs(ctx))) return rc;
+    if ((rc = read_states(ctx))) return rc;
+    ChanState s = ctx->h_cs[ch];
+    s.delt = params->delt;
+    s.time = params->time;
+    s.iterations = params->iterations;
+    s.has_init = params->has_initial_norm_squared ? 1 : 0;
+    s.init_norm2 = params->has_initial_norm_squared ? params->initial_norm_squared : 0.0;
+    HIPCHK(ctx, hipMemcpyAsync(ctx->d_cs + ch, &s, sizeof s, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return STM_OK;
+}
+
+static int copy_field(stm_ctx* ctx, const HostChan& c, double* dev, const double* host,
+                      bool up) {
+    if (up)
+        HIPCHK(ctx, hipMemcpy2DAsync(dev + c.off, sizeof(double) * c.pitch, host,
+                                     sizeof(double) * c.ny, sizeof(double) * c.ny, c.nx,
+                                     hipMemcpyHostToDevice, ctx->stream));
+    else
+        HIPCHK(ctx, hipMemcpy2DAsync((void*)host, sizeof(double) * c.ny, dev + c.off,
+                                     sizeof(double) * c.pitch, sizeof(double) * c.ny, c.nx,
+                                     hipMemcpyDeviceToHost, ctx->stream));
+    return STM_OK;
+}
+
+int stm_upload(stm_ctx* ctx, uint32_t ch, const double* u, const double* v, const double* p,
+               const double* f, const double* g, const double* rhs) {
+    int rc = check_chan(ctx, ch);
+    if (rc || (rc = ready(ctx))) return rc;
+    const HostChan& c = ctx->chans[ch];
+    const double* src[] = {u, v, p, f, g, rhs};
+    double* dst[] = {ctx->u, ctx->v, ctx->p, ctx->f, ctx->g, ctx->rhs};
+    for (int k = 0; k < 6; k++)
+        if (src[k] && (rc = copy_field(ctx, c, dst[k], src[k], true))) return rc;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return STM_OK;
+}
+
+int stm_download(stm_ctx* ctx, uint32_t ch, double* u, double* v, double* p, double* f,
+                 double* g, double* rhs) {
+    int rc = check_chan(ctx, ch);
+    if (rc || (rc = ready(ctx))) return rc;
+    const HostChan& c = ctx->chans[ch];
+    double* dst[] = {u, v, p, f, g, rhs};
+    double* src[] = {ctx->u, ctx->v, ctx->p, ctx->f, ctx->g, ctx->rhs};
+    for (int k = 0; k < 6; k++)
+        if (dst[k] && (rc = copy_field(ctx, c, src[k], dst[k], false))) return rc;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return check_launch(ctx);
+}
+
+int stm_get_state(stm_ctx* ctx, uint32_t ch, stm_state* out) {
+    int rc = check_chan(ctx, ch);
+    if (rc || (rc = ready(ctx))) return rc;
+    if (!out) return fail(ctx, STM_ERR_INVALID_ARG, "null out");
+    if ((rc = read_states(ctx))) return rc;
+    const ChanState& s = ctx->h_cs[ch];
+    stm_state o{};
+    o.delt = s.delt;
+    o.time = s.time;
+    o.initial_norm_squared = s.init_norm2;
+    o.norm_squared = s.norm2;
+    o.residual_squared = s.res2;
+    o.tolerance_squared = s.tol2;
+    o.pressure_range[0] = s.pressure_range[0];
+    o.pressure_range[1] = s.pressure_range[1];
+    o.speed_range[0] = s.speed_range[0];
+    o.speed_range[1] = s.speed_range[1];
+    o.umax = s.umax;
+    o.vmax = s.vmax;
+    o.iterations = s.iterations;
+    o.sor_iterations = s.sor_iters;
+    o.has_initial_norm_squared = s.has_init;
+    o.exhausted = s.exhausted;
+    *out = o;
+    return STM_OK;
+}
+
+int stm_construct(stm_ctx* ctx) {
+    int rc = ready(ctx);
+    if (rc) return rc;
+    if ((rc = stage_ranges(ctx, 1))) return rc;
+    launch_fill(ctx->stream, ctx->total, ctx->f, 0.0);
+    launch_fill(ctx->stream, ctx->total, ctx->g, 0.0);
+    launch_fill(ctx->stream, ctx->total, ctx->rhs, 0.0);
+    if ((rc = stage_fg(ctx))) return rc;
+    if ((rc = stage_rhs(ctx))) return rc;
+    LAUNCH("initial_norm", launch_initial_norm(ctx->stream, nch(ctx), ctx->d_cd, ctx->d_cs,
+                                               ctx->rhs, ctx->p, ctx->scratch));
+    return read_states(ctx);
+}
+
+int stm_run_simulation_tick(stm_ctx* ctx, uint32_t* sor_iterations, double* norm_squared) {
+    int rc = ready(ctx);
+    if (rc || (rc = tick(ctx)) || (rc = read_states(ctx))) return rc;
+    for (int k = 0; k < nch(ctx); k++) {
+        if (sor_iterations) sor_iterations[k] = ctx->h_cs[k].sor_iters;
+        if (norm_squared) norm_squared[k] = ctx->h_cs[k].norm2;
+    }
+    return drain(ctx);
+}
+
+int stm_run_ticks(stm_ctx* ctx, uint32_t nticks) {
+    int rc = ready(ctx);
+    if (rc) return rc;
+    for (uint32_t t = 0; t < nticks; t++)
+        if ((rc = tick(ctx))) return rc;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if ((rc = check_launch(ctx))) return rc;
+    return drain(ctx);
+}
+
+int stm_set_boundary_u_and_v(stm_ctx* ctx) {
+    int rc = ready(ctx);
+    if (rc || (rc = stage_bc(ctx))) return rc;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return check_launch(ctx);
+}
+
+int stm_calculate_f_and_g(stm_ctx* ctx) {
+    int rc = ready(ctx);
+    if (rc || (rc = stage_fg(ctx))) return rc;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return check_launch(ctx);
+}
+
+int stm_calculate_rhs(stm_ctx* ctx) {
+    int rc = ready(ctx);
+    if (rc || (rc = stage_rhs(ctx))) return rc;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return check_launch(ctx);
+}
+
+int stm_solve_pressure(stm_ctx* ctx) {
+    int rc = ready(ctx);
+    if (rc) return rc;
+    // the rhs^2 partials of the criterion come from the RHS pass
+    if ((rc = stage_rhs(ctx)) || (rc = stage_solve(ctx))) return rc;
+    if ((rc = stage_ranges(ctx, 2))) return rc;
+    return read_states(ctx);
+}
+
+int stm_set_u_and_v(stm_ctx* ctx) {
+    int rc = ready(ctx);
+    if (rc || (rc = stage_adap(ctx)) || (rc = stage_ranges(ctx, 0))) return rc;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return check_launch(ctx);
+}
+
+int stm_copy_pressure_to_boundaries(stm_ctx* ctx) {
+    int rc = ready(ctx);
+    if (rc) return rc;
+    LAUNCH("ghost", launch_ghost(ctx->stream, ctx->nbtot, ctx->d_be, ctx->d_cs, ctx->p, -1, 0));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return check_launch(ctx);
+}
+
+int stm_rb_sweeps(stm_ctx* ctx, uint32_t n) {
+    int rc = ready(ctx);
+    if (rc) return rc;
+    // make every channel active for exactly n sweeps
+    if ((rc = read_states(ctx))) return rc;
+    std::vector<ChanState> cs(ctx->h_cs, ctx->h_cs + nch(ctx));
+    std::vector<ChanDesc> cd;
+    for (auto& c : ctx->chans) {
+        ChanDesc d = make_desc(c);
+        d.itermax = (int32_t)n;
+        cd.push_back(d);
+    }
+    for (auto& s : cs) {
+        s.active = 1;
+        s.sor_iters = 0;
+    }
+    HIPCHK(ctx, hipMemcpyAsync(ctx->d_cd, cd.data(), sizeof(ChanDesc) * cd.size(),
+                               hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->d_cs, cs.data(), sizeof(ChanState) * cs.size(),
+                               hipMemcpyHostToDevice, ctx->stream));
+    for (uint32_t s = 0; s < n; s++)
+        if ((rc = rb_sweep(ctx, (int)s))) return rc;
+    for (auto& s : cs) s.active = 0;
+    HIPCHK(ctx, hipMemcpyAsync(ctx->d_cs, cs.data(), sizeof(ChanState) * cs.size(),
+                               hipMemcpyHostToDevice, ctx->stream));
+    if ((rc = upload_descs(ctx))) return rc;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if ((rc = check_launch(ctx))) return rc;
+    return drain(ctx);
+}
+
+int stm_synchronize(stm_ctx* ctx) {
+    if (!ctx) return fail(nullptr, STM_ERR_INVALID_ARG, "null context");
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    int rc = check_launch(ctx);
+    if (rc) return rc;
+    return drain(ctx);
+}
+
+int stm_profile(stm_ctx* ctx, uint32_t sample) {
+    if (!ctx) return fail(nullptr, STM_ERR_INVALID_ARG, "null context");
+    ctx->sample = sample;
+    return STM_OK;
+}
+
+int stm_kernel_stats(stm_ctx* ctx, const char* name, double* total_ms, uint64_t* timed,
+                     uint64_t* launches) {
+    if (!ctx || !name) return fail(ctx, STM_ERR_INVALID_ARG, "null argument");
+    int rc = drain(ctx);
+    if (rc) return rc;
+    auto it = ctx->stats.find(name);
+    const KStat k = it == ctx->stats.end() ? KStat{} : it->second;
+    if (total_ms) *total_ms = k.ms;
+    if (timed) *timed = k.timed;
+    if (launches) *launches = k.launches;
+    return STM_OK;
+}
+
+int stm_reset_stats(stm_ctx* ctx) {
+    if (!ctx) return fail(nullptr, STM_ERR_INVALID_ARG, "null context");
+    int rc = drain(ctx);
+    if (rc) return rc;
+    ctx->stats.clear();
+    return STM_OK;
+}
+
+}  // extern "C"

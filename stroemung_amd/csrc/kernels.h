@@ -1,0 +1,44 @@
+// Host-side launchers of the gfx950 kernels (one translation unit per group).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "stm_internal.h"
+
+namespace stm {
+
+// kernels_step.hip
+void launch_bc_velocity(hipStream_t, int nchan, const ChanDesc*, const BEntry*, double2* restore,
+                        double* u, double* v);
+void launch_fg(hipStream_t, int ntiles, const ChanDesc*, const ChanState*, const Tile*,
+               const uint8_t* flags, const double* u, const double* v, double* f, double* g);
+void launch_rhs(hipStream_t, int ntiles, const ChanDesc*, const ChanState*, const Tile*,
+                const uint8_t* flags, const double* f, const double* g, double* rhs,
+                double* part);
+void launch_adap_uv(hipStream_t, int ntiles, const ChanDesc*, const ChanState*, const Tile*,
+                    const double* f, const double* g, const double* p, double* u, double* v);
+void launch_restore(hipStream_t, int nbtot, const BEntry*, const double2* restore, double* u,
+                    double* v);
+void launch_ranges(hipStream_t, int ntiles, int nchan, const ChanDesc*, ChanState*, const Tile*,
+                   const uint8_t* flags, const double* u, const double* v, const double* p,
+                   double* part, int want_pressure);
+void launch_delt(hipStream_t, int nchan, const ChanDesc*, ChanState*);
+void launch_advance_time(hipStream_t, int nchan, ChanState*);
+void launch_fill(hipStream_t, int64_t n, double* a, double val);
+
+// kernels_pressure.hip
+void launch_ghost(hipStream_t, int nbtot, const BEntry*, const ChanState*, double* p, int colour,
+                  int only_active);
+void launch_rb_half(hipStream_t, int ntiles, const ChanDesc*, const ChanState*, const Tile*,
+                    const uint8_t* flags, const double* rhs, double* p, int colour, int sweep);
+void launch_resid(hipStream_t, int ntiles, const ChanDesc*, const ChanState*, const Tile*,
+                  const uint8_t* flags, const double* rhs, const double* p, double* part);
+void launch_solve_begin(hipStream_t, int nchan, const ChanDesc*, ChanState*, const double* part,
+                        int32_t* active);
+void launch_solve_check(hipStream_t, int nchan, const ChanDesc*, ChanState*, const double* part,
+                        int nsweeps, int32_t* active);
+void launch_initial_norm(hipStream_t, int nchan, const ChanDesc*, ChanState*, const double* rhs,
+                         const double* p, double* scratch);
+void launch_lex_solve(hipStream_t, int nchan, const ChanDesc*, ChanState*, const BEntry*,
+                      const uint8_t* flags, const double* rhs, double* p, double* scratch);
+
+}  // namespace stm

@@ -1,0 +1,422 @@
+// gfx950 kernels for the non-pressure stages of one NaSt2D tick:
+// SETBCOND, COMP_FG, COMP_RHS, ADAP_UV with its restore list, the
+// max|u|,|v| / range reductions and the adaptive time step.
+//
+// Built with -ffp-contract=off: every expression keeps the reference's
+// association order and no a*b+c is fused, so these stages are bit-exact
+// against the Rust reference (which never fuses).  Plain '/' is the IEEE
+// correctly rounded divide on gfx950.
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+#include "stm_internal.h"
+
+namespace stm {
+
+__device__ __forceinline__ double sq(double x) { return x * x; }
+
+// Wall rule for NoSlip / Inflow / moving-wall cells: the ghost tangential
+// velocity.  NoSlip and Inflow negate (src/grid/mod.rs:443, 463, ...); the
+// moving wall reflects about its own velocity.
+__device__ __forceinline__ double reflect(bool moving, double b, double fluid) {
+    return moving ? (2.0 * b) - fluid : -fluid;
+}
+
+// ---------------------------------------------------------------- SETBCOND
+//
+// src/grid/mod.rs:414-651.  The reference walks the (x, y)-sorted boundary
+// list once.  Its reads all hit fluid cells, its writes to fluid cells are
+// constants (v[N] = bv by North-type wall cells, u[W] = bu by West-type wall
+// cells) with a single writer each, and the only read that follows its
+// writer in list order is v[W] of a West / NorthWest cell, written by the wall
+// cell at (x-1, y+1).  So: phase 1 computes every boundary cell's own values
+// and its restore values from the pre-pass fields (substituting that writer's
+// constant for v[W]); phase 2, after a barrier, stores the fluid-face
+// constants.  One workgroup per channel.
+__global__ void __launch_bounds__(256)
+k_bc_velocity(const ChanDesc* __restrict__ cd, const BEntry* __restrict__ be,
+              double2* __restrict__ restore, double* __restrict__ u, double* __restrict__ v)
+{
+    const ChanDesc d = cd[blockIdx.x];
+    for (int k = threadIdx.x; k < d.nb; k += blockDim.x) {
+        const BEntry e = be[d.boff + k];
+        const int64_t i = e.gidx, P = e.pitch;
+        const int64_t N = i - 1, S = i + 1, E = i + P, W = i - P;
+        const double ub = u[i], vb = v[i];
+        double nu = ub, nv = vb, ru, rv;
+        const uint8_t ed = e.edge;
+        if (ed == E_NONE) {
+            ru = ub;
+            rv = vb;
+        } else if (e.type == T_OUTFLOW) {  // src/grid/mod.rs:489-536
+            const double vw = e.vw_rule ? e.vw_const : (ed == E_W || ed == E_NW ? v[W] : 0.0);
+            switch (ed) {
+            case E_N:  nu = u[N]; nv = v[N]; break;
+            case E_NE: nu = u[N]; nv = v[E]; break;
+            case E_E:  nu = u[E]; nv = v[E]; break;
+            case E_SE: nu = u[E]; nv = v[S]; break;
+            case E_S:  nu = u[S]; nv = v[S]; break;
+            case E_SW: nu = u[W]; nv = v[S]; break;
+            case E_W:  nu = u[W]; nv = vw; break;
+            case E_NW: nu = u[N]; nv = vw; break;
+            }
+            // src/grid/mod.rs:595-648; an outflow cell writes no fluid face
+            const bool wt = ed == E_SW || ed == E_W || ed == E_NW;
+            const bool nt = ed == E_N || ed == E_NE || ed == E_NW;
+            ru = wt ? u[W] : nu;
+            rv = nt ? v[N] : nv;
+        } else {  // NoSlip / Inflow (src/grid/mod.rs:437-488, 537-586), moving wall
+            const bool mv = e.type == T_MOVING;
+            const double bu = e.bu, bv = e.bv;
+            const double vw = e.vw_rule ? e.vw_const : (ed == E_W || ed == E_NW ? v[W] : 0.0);
+            switch (ed) {
+            case E_N:  nu = reflect(mv, bu, u[N]); break;
+            case E_NE: nu = bu; nv = reflect(mv, bv, v[E]); break;
+            case E_E:  nu = bu; nv = reflect(mv, bv, v[E]); break;
+            case E_SE: nu = bu; nv = bv; break;
+            case E_S:  nu = reflect(mv, bu, u[S]); nv = bv; break;
+            case E_SW: nu = reflect(mv, bu, u[S]); nv = bv; break;
+            case E_W:  nv = reflect(mv, bv, vw); break;
+            case E_NW: nu = reflect(mv, bu, u[N]); nv = reflect(mv, bv, vw); break;
+            }
+            const bool wt = ed == E_SW || ed == E_W || ed == E_NW;
+            const bool nt = ed == E_N || ed == E_NE || ed == E_NW;
+            ru = wt ? bu : nu;
+            rv = nt ? bv : nv;
+        }
+        u[i] = nu;
+        v[i] = nv;
+        restore[d.boff + k] = make_double2(ru, rv);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < d.nb; k += blockDim.x) {
+        const BEntry e = be[d.boff + k];
+        const uint8_t ed = e.edge;
+        if (ed == E_NONE || e.type == T_OUTFLOW) continue;
+        if (ed == E_N || ed == E_NE || ed == E_NW) v[e.gidx - 1] = e.bv;
+        if (ed == E_SW || ed == E_W || ed == E_NW) u[e.gidx - e.pitch] = e.bu;
+    }
+}
+
+// ---------------------------------------------------------------- COMP_FG
+
+// src/math.rs:162-174
+__device__ __forceinline__ double laplacian(double c, double n, double s, double e, double w,
+                                            double delx, double dely) {
+    const double d2edx2 = ((e - (2. * c)) + w) / sq(delx);
+    const double d2edy2 = ((s - (2. * c)) + n) / sq(dely);
+    return d2edx2 + d2edy2;
+}
+
+// F and G of one interior cell (src/simulation.rs:349-392 with
+// src/math.rs:19-150).  ne = (x+1, y-1), sw = (x-1, y+1).
+__device__ __forceinline__ void fg_cell(
+    double uc, double un, double us, double ue, double uw, double usw,
+    double vc, double vn, double vs, double ve, double vw, double vne,
+    double delx, double dely, double delt, double gamma, double re, double& F, double& G)
+{
+    // du2dx, src/math.rs:19-33
+    double du2dx;
+    {
+        const double l1 = sq(uc + ue), r1 = sq(uw + uc);
+        const double left = l1 - r1;
+        const double l2 = fabs(uc + ue) * (uc - ue);
+        const double r2 = fabs(uw + uc) * (uw - uc);
+        du2dx = (left + (gamma * (l2 - r2))) / (4.0 * delx);
+    }
+    // duvdy, src/math.rs:97-120
+    double duvdy;
+    {
+        const double l1 = (vc + ve) * (uc + us);
+        const double r1 = (vn + vne) * (un + uc);
+        const double left = l1 - r1;
+        const double l2 = fabs(vc + ve) * (uc - us);
+        const double r2 = fabs(vn + vne) * (un - uc);
+        duvdy = (left + (gamma * (l2 - r2))) / (4.0 * dely);
+    }
+    // duvdx, src/math.rs:53-77
+    double duvdx;
+    {
+        const double l1 = (uc + us) * (vc + ve);
+        const double r1 = (uw + usw) * (vw + vc);
+        const double left = l1 - r1;
+        const double l2 = fabs(uc + us) * (vc - ve);
+        const double r2 = fabs(uw + usw) * (vw - vc);
+        duvdx = (left + (gamma * (l2 - r2))) / (4.0 * delx);
+    }
+    // dv2dy, src/math.rs:136-150
+    double dv2dy;
+    {
+        const double l1 = sq(vc + vs), r1 = sq(vn + vc);
+        const double left = l1 - r1;
+        const double l2 = fabs(vc + vs) * (vc - vs);
+        const double r2 = fabs(vn + vc) * (vn - vc);
+        dv2dy = (left + (gamma * (l2 - r2))) / (4.0 * dely);
+    }
+    const double lu = laplacian(uc, un, us, ue, uw, delx, dely);
+    const double lv = laplacian(vc, vn, vs, ve, vw, delx, dely);
+    F = uc + (delt * (((lu / re) - du2dx) - duvdy));
+    G = vc + (delt * (((lv / re) - duvdx) - dv2dy));
+}
+
+// src/simulation.rs:122-202.  F, G on interior cells; then f = u, g = v on
+// boundary cells, and on fluid cells whose east (south) neighbour is a
+// boundary, f = u (g = v) -- the West-type (North-type) restore of
+// src/simulation.rs:167-201.  Fluid cells on the outer ring keep their f, g.
+__global__ void __launch_bounds__(TILE_THREADS)
+k_fg(const ChanDesc* __restrict__ cd, const ChanState* __restrict__ cs,
+     const Tile* __restrict__ tiles, const uint8_t* __restrict__ flags,
+     const double* __restrict__ u, const double* __restrict__ v,
+     double* __restrict__ f, double* __restrict__ g)
+{
+    const Tile t = tiles[blockIdx.x];
+    const ChanDesc& d = cd[t.chan];
+    const int x = t.x0 + threadIdx.y, y = t.y0 + threadIdx.x;
+    if (x >= d.nx || y >= d.ny) return;
+    const int64_t P = d.pitch, i = d.off + x * P + y;
+    const uint8_t fl = flags[i];
+    if (!flag_fluid(fl)) {
+        f[i] = u[i];
+        g[i] = v[i];
+        return;
+    }
+    double F = 0.0, G = 0.0;
+    bool wf = false, wg = false;
+    if (x >= 1 && x <= d.nx - 2 && y >= 1 && y <= d.ny - 2) {
+        fg_cell(u[i], u[i - 1], u[i + 1], u[i + P], u[i - P], u[i - P + 1],
+                v[i], v[i - 1], v[i + 1], v[i + P], v[i - P], v[i + P - 1],
+                d.delx, d.dely, cs[t.chan].delt, d.gamma, d.reynolds, F, G);
+        wf = wg = true;
+    }
+    if (x + 1 < d.nx && !flag_fluid(flags[i + P])) { F = u[i]; wf = true; }
+    if (y + 1 < d.ny && !flag_fluid(flags[i + 1])) { G = v[i]; wg = true; }
+    if (wf) f[i] = F;
+    if (wg) g[i] = G;
+}
+
+// Deterministic block sum: fixed LDS tree over TILE_THREADS values.
+__device__ __forceinline__ double block_sum(double val, double* sh) {
+    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+    sh[tid] = val;
+    __syncthreads();
+    for (int s = TILE_THREADS / 2; s > 0; s >>= 1) {
+        if (tid < s) sh[tid] = sh[tid] + sh[tid + s];
+        __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// src/simulation.rs:204-214: rhs on x in [1, nx-1], y in [1, ny-1].  Also the
+// per-tile sum of rhs^2 over interior fluid cells (red-black criterion).
+__global__ void __launch_bounds__(TILE_THREADS)
+k_rhs(const ChanDesc* __restrict__ cd, const ChanState* __restrict__ cs,
+      const Tile* __restrict__ tiles, const uint8_t* __restrict__ flags,
+      const double* __restrict__ f, const double* __restrict__ g, double* __restrict__ rhs,
+      double* __restrict__ part)
+{
+    __shared__ double sh[TILE_THREADS];
+    const Tile t = tiles[blockIdx.x];
+    const ChanDesc& d = cd[t.chan];
+    const int x = t.x0 + threadIdx.y, y = t.y0 + threadIdx.x;
+    double r2 = 0.0;
+    if (x >= 1 && x < d.nx && y >= 1 && y < d.ny) {
+        const int64_t P = d.pitch, i = d.off + x * P + y;
+        const double r = (((f[i] - f[i - P]) / d.delx) + ((g[i] - g[i - 1]) / d.dely))
+                         / cs[t.chan].delt;
+        rhs[i] = r;
+        if (x <= d.nx - 2 && y <= d.ny - 2 && flag_fluid(flags[i])) r2 = r * r;
+    }
+    const double s = block_sum(r2, sh);
+    if (threadIdx.x == 0 && threadIdx.y == 0) part[blockIdx.x * NPART + 0] = s;
+}
+
+// ---------------------------------------------------------------- ADAP_UV
+
+// src/simulation.rs:287-311 on x in [0, nx-2], y in [0, ny-2].
+__global__ void __launch_bounds__(TILE_THREADS)
+k_adap_uv(const ChanDesc* __restrict__ cd, const ChanState* __restrict__ cs,
+          const Tile* __restrict__ tiles, const double* __restrict__ f,
+          const double* __restrict__ g, const double* __restrict__ p,
+          double* __restrict__ u, double* __restrict__ v)
+{
+    const Tile t = tiles[blockIdx.x];
+    const ChanDesc& d = cd[t.chan];
+    const int x = t.x0 + threadIdx.y, y = t.y0 + threadIdx.x;
+    if (x > d.nx - 2 || y > d.ny - 2) return;
+    const int64_t P = d.pitch, i = d.off + x * P + y;
+    const double delt = cs[t.chan].delt;
+    const double pc = p[i];
+    u[i] = f[i] - ((delt / d.delx) * (p[i + P] - pc));
+    v[i] = g[i] - ((delt / d.dely) * (p[i + 1] - pc));
+}
+
+// src/simulation.rs:313-320: the net effect of the restore list, per cell.
+__global__ void __launch_bounds__(256)
+k_restore(int nbtot, const BEntry* __restrict__ be, const double2* __restrict__ restore,
+          double* __restrict__ u, double* __restrict__ v)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nbtot) return;
+    const double2 r = restore[k];
+    const int64_t i = be[k].gidx;
+    u[i] = r.x;
+    v[i] = r.y;
+}
+
+// ---------------------------------------------------------------- reductions
+
+// Per-tile partials over fluid cells: max|u|, max|v| over fluid faces, the
+// min / max of u^2+v^2 (src/grid/mod.rs:253-268) and of p (237-251).
+__global__ void __launch_bounds__(TILE_THREADS)
+k_ranges(const ChanDesc* __restrict__ cd, const Tile* __restrict__ tiles,
+         const uint8_t* __restrict__ flags, const double* __restrict__ u,
+         const double* __restrict__ v, const double* __restrict__ p, double* __restrict__ part)
+{
+    __shared__ double sh[6][TILE_THREADS];
+    const Tile t = tiles[blockIdx.x];
+    const ChanDesc& d = cd[t.chan];
+    const int x = t.x0 + threadIdx.y, y = t.y0 + threadIdx.x;
+    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+    double um = 0.0, vm = 0.0, smin = __DBL_MAX__, smax = 0.0, pmin = __DBL_MAX__, pmax = 0.0;
+    if (x < d.nx && y < d.ny) {
+        const int64_t P = d.pitch, i = d.off + x * P + y;
+        if (flag_fluid(flags[i])) {
+            const double uc = u[i], vc = v[i];
+            um = fabs(uc);
+            vm = fabs(vc);
+            if (x > 0 && !flag_fluid(flags[i - P])) { const double a = fabs(u[i - P]); um = a > um ? a : um; }
+            if (y > 0 && !flag_fluid(flags[i - 1])) { const double a = fabs(v[i - 1]); vm = a > vm ? a : vm; }
+            const double sp = sq(uc) + sq(vc);
+            smin = fmin(__DBL_MAX__, sp);
+            smax = fmax(0.0, sp);
+            pmin = fmin(__DBL_MAX__, p[i]);
+            pmax = fmax(0.0, p[i]);
+        }
+    }
+    sh[0][tid] = um; sh[1][tid] = vm; sh[2][tid] = smin;
+    sh[3][tid] = smax; sh[4][tid] = pmin; sh[5][tid] = pmax;
+    __syncthreads();
+    for (int s = TILE_THREADS / 2; s > 0; s >>= 1) {
+        if (tid < s) {
+            double a, b;
+            a = sh[0][tid]; b = sh[0][tid + s]; sh[0][tid] = b > a ? b : a;
+            a = sh[1][tid]; b = sh[1][tid + s]; sh[1][tid] = b > a ? b : a;
+            sh[2][tid] = fmin(sh[2][tid], sh[2][tid + s]);
+            sh[3][tid] = fmax(sh[3][tid], sh[3][tid + s]);
+            sh[4][tid] = fmin(sh[4][tid], sh[4][tid + s]);
+            sh[5][tid] = fmax(sh[5][tid], sh[5][tid + s]);
+        }
+        __syncthreads();
+    }
+    if (tid == 0)
+        for (int k = 0; k < 6; k++) part[blockIdx.x * NPART + k] = sh[k][0];
+}
+
+// One thread per channel: fold the range partials of its tiles.
+// want_pressure: 1 = always (construction), 2 = only if the solve hit
+// itermax (src/simulation.rs:283), 0 = never.
+__global__ void k_ranges_reduce(int nchan, const ChanDesc* __restrict__ cd,
+                                ChanState* __restrict__ cs, const double* __restrict__ part,
+                                int want_pressure)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nchan) return;
+    const ChanDesc& d = cd[c];
+    double um = 0.0, vm = 0.0, smin = __DBL_MAX__, smax = 0.0, pmin = __DBL_MAX__, pmax = 0.0;
+    for (int k = d.tile0; k < d.tile0 + d.ntiles; k++) {
+        const double* q = part + (int64_t)k * NPART;
+        um = q[0] > um ? q[0] : um;
+        vm = q[1] > vm ? q[1] : vm;
+        smin = fmin(smin, q[2]);
+        smax = fmax(smax, q[3]);
+        pmin = fmin(pmin, q[4]);
+        pmax = fmax(pmax, q[5]);
+    }
+    ChanState& s = cs[c];
+    s.umax = um;
+    s.vmax = vm;
+    s.speed_range[0] = sqrt(smin);
+    s.speed_range[1] = sqrt(smax);
+    if (want_pressure == 1 || (want_pressure == 2 && s.exhausted)) {
+        s.pressure_range[0] = pmin;
+        s.pressure_range[1] = pmax;
+    }
+}
+
+// Adaptive time step (Griebel, Dornseifer, Neunhoeffer: NaSt2D COMP_DELT):
+// delt = tau * min(Re/2 / (1/dx^2 + 1/dy^2), dx/|u|max, dy/|v|max).
+__global__ void k_delt(int nchan, const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nchan) return;
+    const ChanDesc& d = cd[c];
+    if (!(d.tau > 0.0)) return;
+    ChanState& s = cs[c];
+    double dt = (d.reynolds / 2.0) / ((1.0 / sq(d.delx)) + (1.0 / sq(d.dely)));
+    if (s.umax > 0.0) { const double t = d.delx / s.umax; if (t < dt) dt = t; }
+    if (s.vmax > 0.0) { const double t = d.dely / s.vmax; if (t < dt) dt = t; }
+    s.delt = d.tau * dt;
+}
+
+// src/simulation.rs:330-331
+__global__ void k_advance_time(int nchan, ChanState* __restrict__ cs)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nchan) return;
+    cs[c].time = cs[c].time + cs[c].delt;
+    cs[c].iterations += 1;
+}
+
+__global__ void __launch_bounds__(256)
+k_fill(int64_t n, double* __restrict__ a, double val)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) a[k] = val;
+}
+
+// ---------------------------------------------------------------- launchers
+
+#define TB dim3(TILE_Y, TILE_X)
+
+void launch_bc_velocity(hipStream_t st, int nchan, const ChanDesc* cd, const BEntry* be,
+                        double2* restore, double* u, double* v) {
+    k_bc_velocity<<<nchan, 256, 0, st>>>(cd, be, restore, u, v);
+}
+void launch_fg(hipStream_t st, int ntiles, const ChanDesc* cd, const ChanState* cs,
+               const Tile* tiles, const uint8_t* flags, const double* u, const double* v,
+               double* f, double* g) {
+    k_fg<<<ntiles, TB, 0, st>>>(cd, cs, tiles, flags, u, v, f, g);
+}
+void launch_rhs(hipStream_t st, int ntiles, const ChanDesc* cd, const ChanState* cs,
+                const Tile* tiles, const uint8_t* flags, const double* f, const double* g,
+                double* rhs, double* part) {
+    k_rhs<<<ntiles, TB, 0, st>>>(cd, cs, tiles, flags, f, g, rhs, part);
+}
+void launch_adap_uv(hipStream_t st, int ntiles, const ChanDesc* cd, const ChanState* cs,
+                    const Tile* tiles, const double* f, const double* g, const double* p,
+                    double* u, double* v) {
+    k_adap_uv<<<ntiles, TB, 0, st>>>(cd, cs, tiles, f, g, p, u, v);
+}
+void launch_restore(hipStream_t st, int nbtot, const BEntry* be, const double2* restore,
+                    double* u, double* v) {
+    if (nbtot > 0) k_restore<<<(nbtot + 255) / 256, 256, 0, st>>>(nbtot, be, restore, u, v);
+}
+void launch_ranges(hipStream_t st, int ntiles, int nchan, const ChanDesc* cd, ChanState* cs,
+                   const Tile* tiles, const uint8_t* flags, const double* u, const double* v,
+                   const double* p, double* part, int want_pressure) {
+    k_ranges<<<ntiles, TB, 0, st>>>(cd, tiles, flags, u, v, p, part);
+    k_ranges_reduce<<<(nchan + 63) / 64, 64, 0, st>>>(nchan, cd, cs, part, want_pressure);
+}
+void launch_delt(hipStream_t st, int nchan, const ChanDesc* cd, ChanState* cs) {
+    k_delt<<<(nchan + 63) / 64, 64, 0, st>>>(nchan, cd, cs);
+}
+void launch_advance_time(hipStream_t st, int nchan, ChanState* cs) {
+    k_advance_time<<<(nchan + 63) / 64, 64, 0, st>>>(nchan, cs);
+}
+void launch_fill(hipStream_t st, int64_t n, double* a, double val) {
+    if (n > 0) k_fill<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, a, val);
+}
+
+}  // namespace stm

@@ -38,7 +38,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=1024, help="interior cells per side")
     ap.add_argument("--eps", type=float, default=1e-6)
-    ap.add_argument("--check-every", type=int, default=16)
+    ap.add_argument("--check-every", type=int, default=0,
+                    help="sweeps per blocked launch (1/2/4/8); 0 = autotuned")
     ap.add_argument("--sample", type=int, default=32, help="time every n-th launch")
     ap.add_argument("--cpu-sweeps", type=int, default=150)
     ap.add_argument("--no-cpu", action="store_true")
@@ -104,6 +105,8 @@ def main():
     ctx.construct()
     for _ in range(a.warmup):
         ctx.tick()
+    tuned = ctx.autotune(8)
+    variant, ksweeps, tile_x, tile_y = ctx.solver_variant()
     ctx.synchronize()
     ctx.reset_stats()
     ctx.profile(a.sample)
@@ -120,14 +123,15 @@ def main():
     cells = n * n
     value = cells * a.steps / elapsed
 
-    kname = "rb_half"
+    kname = "rb_tb"
     ms, timed, launches = ctx.kernel_stats(kname)
     avg_s = ms / 1e3 / max(timed, 1)
-    bytes_per_launch = SWEEP_BYTES_PER_CELL / 2.0 * cells
+    # one launch advances every interior cell by ksweeps red-black sweeps
+    bytes_per_launch = SWEEP_BYTES_PER_CELL * ksweeps * cells
     achieved = bytes_per_launch / avg_s / 1e9 if timed else None
     kernels = {}
-    for k in ("rb_half", "ghost", "resid", "solve_check", "fg", "rhs", "adap_uv",
-              "bc_velocity", "ranges"):
+    for k in ("rb_tb", "pbuf_fix", "solve_begin", "fg", "rhs", "adap_uv", "restore",
+              "bc_velocity", "ranges", "delt"):
         kms, kt, kl = ctx.kernel_stats(k)
         if kl:
             kernels[k] = {"launches": kl, "timed": kt,
@@ -150,7 +154,8 @@ def main():
             "workload": "lid-driven cavity %dx%d, Re=1000, tau=0.5, gamma=0.9, red-black SOR "
                         "omega=1.8 to relative residual %g, fp64, zero initial fields" % (n, n, a.eps),
             "solver": "rb",
-            "check_every": a.check_every,
+            "variant": {"id": variant, "sweeps_per_launch": ksweeps, "tile": [tile_x, tile_y],
+                        "autotune_us_per_sweep": tuned[1]},
             "sweeps_per_step": sweeps_per_step,
             "final_delt": st["delt"],
             "final_time": st["time"],

@@ -164,6 +164,18 @@ int stm_copy_pressure_to_boundaries(stm_ctx *ctx); /* src/grid/mod.rs:343-412 */
  * with no residual test (no reference counterpart). */
 int stm_rb_sweeps(stm_ctx *ctx, uint32_t n);
 
+/* ---- red-black solver configuration (no reference counterpart)
+ * The red-black solve runs as temporally blocked launches of K sweeps each,
+ * with the residual checked after every launch; the variant fixes K and the
+ * output tile.  params.check_every = K (1, 2, 4, 8) restricts the choice to
+ * that K; check_every <= 0 lets the tuner pick any variant. */
+/* variant >= 0 selects it; -1 only queries.  Reports the variant in use. */
+int stm_solver_variant(stm_ctx *ctx, int32_t variant, int32_t *current, int32_t *sweeps,
+                       int32_t *tile_x, int32_t *tile_y);
+/* Times `launches` launches of each permitted variant on the current fields
+ * (restored afterwards) and keeps the fastest per sweep. */
+int stm_autotune(stm_ctx *ctx, uint32_t launches, int32_t *chosen, double *us_per_sweep);
+
 /* ---- measurement (no reference counterpart) */
 int stm_synchronize(stm_ctx *ctx);
 /* Time every `sample`-th launch of each kernel with HIP events on the

@@ -85,6 +85,8 @@ SIGNATURES = {
     "stm_set_u_and_v": (C.c_int, [_CTX]),
     "stm_copy_pressure_to_boundaries": (C.c_int, [_CTX]),
     "stm_rb_sweeps": (C.c_int, [_CTX, C.c_uint32]),
+    "stm_solver_variant": (C.c_int, [_CTX, C.c_int32] + [C.POINTER(C.c_int32)] * 4),
+    "stm_autotune": (C.c_int, [_CTX, C.c_uint32, C.POINTER(C.c_int32), _D]),
     "stm_synchronize": (C.c_int, [_CTX]),
     "stm_profile": (C.c_int, [_CTX, C.c_uint32]),
     "stm_kernel_stats": (C.c_int, [_CTX, C.c_char_p, _D, C.POINTER(C.c_uint64),

@@ -36,6 +36,7 @@ struct HostChan {
     std::vector<uint32_t> blist;
     double fluid_cells = 0.0;
     int32_t tile0 = 0, ntiles = 0, boff = 0;
+    int32_t tb0 = 0, ntb = 0;      // blocked-solver tiles of the current variant
 };
 
 struct KStat {
@@ -61,6 +62,7 @@ struct stm_ctx {
     bool any_rb = false, any_lex = false;
     double *u = nullptr, *v = nullptr, *p = nullptr, *f = nullptr, *g = nullptr, *rhs = nullptr;
     double* scratch = nullptr;
+    double* p2 = nullptr;          // second pressure buffer of the blocked solver
     uint8_t* flags = nullptr;
     ChanDesc* d_cd = nullptr;
     ChanState* d_cs = nullptr;
@@ -70,6 +72,10 @@ struct stm_ctx {
     double* d_part = nullptr;
     int32_t* d_active = nullptr;
     int32_t* h_active = nullptr;   // pinned
+    int32_t* d_counters = nullptr; // per-channel workgroup tickets
+    TbTile* d_tb = nullptr;
+    int ntb = 0, tb_capacity = 0, part_capacity = 0;
+    int variant = -1;              // blocked-solver variant in use
     ChanState* h_cs = nullptr;     // pinned
     int be_capacity = 0;
     // profiling
@@ -193,6 +199,8 @@ ChanDesc make_desc(const HostChan& c) {
     d.ntiles = c.ntiles;
     d.itermax = (int32_t)q.max_iterations;
     d.check_every = q.check_every > 0 ? q.check_every : 1;
+    d.tb0 = c.tb0;
+    d.ntb = c.ntb;
     d.delx = q.delx;
     d.dely = q.dely;
     d.gamma = q.gamma;
@@ -285,6 +293,67 @@ int upload_descs(stm_ctx* ctx) {
     return STM_OK;
 }
 
+// Split n cells into ceil(n / tmax) near-equal consecutive parts.
+std::vector<std::pair<int, int>> split(int n, int tmax) {
+    const int parts = (n + tmax - 1) / tmax;
+    std::vector<std::pair<int, int>> out;
+    int start = 0;
+    for (int k = 0; k < parts; k++) {
+        const int len = n / parts + (k < n % parts ? 1 : 0);
+        out.emplace_back(start, len);
+        start += len;
+    }
+    return out;
+}
+
+int count_tb(stm_ctx* ctx, int variant) {
+    const TbVariant& v = TB_VARIANTS[variant];
+    int n = 0;
+    for (auto& c : ctx->chans)
+        n += (int)(split((int)c.nx, v.oxm).size() * split((int)c.ny, v.oym).size());
+    return n;
+}
+
+// Output tiles of the blocked solver for `variant`, channel by channel.
+int plan_tb(stm_ctx* ctx, int variant) {
+    const TbVariant& v = TB_VARIANTS[variant];
+    std::vector<TbTile> tl;
+    for (size_t k = 0; k < ctx->chans.size(); k++) {
+        HostChan& c = ctx->chans[k];
+        c.tb0 = (int32_t)tl.size();
+        for (auto xs : split((int)c.nx, v.oxm))
+            for (auto ys : split((int)c.ny, v.oym))
+                tl.push_back(TbTile{(int32_t)k, xs.first, ys.first, xs.second, ys.second, 0, 0, 0});
+        c.ntb = (int32_t)tl.size() - c.tb0;
+    }
+    if ((int)tl.size() > ctx->tb_capacity) {
+        if (ctx->d_tb) HIPCHK(ctx, hipFree(ctx->d_tb));
+        ctx->d_tb = nullptr;
+        HIPCHK(ctx, hipMalloc(&ctx->d_tb, sizeof(TbTile) * tl.size()));
+        ctx->tb_capacity = (int)tl.size();
+    }
+    HIPCHK(ctx, hipMemcpyAsync(ctx->d_tb, tl.data(), sizeof(TbTile) * tl.size(),
+                               hipMemcpyHostToDevice, ctx->stream));
+    ctx->ntb = (int)tl.size();
+    ctx->variant = variant;
+    return upload_descs(ctx);
+}
+
+// The variant for the requested check interval (the sweeps per launch):
+// check_every = K picks the K-sweep variant with 64x64 tiles (48x48 for 8);
+// check_every <= 0 leaves the choice to the tuner, starting from K = 4.
+int default_variant(stm_ctx* ctx) {
+    const int want = ctx->chans.empty() ? 4 : ctx->chans[0].prm.check_every;
+    for (int v = 0; v < N_TB_VARIANTS; v++)
+        if (TB_VARIANTS[v].k == (want > 0 ? want : 4)) return v;
+    return 2;
+}
+
+bool variant_allowed(stm_ctx* ctx, int v) {
+    const int want = ctx->chans[0].prm.check_every;
+    return want <= 0 || TB_VARIANTS[v].k == want;
+}
+
 int finalize(stm_ctx* ctx) {
     if (ctx->finalized) return STM_OK;
     if (ctx->chans.empty()) return fail(ctx, STM_ERR_STATE, "no grid added");
@@ -309,7 +378,8 @@ int finalize(stm_ctx* ctx) {
     ctx->total = off;
     ctx->ntiles = tiles;
     const size_t bytes = sizeof(double) * (size_t)off;
-    double** fields[] = {&ctx->u, &ctx->v, &ctx->p, &ctx->f, &ctx->g, &ctx->rhs, &ctx->scratch};
+    double** fields[] = {&ctx->u, &ctx->v, &ctx->p, &ctx->f, &ctx->g, &ctx->rhs, &ctx->scratch,
+                         &ctx->p2};
     for (double** fp : fields) {
         HIPCHK(ctx, hipMalloc(fp, bytes));
         HIPCHK(ctx, hipMemsetAsync(*fp, 0, bytes, ctx->stream));
@@ -320,8 +390,14 @@ int finalize(stm_ctx* ctx) {
     HIPCHK(ctx, hipMalloc(&ctx->d_cd, sizeof(ChanDesc) * nch));
     HIPCHK(ctx, hipMalloc(&ctx->d_cs, sizeof(ChanState) * nch));
     HIPCHK(ctx, hipMalloc(&ctx->d_tiles, sizeof(Tile) * tl.size()));
-    HIPCHK(ctx, hipMalloc(&ctx->d_part, sizeof(double) * NPART * tl.size()));
+    size_t maxtiles = tl.size();
+    for (int v = 0; v < N_TB_VARIANTS; v++)
+        maxtiles = std::max(maxtiles, (size_t)count_tb(ctx, v));
+    HIPCHK(ctx, hipMalloc(&ctx->d_part, sizeof(double) * NPART * maxtiles));
+    ctx->part_capacity = (int)maxtiles;
     HIPCHK(ctx, hipMalloc(&ctx->d_active, sizeof(int32_t) * nch));
+    HIPCHK(ctx, hipMalloc(&ctx->d_counters, sizeof(int32_t) * nch));
+    HIPCHK(ctx, hipMemsetAsync(ctx->d_counters, 0, sizeof(int32_t) * nch, ctx->stream));
     HIPCHK(ctx, hipHostMalloc(&ctx->h_active, sizeof(int32_t) * nch, hipHostMallocDefault));
     HIPCHK(ctx, hipHostMalloc(&ctx->h_cs, sizeof(ChanState) * nch, hipHostMallocDefault));
     HIPCHK(ctx, hipMemcpyAsync(ctx->d_tiles, tl.data(), sizeof(Tile) * tl.size(),
@@ -330,7 +406,7 @@ int finalize(stm_ctx* ctx) {
     if (rc) return rc;
     for (auto& c : ctx->chans)
         if ((rc = upload_flags(ctx, c))) return rc;
-    if ((rc = upload_descs(ctx))) return rc;
+    if ((rc = plan_tb(ctx, default_variant(ctx)))) return rc;
     std::vector<ChanState> cs(nch);
     for (size_t k = 0; k < nch; k++) {
         const stm_params& q = ctx->chans[k].prm;
@@ -440,23 +516,6 @@ int stage_rhs(stm_ctx* ctx) {
     return STM_OK;
 }
 
-int rb_sweep(stm_ctx* ctx, int s) {
-    LAUNCH("ghost", launch_ghost(ctx->stream, ctx->nbtot, ctx->d_be, ctx->d_cs, ctx->p, -1, 1));
-    for (int colour = 0; colour < 2; colour++)
-        LAUNCH("rb_half", launch_rb_half(ctx->stream, ctx->ntiles, ctx->d_cd, ctx->d_cs,
-                                         ctx->d_tiles, ctx->flags, ctx->rhs, ctx->p, colour, s));
-    return STM_OK;
-}
-
-int rb_check(stm_ctx* ctx, int nsweeps) {
-    LAUNCH("ghost", launch_ghost(ctx->stream, ctx->nbtot, ctx->d_be, ctx->d_cs, ctx->p, -1, 1));
-    LAUNCH("resid", launch_resid(ctx->stream, ctx->ntiles, ctx->d_cd, ctx->d_cs, ctx->d_tiles,
-                                 ctx->flags, ctx->rhs, ctx->p, ctx->d_part));
-    LAUNCH("solve_check", launch_solve_check(ctx->stream, nch(ctx), ctx->d_cd, ctx->d_cs,
-                                             ctx->d_part, nsweeps, ctx->d_active));
-    return STM_OK;
-}
-
 // Reads the per-channel "still running" flags; *any = some channel runs.
 int poll_active(stm_ctx* ctx, bool* any) {
     HIPCHK(ctx, hipMemcpyAsync(ctx->h_active, ctx->d_active, sizeof(int32_t) * nch(ctx),
@@ -466,6 +525,29 @@ int poll_active(stm_ctx* ctx, bool* any) {
     if (rc) return rc;
     *any = false;
     for (int k = 0; k < nch(ctx); k++) *any = *any || ctx->h_active[k];
+    return STM_OK;
+}
+
+// Blocked red-black launches until no channel is active.  Launches are
+// queued in growing batches between polls of the per-channel flags; a
+// launch finding its channel stopped returns at once.  check = 0 runs the
+// sweeps without the residual (fixed sweep counts).
+int blocked_solve(stm_ctx* ctx, int check) {
+    int batch = 2;
+    for (;;) {
+        for (int b = 0; b < batch; b++)
+            LAUNCH("rb_tb", launch_rb_tb(ctx->stream, ctx->variant, ctx->ntb, ctx->d_cd,
+                                         ctx->d_cs, ctx->d_tb, ctx->flags, ctx->rhs, ctx->p,
+                                         ctx->p2, ctx->d_part, ctx->d_counters, ctx->d_active,
+                                         check));
+        bool any = false;
+        int rc = poll_active(ctx, &any);
+        if (rc) return rc;
+        if (!any) break;
+        if (batch < 64) batch *= 2;
+    }
+    LAUNCH("pbuf_fix", launch_pbuf_fix(ctx->stream, ctx->ntiles, nch(ctx), ctx->d_cd, ctx->d_cs,
+                                       ctx->d_tiles, ctx->p2, ctx->p));
     return STM_OK;
 }
 
@@ -479,24 +561,7 @@ int stage_solve(stm_ctx* ctx) {
                                              ctx->d_be, ctx->flags, ctx->rhs, ctx->p,
                                              ctx->scratch));
     if (!ctx->any_rb) return STM_OK;
-    const int every = ctx->check_every;
-    int batch = 1;     // check intervals queued between polls; grows to 32
-    for (;;) {
-        for (int b = 0; b < batch; b++) {
-            for (int s = 0; s < every; s++) {
-                int rc = rb_sweep(ctx, s);
-                if (rc) return rc;
-            }
-            int rc = rb_check(ctx, every);
-            if (rc) return rc;
-        }
-        bool any = false;
-        int rc = poll_active(ctx, &any);
-        if (rc) return rc;
-        if (!any) break;
-        if (batch < 32) batch *= 2;
-    }
-    return STM_OK;
+    return blocked_solve(ctx, 1);
 }
 
 int stage_adap(stm_ctx* ctx) {
@@ -548,6 +613,13 @@ int check_params(stm_ctx* ctx, const stm_params* q) {
         return fail(ctx, STM_ERR_INVALID_ARG, "cell size must be positive");
     if (q->solver != STM_SOLVER_LEX && q->solver != STM_SOLVER_RB)
         return fail(ctx, STM_ERR_INVALID_ARG, "unknown solver %d", q->solver);
+    if (q->check_every > 0) {
+        bool ok = false;
+        for (int v = 0; v < N_TB_VARIANTS; v++) ok = ok || TB_VARIANTS[v].k == q->check_every;
+        if (!ok)
+            return fail(ctx, STM_ERR_INVALID_ARG,
+                        "check_every must be 1, 2, 4 or 8 (or <= 0 to let the tuner pick)");
+    }
     return STM_OK;
 }
 
@@ -837,35 +909,132 @@ int stm_copy_pressure_to_boundaries(stm_ctx* ctx) {
     return check_launch(ctx);
 }
 
-int stm_rb_sweeps(stm_ctx* ctx, uint32_t n) {
-    int rc = ready(ctx);
+// Sets every red-black channel running with itermax = n (or restores the
+// descriptors when n == 0) and zero sweeps done; tol2 < 0 never converges.
+int arm_channels(stm_ctx* ctx, int itermax_override, bool never_converge,
+                 std::vector<ChanState>* saved) {
+    int rc = read_states(ctx);
     if (rc) return rc;
-    // make every channel active for exactly n sweeps
-    if ((rc = read_states(ctx))) return rc;
+    if (saved) saved->assign(ctx->h_cs, ctx->h_cs + nch(ctx));
     std::vector<ChanState> cs(ctx->h_cs, ctx->h_cs + nch(ctx));
     std::vector<ChanDesc> cd;
-    for (auto& c : ctx->chans) {
-        ChanDesc d = make_desc(c);
-        d.itermax = (int32_t)n;
+    for (size_t k = 0; k < ctx->chans.size(); k++) {
+        ChanDesc d = make_desc(ctx->chans[k]);
+        if (itermax_override > 0) d.itermax = itermax_override;
         cd.push_back(d);
-    }
-    for (auto& s : cs) {
-        s.active = 1;
+        ChanState& s = cs[k];
+        s.active = ctx->chans[k].prm.solver == STM_SOLVER_RB ? 1 : 0;
         s.sor_iters = 0;
+        s.pbuf = 0;
+        if (never_converge) s.tol2 = -1.0;
     }
     HIPCHK(ctx, hipMemcpyAsync(ctx->d_cd, cd.data(), sizeof(ChanDesc) * cd.size(),
                                hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(ctx, hipMemcpyAsync(ctx->d_cs, cs.data(), sizeof(ChanState) * cs.size(),
                                hipMemcpyHostToDevice, ctx->stream));
-    for (uint32_t s = 0; s < n; s++)
-        if ((rc = rb_sweep(ctx, (int)s))) return rc;
-    for (auto& s : cs) s.active = 0;
-    HIPCHK(ctx, hipMemcpyAsync(ctx->d_cs, cs.data(), sizeof(ChanState) * cs.size(),
+    return STM_OK;
+}
+
+int stm_rb_sweeps(stm_ctx* ctx, uint32_t n) {
+    int rc = ready(ctx);
+    if (rc) return rc;
+    if (n == 0) return STM_OK;
+    std::vector<ChanState> saved;
+    if ((rc = arm_channels(ctx, (int)n, false, &saved))) return rc;
+    if ((rc = blocked_solve(ctx, 0))) return rc;
+    // restore the scalar state, keep the new p
+    for (auto& s : saved) s.pbuf = 0;
+    HIPCHK(ctx, hipMemcpyAsync(ctx->d_cs, saved.data(), sizeof(ChanState) * saved.size(),
                                hipMemcpyHostToDevice, ctx->stream));
     if ((rc = upload_descs(ctx))) return rc;
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     if ((rc = check_launch(ctx))) return rc;
     return drain(ctx);
+}
+
+int stm_solver_variant(stm_ctx* ctx, int32_t variant, int32_t* current, int32_t* sweeps,
+                       int32_t* tile_x, int32_t* tile_y) {
+    int rc = ready(ctx);
+    if (rc) return rc;
+    if (variant >= N_TB_VARIANTS)
+        return fail(ctx, STM_ERR_INVALID_ARG, "variant %d out of range (%d)", variant,
+                    N_TB_VARIANTS);
+    if (variant >= 0) {
+        if (!variant_allowed(ctx, variant))
+            return fail(ctx, STM_ERR_INVALID_ARG,
+                        "variant %d does %d sweeps per check, params ask for %d", variant,
+                        TB_VARIANTS[variant].k, ctx->chans[0].prm.check_every);
+        if ((rc = plan_tb(ctx, variant))) return rc;
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    const TbVariant& v = TB_VARIANTS[ctx->variant];
+    if (current) *current = ctx->variant;
+    if (sweeps) *sweeps = v.k;
+    if (tile_x) *tile_x = v.oxm;
+    if (tile_y) *tile_y = v.oym;
+    return STM_OK;
+}
+
+// Times `launches` blocked launches of every permitted variant on the
+// current fields (which are restored afterwards) and keeps the fastest per
+// sweep.
+int stm_autotune(stm_ctx* ctx, uint32_t launches, int32_t* chosen, double* us_per_sweep) {
+    int rc = ready(ctx);
+    if (rc) return rc;
+    if (!ctx->any_rb) return fail(ctx, STM_ERR_STATE, "no red-black channel to tune");
+    if (launches == 0) launches = 8;
+    const size_t bytes = sizeof(double) * (size_t)ctx->total;
+    double* backup = nullptr;
+    HIPCHK(ctx, hipMalloc(&backup, bytes));
+    HIPCHK(ctx, hipMemcpyAsync(backup, ctx->p, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    hipEvent_t a, b;
+    HIPCHK(ctx, hipEventCreate(&a));
+    HIPCHK(ctx, hipEventCreate(&b));
+    std::vector<ChanState> saved;
+    int best = -1;
+    double best_us = 1e300;
+    for (int v = 0; v < N_TB_VARIANTS && rc == STM_OK; v++) {
+        if (!variant_allowed(ctx, v)) continue;
+        if ((rc = plan_tb(ctx, v))) break;
+        if ((rc = arm_channels(ctx, 1 << 30, true, saved.empty() ? &saved : nullptr))) break;
+        for (int w = 0; w < 2; w++)
+            launch_rb_tb(ctx->stream, v, ctx->ntb, ctx->d_cd, ctx->d_cs, ctx->d_tb, ctx->flags,
+                         ctx->rhs, ctx->p, ctx->p2, ctx->d_part, ctx->d_counters,
+                         ctx->d_active, 1);
+        hipEventRecord(a, ctx->stream);
+        for (uint32_t l = 0; l < launches; l++)
+            launch_rb_tb(ctx->stream, v, ctx->ntb, ctx->d_cd, ctx->d_cs, ctx->d_tb, ctx->flags,
+                         ctx->rhs, ctx->p, ctx->p2, ctx->d_part, ctx->d_counters,
+                         ctx->d_active, 1);
+        hipEventRecord(b, ctx->stream);
+        if (hipEventSynchronize(b) != hipSuccess || (rc = check_launch(ctx))) {
+            if (!rc) rc = fail(ctx, STM_ERR_HIP, "autotune launch failed");
+            break;
+        }
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, a, b);
+        const double us = ms * 1e3 / ((double)launches * TB_VARIANTS[v].k);
+        if (us < best_us) {
+            best_us = us;
+            best = v;
+        }
+        hipMemcpyAsync(ctx->p, backup, bytes, hipMemcpyDeviceToDevice, ctx->stream);
+    }
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    if (!saved.empty()) {
+        for (auto& s : saved) s.pbuf = 0;
+        hipMemcpyAsync(ctx->d_cs, saved.data(), sizeof(ChanState) * saved.size(),
+                       hipMemcpyHostToDevice, ctx->stream);
+    }
+    hipStreamSynchronize(ctx->stream);
+    hipFree(backup);
+    if (rc) return rc;
+    if ((rc = plan_tb(ctx, best))) return rc;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (chosen) *chosen = best;
+    if (us_per_sweep) *us_per_sweep = best_us;
+    return STM_OK;
 }
 
 int stm_synchronize(stm_ctx* ctx) {

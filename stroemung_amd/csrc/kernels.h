@@ -28,17 +28,24 @@ void launch_fill(hipStream_t, int64_t n, double* a, double val);
 // kernels_pressure.hip
 void launch_ghost(hipStream_t, int nbtot, const BEntry*, const ChanState*, double* p, int colour,
                   int only_active);
-void launch_rb_half(hipStream_t, int ntiles, const ChanDesc*, const ChanState*, const Tile*,
-                    const uint8_t* flags, const double* rhs, double* p, int colour, int sweep);
-void launch_resid(hipStream_t, int ntiles, const ChanDesc*, const ChanState*, const Tile*,
-                  const uint8_t* flags, const double* rhs, const double* p, double* part);
 void launch_solve_begin(hipStream_t, int nchan, const ChanDesc*, ChanState*, const double* part,
                         int32_t* active);
-void launch_solve_check(hipStream_t, int nchan, const ChanDesc*, ChanState*, const double* part,
-                        int nsweeps, int32_t* active);
 void launch_initial_norm(hipStream_t, int nchan, const ChanDesc*, ChanState*, const double* rhs,
                          const double* p, double* scratch);
 void launch_lex_solve(hipStream_t, int nchan, const ChanDesc*, ChanState*, const BEntry*,
                       const uint8_t* flags, const double* rhs, double* p, double* scratch);
+
+// kernels_rbtb.hip: temporally blocked red-black SOR
+struct TbVariant {
+    int k;          // sweeps per launch
+    int oxm, oym;   // largest output tile
+};
+extern const TbVariant TB_VARIANTS[];
+extern const int N_TB_VARIANTS;
+void launch_rb_tb(hipStream_t, int variant, int ntb, const ChanDesc*, ChanState*, const TbTile*,
+                  const uint8_t* flags, const double* rhs, double* pa, double* pb, double* part,
+                  int32_t* counters, int32_t* active, int check);
+void launch_pbuf_fix(hipStream_t, int ntiles, int nchan, const ChanDesc*, ChanState*,
+                     const Tile*, const double* pb, double* pa);
 
 }  // namespace stm

@@ -3,8 +3,8 @@
 //    inside one workgroup per channel -- bit-exact with the reference's
 //    x-outer / y-inner in-place sweep, including its unmasked sequential
 //    residual fold and stopping rule (src/simulation.rs:216-285);
-//  * red-black SOR with a ghost refresh per sweep and a fluid-masked
-//    residual, reduced deterministically, checked every `check_every` sweeps.
+//  * the set-up of the red-black solve (its sweeps live in kernels_rbtb.hip)
+//    and the ghost refresh used by the stage entry points.
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
@@ -43,62 +43,6 @@ k_ghost(int nbtot, const BEntry* __restrict__ be, const ChanState* __restrict__ 
     p[e.gidx] = ghost_value(p, e.gidx, e.pitch, e.edge);
 }
 
-// ------------------------------------------------------------ red-black SOR
-
-// One half-sweep over interior fluid cells with (x + y) & 1 == colour:
-//   p = fma(omw, p, mid * (fma(pE + pW, 1/dx^2, (pS + pN) * (1/dy^2)) - rhs))
-// `sweep` is the index of this sweep inside the current check interval; a
-// channel whose solve has stopped, or that would exceed itermax, skips it.
-__global__ void __launch_bounds__(TILE_THREADS)
-k_rb_half(const ChanDesc* __restrict__ cd, const ChanState* __restrict__ cs,
-          const Tile* __restrict__ tiles, const uint8_t* __restrict__ flags,
-          const double* __restrict__ rhs, double* __restrict__ p, int colour, int sweep)
-{
-    const Tile t = tiles[blockIdx.x];
-    const ChanDesc& d = cd[t.chan];
-    const ChanState& s = cs[t.chan];
-    if (!s.active || (int)s.sor_iters + sweep >= d.itermax) return;
-    const int x = t.x0 + threadIdx.y, y = t.y0 + threadIdx.x;
-    if (x < 1 || x > d.nx - 2 || y < 1 || y > d.ny - 2 || ((x + y) & 1) != colour) return;
-    const int64_t P = d.pitch, i = d.off + x * P + y;
-    if (!flag_fluid(flags[i])) return;
-    const double tt = fma(p[i + P] + p[i - P], d.rdx2, (p[i + 1] + p[i - 1]) * d.rdy2);
-    p[i] = fma(d.omw, p[i], d.mid * (tt - rhs[i]));
-}
-
-// Per-tile sum of the masked residual^2 (ghosts must be fresh):
-//   r = fma((pE-p)-(p-pW), 1/dx^2, ((pS-p)-(p-pN)) * (1/dy^2)) - rhs
-__global__ void __launch_bounds__(TILE_THREADS)
-k_resid(const ChanDesc* __restrict__ cd, const ChanState* __restrict__ cs,
-        const Tile* __restrict__ tiles, const uint8_t* __restrict__ flags,
-        const double* __restrict__ rhs, const double* __restrict__ p, double* __restrict__ part)
-{
-    __shared__ double sh[TILE_THREADS];
-    const Tile t = tiles[blockIdx.x];
-    const ChanDesc& d = cd[t.chan];
-    if (!cs[t.chan].active) return;
-    const int x = t.x0 + threadIdx.y, y = t.y0 + threadIdx.x;
-    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
-    double r2 = 0.0;
-    if (x >= 1 && x <= d.nx - 2 && y >= 1 && y <= d.ny - 2) {
-        const int64_t P = d.pitch, i = d.off + x * P + y;
-        if (flag_fluid(flags[i])) {
-            const double pc = p[i];
-            const double a = (p[i + P] - pc) - (pc - p[i - P]);
-            const double b = (p[i + 1] - pc) - (pc - p[i - 1]);
-            const double r = fma(a, d.rdx2, b * d.rdy2) - rhs[i];
-            r2 = r * r;
-        }
-    }
-    sh[tid] = r2;
-    __syncthreads();
-    for (int st = TILE_THREADS / 2; st > 0; st >>= 1) {
-        if (tid < st) sh[tid] = sh[tid] + sh[tid + st];
-        __syncthreads();
-    }
-    if (tid == 0) part[blockIdx.x * NPART + 1] = sh[0];
-}
-
 // Sum slot `slot` of the partials of one channel's tiles in a fixed order.
 __device__ double chan_sum(const ChanDesc& d, const double* __restrict__ part, int slot,
                            double* sh) {
@@ -131,33 +75,6 @@ k_solve_begin(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
         s.sor_iters = 0;
         s.exhausted = 0;
         s.active = (rb && d.itermax > 0) ? 1 : 0;
-        active_out[blockIdx.x] = s.active;
-    }
-}
-
-// After `nsweeps` sweeps: fold the residual, count the sweeps, stop the
-// channel when sum r^2 <= tol2 or at itermax.
-__global__ void __launch_bounds__(256)
-k_solve_check(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
-              const double* __restrict__ part, int nsweeps, int32_t* __restrict__ active_out)
-{
-    __shared__ double sh[256];
-    const ChanDesc& d = cd[blockIdx.x];
-    ChanState& s = cs[blockIdx.x];
-    if (!s.active) return;   // uniform across the block
-    const double res2 = chan_sum(d, part, 1, sh);
-    if (threadIdx.x == 0) {
-        const int left = d.itermax - (int)s.sor_iters;
-        s.sor_iters += (uint32_t)(nsweeps < left ? nsweeps : left);
-        s.res2 = res2;
-        s.norm2 = res2 / d.fluid_cells;
-        if (res2 <= s.tol2) {
-            s.active = 0;
-            s.exhausted = 0;
-        } else if ((int)s.sor_iters >= d.itermax) {
-            s.active = 0;
-            s.exhausted = 1;
-        }
         active_out[blockIdx.x] = s.active;
     }
 }
@@ -283,23 +200,9 @@ void launch_ghost(hipStream_t st, int nbtot, const BEntry* be, const ChanState* 
     if (nbtot > 0)
         k_ghost<<<(nbtot + 255) / 256, 256, 0, st>>>(nbtot, be, cs, p, colour, only_active);
 }
-void launch_rb_half(hipStream_t st, int ntiles, const ChanDesc* cd, const ChanState* cs,
-                    const Tile* tiles, const uint8_t* flags, const double* rhs, double* p,
-                    int colour, int sweep) {
-    k_rb_half<<<ntiles, TB, 0, st>>>(cd, cs, tiles, flags, rhs, p, colour, sweep);
-}
-void launch_resid(hipStream_t st, int ntiles, const ChanDesc* cd, const ChanState* cs,
-                  const Tile* tiles, const uint8_t* flags, const double* rhs, const double* p,
-                  double* part) {
-    k_resid<<<ntiles, TB, 0, st>>>(cd, cs, tiles, flags, rhs, p, part);
-}
 void launch_solve_begin(hipStream_t st, int nchan, const ChanDesc* cd, ChanState* cs,
                         const double* part, int32_t* active) {
     k_solve_begin<<<nchan, 256, 0, st>>>(cd, cs, part, active);
-}
-void launch_solve_check(hipStream_t st, int nchan, const ChanDesc* cd, ChanState* cs,
-                        const double* part, int nsweeps, int32_t* active) {
-    k_solve_check<<<nchan, 256, 0, st>>>(cd, cs, part, nsweeps, active);
 }
 void launch_initial_norm(hipStream_t st, int nchan, const ChanDesc* cd, ChanState* cs,
                          const double* rhs, const double* p, double* scratch) {

@@ -314,18 +314,19 @@ k_ranges(const ChanDesc* __restrict__ cd, const Tile* __restrict__ tiles,
         for (int k = 0; k < 6; k++) part[blockIdx.x * NPART + k] = sh[k][0];
 }
 
-// One thread per channel: fold the range partials of its tiles.
+// One workgroup per channel: fold the range partials of its tiles (min and
+// max are order-free, so the result is exact).
 // want_pressure: 1 = always (construction), 2 = only if the solve hit
 // itermax (src/simulation.rs:283), 0 = never.
-__global__ void k_ranges_reduce(int nchan, const ChanDesc* __restrict__ cd,
-                                ChanState* __restrict__ cs, const double* __restrict__ part,
-                                int want_pressure)
+__global__ void __launch_bounds__(256)
+k_ranges_reduce(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
+                const double* __restrict__ part, int want_pressure)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nchan) return;
+    __shared__ double sh[6][256];
+    const int c = blockIdx.x, tid = threadIdx.x;
     const ChanDesc& d = cd[c];
     double um = 0.0, vm = 0.0, smin = __DBL_MAX__, smax = 0.0, pmin = __DBL_MAX__, pmax = 0.0;
-    for (int k = d.tile0; k < d.tile0 + d.ntiles; k++) {
+    for (int k = d.tile0 + tid; k < d.tile0 + d.ntiles; k += 256) {
         const double* q = part + (int64_t)k * NPART;
         um = q[0] > um ? q[0] : um;
         vm = q[1] > vm ? q[1] : vm;
@@ -334,6 +335,23 @@ __global__ void k_ranges_reduce(int nchan, const ChanDesc* __restrict__ cd,
         pmin = fmin(pmin, q[4]);
         pmax = fmax(pmax, q[5]);
     }
+    sh[0][tid] = um; sh[1][tid] = vm; sh[2][tid] = smin;
+    sh[3][tid] = smax; sh[4][tid] = pmin; sh[5][tid] = pmax;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (tid < st) {
+            double a, b;
+            a = sh[0][tid]; b = sh[0][tid + st]; sh[0][tid] = b > a ? b : a;
+            a = sh[1][tid]; b = sh[1][tid + st]; sh[1][tid] = b > a ? b : a;
+            sh[2][tid] = fmin(sh[2][tid], sh[2][tid + st]);
+            sh[3][tid] = fmax(sh[3][tid], sh[3][tid + st]);
+            sh[4][tid] = fmin(sh[4][tid], sh[4][tid + st]);
+            sh[5][tid] = fmax(sh[5][tid], sh[5][tid + st]);
+        }
+        __syncthreads();
+    }
+    if (tid) return;
+    um = sh[0][0]; vm = sh[1][0]; smin = sh[2][0]; smax = sh[3][0]; pmin = sh[4][0]; pmax = sh[5][0];
     ChanState& s = cs[c];
     s.umax = um;
     s.vmax = vm;
@@ -407,7 +425,7 @@ void launch_ranges(hipStream_t st, int ntiles, int nchan, const ChanDesc* cd, Ch
                    const Tile* tiles, const uint8_t* flags, const double* u, const double* v,
                    const double* p, double* part, int want_pressure) {
     k_ranges<<<ntiles, TB, 0, st>>>(cd, tiles, flags, u, v, p, part);
-    k_ranges_reduce<<<(nchan + 63) / 64, 64, 0, st>>>(nchan, cd, cs, part, want_pressure);
+    k_ranges_reduce<<<nchan, 256, 0, st>>>(cd, cs, part, want_pressure);
 }
 void launch_delt(hipStream_t st, int nchan, const ChanDesc* cd, ChanState* cs) {
     k_delt<<<(nchan + 63) / 64, 64, 0, st>>>(nchan, cd, cs);

@@ -32,6 +32,7 @@ struct ChanDesc {
     int32_t boff, nb;            // slice of the boundary list
     int32_t tile0, ntiles;       // slice of the tile list
     int32_t itermax, check_every;
+    int32_t tb0, ntb;            // slice of the blocked-solver tile list
     double delx, dely, gamma, reynolds, omega, eps, tau;
     double fluid_cells;
     // derived solver constants, computed on the host exactly as the oracle does
@@ -55,7 +56,7 @@ struct ChanState {
     int32_t has_init;
     int32_t active;              // red-black solve still running
     int32_t exhausted;           // last solve hit itermax
-    int32_t pad;
+    int32_t pbuf;                // blocked solver: latest p is in the second buffer
 };
 
 // One boundary cell (the reference's sorted_boundary_list entry,
@@ -74,6 +75,11 @@ struct BEntry {
 
 struct Tile {
     int32_t chan, x0, y0, pad;
+};
+
+// Output tile of the blocked red-black solver: cells [x0, x0+xn) x [y0, y0+yn).
+struct TbTile {
+    int32_t chan, x0, y0, xn, yn, pad0, pad1, pad2;
 };
 
 constexpr int TILE_X = 4;        // x-rows per tile

@@ -149,6 +149,19 @@ class Context:
     def synchronize(self):
         self._check(lib().stm_synchronize(self._ctx))
 
+    def solver_variant(self, variant=-1):
+        """Select (variant >= 0) or query the blocked red-black variant:
+        returns (variant, sweeps per launch, tile_x, tile_y)."""
+        out = [C.c_int32() for _ in range(4)]
+        self._check(lib().stm_solver_variant(self._ctx, variant, *[C.byref(o) for o in out]))
+        return tuple(o.value for o in out)
+
+    def autotune(self, launches=8):
+        chosen = C.c_int32()
+        us = C.c_double()
+        self._check(lib().stm_autotune(self._ctx, launches, C.byref(chosen), C.byref(us)))
+        return chosen.value, us.value
+
     def profile(self, sample):
         self._check(lib().stm_profile(self._ctx, sample))
 

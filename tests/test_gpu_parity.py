@@ -186,18 +186,27 @@ def test_stages_bit_exact_random_fields(seed, nx, ny):
     _cmp(ctx, ch, osim, ("p",), "ghost p")
 
 
+N_VARIANTS = 8
+
+
+@pytest.mark.parametrize("variant", range(N_VARIANTS))
 @pytest.mark.parametrize("seed,nx,ny", [(6, 50, 33), (7, 300, 140)])
-def test_rb_sweeps_bit_exact(seed, nx, ny):
-    say("test: red-black sweeps", seed, nx, ny)
+def test_rb_sweeps_bit_exact(seed, nx, ny, variant):
+    say("test: red-black sweeps", seed, nx, ny, "variant", variant)
     ct, vel = mixed_grid(seed, nx, ny)
     fields = random_fields(seed, (nx, ny))
     kw = dict(delx=0.3, dely=0.2, delt=0.01, gamma=0.9, reynolds=100.0,
-              sor_absolute_epsilon=1e-6, max_iterations=50, omega=1.7, solver=O.SOLVER_RB)
+              sor_absolute_epsilon=1e-6, max_iterations=50, omega=1.7, solver=O.SOLVER_RB,
+              check_every=0)
     ctx, ch, osim = _pair(ct, vel, fields=fields, **kw)
+    assert ctx.solver_variant(variant)[0] == variant
     assert osim.call("rebuild_boundary_list") == 0
     ctx.stage("rb_sweeps", 7)
-    O.lib().orc_rb_sweeps(osim.s, 7)
+    osim.call("rb_sweeps", 7)
     _cmp(ctx, ch, osim, ("p",), "7 sweeps")
+    ctx.stage("rb_sweeps", 12)
+    osim.call("rb_sweeps", 12)
+    _cmp(ctx, ch, osim, ("p",), "19 sweeps")
 
 
 def test_rb_solve_matches_oracle_iterations():

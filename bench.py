@@ -7,11 +7,12 @@ zero initial fields.  A step is one full tick (boundary conditions, F/G,
 RHS, pressure solve to the residual, velocity update, next step size).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py --profile        # fixed sweeps + one 256 MiB copy, for rocprofv3
 
 Prints one JSON line.  `value` = interior cells * K / wall seconds of the K
 timed ticks, inputs resident in HBM.  The roofline entry is for the dominant
-kernel (the red-black SOR pass), timed with HIP events on the library's
-stream inside the timed region; the CPU baseline is the oracle's
+kernel (the blocked red-black SOR launch), timed with HIP events on the
+library's stream inside the timed region; the CPU baseline is the oracle's
 lexicographic SOR (the reference algorithm, one core) timed on a bounded
 sample of the same grid.
 """
@@ -29,6 +30,7 @@ HBM_PEAK_GBS = 8000.0
 # algorithmic bytes per interior cell of one red-black sweep: each half-sweep
 # reads own p, other p, own rhs and writes own p at 4 B/cell, plus 1 B flags
 SWEEP_BYTES_PER_CELL = 33.0
+COPY_PROBE_BYTES = 256 << 20
 
 
 def parse():
@@ -40,9 +42,11 @@ def parse():
     ap.add_argument("--eps", type=float, default=1e-6)
     ap.add_argument("--check-every", type=int, default=0,
                     help="sweeps per blocked launch (1/2/4/8); 0 = autotuned")
-    ap.add_argument("--sample", type=int, default=32, help="time every n-th launch")
+    ap.add_argument("--sample", type=int, default=16, help="time every n-th launch")
     ap.add_argument("--cpu-sweeps", type=int, default=150)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile", action="store_true",
+                    help="short fixed run for hardware counters, no JSON line")
     return ap.parse_args()
 
 
@@ -87,12 +91,7 @@ def cpu_baseline(n, sweeps_per_step, nsweeps):
     }
 
 
-def main():
-    a = parse()
-    if a.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(--gpus %d requested)" % a.gpus, file=sys.stderr)
-        sys.exit(2)
+def make_context(a):
     from stroemung_amd import presets
     from stroemung_amd.simulation import Context, make_params
 
@@ -103,6 +102,35 @@ def main():
     ctx = Context(0)
     ch = ctx.add_grid(prm, ct, vel)
     ctx.construct()
+    return ctx, ch
+
+
+def profile_run(a):
+    """One tick, the tuned variant, 64 blocked launches of fixed sweeps and
+    one copy of COPY_PROBE_BYTES (the byte-counter reference)."""
+    ctx, ch = make_context(a)
+    ctx.tick()
+    ctx.autotune(8)
+    variant, k, tx, ty = ctx.solver_variant()
+    ctx.stage("rb_sweeps", 64 * k)
+    gbs = ctx.stream_copy(COPY_PROBE_BYTES, 1)
+    ctx.synchronize()
+    print("profile run: variant %d (K=%d, tile %dx%d), %d sweeps; copy probe %d bytes, %.0f GB/s"
+          % (variant, k, tx, ty, 64 * k, COPY_PROBE_BYTES, gbs), flush=True)
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(--gpus %d requested)" % a.gpus, file=sys.stderr)
+        sys.exit(2)
+    if a.profile:
+        profile_run(a)
+        return
+    n = a.n
+    ctx, ch = make_context(a)
+    copy_gbs = ctx.stream_copy(1 << 30, 10)
     for _ in range(a.warmup):
         ctx.tick()
     tuned = ctx.autotune(8)
@@ -157,13 +185,14 @@ def main():
             "variant": {"id": variant, "sweeps_per_launch": ksweeps, "tile": [tile_x, tile_y],
                         "autotune_us_per_sweep": tuned[1]},
             "sweeps_per_step": sweeps_per_step,
+            "us_per_sweep": elapsed * 1e6 / max(sweeps, 1),
             "final_delt": st["delt"],
             "final_time": st["time"],
             "final_residual_ratio": (st["residual_squared"] / st["tolerance_squared"]
                                      if st["tolerance_squared"] else None),
         },
         "roofline": {
-            "kernel": kname,
+            "kernel": "k_rb_strip (blocked red-black SOR launch)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
@@ -173,6 +202,7 @@ def main():
             "bytes_per_launch": bytes_per_launch,
             "avg_launch_us": avg_s * 1e6,
             "timed_launches": timed,
+            "measured_copy_gbs": copy_gbs,
         },
         "kernels": kernels,
     }

@@ -178,6 +178,9 @@ int stm_autotune(stm_ctx *ctx, uint32_t launches, int32_t *chosen, double *us_pe
 
 /* ---- measurement (no reference counterpart) */
 int stm_synchronize(stm_ctx *ctx);
+/* Streaming device-to-device copy of `bytes` (read + write), `reps` times,
+ * on the context's stream: the measured bandwidth ceiling in GB/s. */
+int stm_stream_copy(stm_ctx *ctx, uint64_t bytes, uint32_t reps, double *gbs);
 /* Time every `sample`-th launch of each kernel with HIP events on the
  * context's stream (0 disables). */
 int stm_profile(stm_ctx *ctx, uint32_t sample);

@@ -88,6 +88,7 @@ SIGNATURES = {
     "stm_solver_variant": (C.c_int, [_CTX, C.c_int32] + [C.POINTER(C.c_int32)] * 4),
     "stm_autotune": (C.c_int, [_CTX, C.c_uint32, C.POINTER(C.c_int32), _D]),
     "stm_synchronize": (C.c_int, [_CTX]),
+    "stm_stream_copy": (C.c_int, [_CTX, C.c_uint64, C.c_uint32, _D]),
     "stm_profile": (C.c_int, [_CTX, C.c_uint32]),
     "stm_kernel_stats": (C.c_int, [_CTX, C.c_char_p, _D, C.POINTER(C.c_uint64),
                                    C.POINTER(C.c_uint64)]),

@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <unordered_map>
@@ -72,7 +73,7 @@ struct stm_ctx {
     double* d_part = nullptr;
     int32_t* d_active = nullptr;
     int32_t* h_active = nullptr;   // pinned
-    int32_t* d_counters = nullptr; // per-channel workgroup tickets
+    double* d_tbpart = nullptr;    // blocked solver: residual partials, two launches
     TbTile* d_tb = nullptr;
     int ntb = 0, tb_capacity = 0, part_capacity = 0;
     int variant = -1;              // blocked-solver variant in use
@@ -306,11 +307,23 @@ std::vector<std::pair<int, int>> split(int n, int tmax) {
     return out;
 }
 
+// As split(), with every part starting at an even index (tmax even): the
+// blocked solver loads y as aligned pairs.
+std::vector<std::pair<int, int>> split_even(int n, int tmax) {
+    auto pairs = split((n + 1) / 2, tmax / 2);
+    for (auto& p : pairs) {
+        p.first *= 2;
+        p.second *= 2;
+    }
+    pairs.back().second = n - pairs.back().first;
+    return pairs;
+}
+
 int count_tb(stm_ctx* ctx, int variant) {
     const TbVariant& v = TB_VARIANTS[variant];
     int n = 0;
     for (auto& c : ctx->chans)
-        n += (int)(split((int)c.nx, v.oxm).size() * split((int)c.ny, v.oym).size());
+        n += (int)(split((int)c.nx, v.oxm).size() * split_even((int)c.ny, v.oym).size());
     return n;
 }
 
@@ -322,7 +335,7 @@ int plan_tb(stm_ctx* ctx, int variant) {
         HostChan& c = ctx->chans[k];
         c.tb0 = (int32_t)tl.size();
         for (auto xs : split((int)c.nx, v.oxm))
-            for (auto ys : split((int)c.ny, v.oym))
+            for (auto ys : split_even((int)c.ny, v.oym))
                 tl.push_back(TbTile{(int32_t)k, xs.first, ys.first, xs.second, ys.second, 0, 0, 0});
         c.ntb = (int32_t)tl.size() - c.tb0;
     }
@@ -396,8 +409,7 @@ int finalize(stm_ctx* ctx) {
     HIPCHK(ctx, hipMalloc(&ctx->d_part, sizeof(double) * NPART * maxtiles));
     ctx->part_capacity = (int)maxtiles;
     HIPCHK(ctx, hipMalloc(&ctx->d_active, sizeof(int32_t) * nch));
-    HIPCHK(ctx, hipMalloc(&ctx->d_counters, sizeof(int32_t) * nch));
-    HIPCHK(ctx, hipMemsetAsync(ctx->d_counters, 0, sizeof(int32_t) * nch, ctx->stream));
+    HIPCHK(ctx, hipMalloc(&ctx->d_tbpart, sizeof(double) * 2 * maxtiles));
     HIPCHK(ctx, hipHostMalloc(&ctx->h_active, sizeof(int32_t) * nch, hipHostMallocDefault));
     HIPCHK(ctx, hipHostMalloc(&ctx->h_cs, sizeof(ChanState) * nch, hipHostMallocDefault));
     HIPCHK(ctx, hipMemcpyAsync(ctx->d_tiles, tl.data(), sizeof(Tile) * tl.size(),
@@ -528,18 +540,25 @@ int poll_active(stm_ctx* ctx, bool* any) {
     return STM_OK;
 }
 
-// Blocked red-black launches until no channel is active.  Launches are
-// queued in growing batches between polls of the per-channel flags; a
-// launch finding its channel stopped returns at once.  check = 0 runs the
-// sweeps without the residual (fixed sweep counts).
+// Launch L of the blocked solver on the current variant.
+void launch_blocked(stm_ctx* ctx, int variant, int L, int mode) {
+    const int cap = ctx->part_capacity;
+    double* prev = ctx->d_tbpart + (size_t)((L + 1) & 1) * cap;
+    double* cur = ctx->d_tbpart + (size_t)(L & 1) * cap;
+    launch_rb_tb(ctx->stream, variant, ctx->ntb, ctx->d_cd, ctx->d_cs, ctx->d_tb, ctx->flags,
+                 ctx->rhs, ctx->p, ctx->p2, prev, cur, ctx->d_active, L, mode);
+}
+
+// Blocked red-black launches until no channel is running.  Launch L also
+// takes the stop decision on launch L - 1, so a solve ends with one launch
+// that only decides.  Launches are queued in growing batches between polls
+// of the per-channel flags; a launch finding its channel stopped returns at
+// once.  check = 0 runs a fixed number of sweeps without the residual.
 int blocked_solve(stm_ctx* ctx, int check) {
-    int batch = 2;
+    int batch = 2, L = 0;
     for (;;) {
-        for (int b = 0; b < batch; b++)
-            LAUNCH("rb_tb", launch_rb_tb(ctx->stream, ctx->variant, ctx->ntb, ctx->d_cd,
-                                         ctx->d_cs, ctx->d_tb, ctx->flags, ctx->rhs, ctx->p,
-                                         ctx->p2, ctx->d_part, ctx->d_counters, ctx->d_active,
-                                         check));
+        for (int b = 0; b < batch; b++, L++)
+            LAUNCH("rb_tb", launch_blocked(ctx, ctx->variant, L, check ? TB_CHECK : 0));
         bool any = false;
         int rc = poll_active(ctx, &any);
         if (rc) return rc;
@@ -667,9 +686,9 @@ void stm_destroy(stm_ctx* ctx) {
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
     drain(ctx);
     for (hipEvent_t e : ctx->free_events) hipEventDestroy(e);
-    void* bufs[] = {ctx->u, ctx->v, ctx->p, ctx->f, ctx->g, ctx->rhs, ctx->scratch, ctx->flags,
-                    ctx->d_cd, ctx->d_cs, ctx->d_be, ctx->d_restore, ctx->d_tiles, ctx->d_part,
-                    ctx->d_active};
+    void* bufs[] = {ctx->u, ctx->v, ctx->p, ctx->f, ctx->g, ctx->rhs, ctx->scratch, ctx->p2,
+                    ctx->flags, ctx->d_cd, ctx->d_cs, ctx->d_be, ctx->d_restore, ctx->d_tiles,
+                    ctx->d_part, ctx->d_active, ctx->d_tbpart, ctx->d_tb};
     for (void* b : bufs)
         if (b) hipFree(b);
     if (ctx->h_active) hipHostFree(ctx->h_active);
@@ -924,6 +943,7 @@ int arm_channels(stm_ctx* ctx, int itermax_override, bool never_converge,
         cd.push_back(d);
         ChanState& s = cs[k];
         s.active = ctx->chans[k].prm.solver == STM_SOLVER_RB ? 1 : 0;
+        s.stop_launch = s.active ? -1 : 0;
         s.sor_iters = 0;
         s.pbuf = 0;
         if (never_converge) s.tol2 = -1.0;
@@ -932,6 +952,12 @@ int arm_channels(stm_ctx* ctx, int itermax_override, bool never_converge,
                                hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(ctx, hipMemcpyAsync(ctx->d_cs, cs.data(), sizeof(ChanState) * cs.size(),
                                hipMemcpyHostToDevice, ctx->stream));
+    // the flags the host polls must say "running" too
+    std::vector<int32_t> act(cs.size());
+    for (size_t k = 0; k < cs.size(); k++) act[k] = cs[k].active;
+    HIPCHK(ctx, hipMemcpyAsync(ctx->d_active, act.data(), sizeof(int32_t) * act.size(),
+                               hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     return STM_OK;
 }
 
@@ -997,15 +1023,10 @@ int stm_autotune(stm_ctx* ctx, uint32_t launches, int32_t* chosen, double* us_pe
         if (!variant_allowed(ctx, v)) continue;
         if ((rc = plan_tb(ctx, v))) break;
         if ((rc = arm_channels(ctx, 1 << 30, true, saved.empty() ? &saved : nullptr))) break;
-        for (int w = 0; w < 2; w++)
-            launch_rb_tb(ctx->stream, v, ctx->ntb, ctx->d_cd, ctx->d_cs, ctx->d_tb, ctx->flags,
-                         ctx->rhs, ctx->p, ctx->p2, ctx->d_part, ctx->d_counters,
-                         ctx->d_active, 1);
+        int L = 0;
+        for (; L < 2; L++) launch_blocked(ctx, v, L, TB_CHECK);
         hipEventRecord(a, ctx->stream);
-        for (uint32_t l = 0; l < launches; l++)
-            launch_rb_tb(ctx->stream, v, ctx->ntb, ctx->d_cd, ctx->d_cs, ctx->d_tb, ctx->flags,
-                         ctx->rhs, ctx->p, ctx->p2, ctx->d_part, ctx->d_counters,
-                         ctx->d_active, 1);
+        for (uint32_t l = 0; l < launches; l++, L++) launch_blocked(ctx, v, L, TB_CHECK);
         hipEventRecord(b, ctx->stream);
         if (hipEventSynchronize(b) != hipSuccess || (rc = check_launch(ctx))) {
             if (!rc) rc = fail(ctx, STM_ERR_HIP, "autotune launch failed");
@@ -1035,6 +1056,37 @@ int stm_autotune(stm_ctx* ctx, uint32_t launches, int32_t* chosen, double* us_pe
     if (chosen) *chosen = best;
     if (us_per_sweep) *us_per_sweep = best_us;
     return STM_OK;
+}
+
+int stm_stream_copy(stm_ctx* ctx, uint64_t bytes, uint32_t reps, double* gbs) {
+    if (!ctx || !gbs || bytes < 16 || reps == 0)
+        return fail(ctx, STM_ERR_INVALID_ARG, "invalid arguments");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    bytes &= ~(uint64_t)15;
+    double *a = nullptr, *b = nullptr;
+    HIPCHK(ctx, hipMalloc(&a, bytes));
+    if (hipMalloc(&b, bytes) != hipSuccess) {
+        hipFree(a);
+        return fail(ctx, STM_ERR_HIP, "hipMalloc of %llu bytes failed", (unsigned long long)bytes);
+    }
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    launch_fill(ctx->stream, (int64_t)(bytes / 8), a, 1.0);
+    launch_copy(ctx->stream, (int64_t)bytes, a, b);   // warm-up
+    hipEventRecord(e0, ctx->stream);
+    for (uint32_t r = 0; r < reps; r++) launch_copy(ctx->stream, (int64_t)bytes, a, b);
+    hipEventRecord(e1, ctx->stream);
+    hipError_t e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(a);
+    hipFree(b);
+    if (e != hipSuccess) return fail(ctx, STM_ERR_HIP, "copy probe: %s", hipGetErrorString(e));
+    *gbs = 2.0 * (double)bytes * reps / (ms * 1e-3) / 1e9;
+    return check_launch(ctx);
 }
 
 int stm_synchronize(stm_ctx* ctx) {

@@ -24,6 +24,7 @@ void launch_ranges(hipStream_t, int ntiles, int nchan, const ChanDesc*, ChanStat
 void launch_delt(hipStream_t, int nchan, const ChanDesc*, ChanState*);
 void launch_advance_time(hipStream_t, int nchan, ChanState*);
 void launch_fill(hipStream_t, int64_t n, double* a, double val);
+void launch_copy(hipStream_t, int64_t bytes, const double* src, double* dst);
 
 // kernels_pressure.hip
 void launch_ghost(hipStream_t, int nbtot, const BEntry*, const ChanState*, double* p, int colour,
@@ -42,9 +43,13 @@ struct TbVariant {
 };
 extern const TbVariant TB_VARIANTS[];
 extern const int N_TB_VARIANTS;
+// mode bit TB_CHECK: form the residual after the sweeps and stop on it.
+enum : int { TB_CHECK = 1 };
+// Launch L of a blocked solve: reads the partials of launch L - 1 from
+// part_prev, writes its own to part_cur (one double per tile).
 void launch_rb_tb(hipStream_t, int variant, int ntb, const ChanDesc*, ChanState*, const TbTile*,
-                  const uint8_t* flags, const double* rhs, double* pa, double* pb, double* part,
-                  int32_t* counters, int32_t* active, int check);
+                  const uint8_t* flags, const double* rhs, double* pa, double* pb,
+                  const double* part_prev, double* part_cur, int32_t* active, int L, int mode);
 void launch_pbuf_fix(hipStream_t, int ntiles, int nchan, const ChanDesc*, ChanState*,
                      const Tile*, const double* pb, double* pa);
 

@@ -75,6 +75,8 @@ k_solve_begin(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
         s.sor_iters = 0;
         s.exhausted = 0;
         s.active = (rb && d.itermax > 0) ? 1 : 0;
+        s.stop_launch = s.active ? -1 : 0;
+        s.pbuf = 0;
         active_out[blockIdx.x] = s.active;
     }
 }

@@ -1,27 +1,38 @@
 // Temporally blocked red-black SOR for gfx950.
 //
-// One launch advances every active channel by up to K full sweeps (ghost
-// refresh, red half, black half).  Each workgroup owns an output tile of at
-// most OXM x OYM cells; it loads the tile plus a halo of H = 2K + 3 cells
-// into LDS, performs the K sweeps there, and writes the output tile back.
-// A dependency reaches one cell per half-sweep plus one through a ghost
-// refresh, so after K sweeps every cell at distance >= 2K + 1 from the loaded
-// region is exact; the two extra cells cover the final ghost refresh and
-// the residual.  The result is therefore bit-identical to K separate global
-// sweeps (same operations, same operands), while HBM / cache traffic drops
-// by roughly K times.
+// Launch L of a solve advances every running channel by up to K full sweeps.
+// Each workgroup owns an output tile; it loads the tile plus a halo of
+// H = 2K + 4 cells, performs the K sweeps on chip and writes the output tile
+// back.  A dependency reaches one cell per half-sweep plus one through a
+// ghost refresh, so after K sweeps every cell at distance >= 2K + 1 from the
+// loaded region is exact; the remaining halo covers the final ghost refresh
+// and the residual (H is even so that loaded rows start on a 16-byte pair).
+// The result is bit-identical to K separate global sweeps of
+//     refresh all ghosts; red half-sweep; black half-sweep
+// (same operations on the same operands) while the cache / HBM traffic drops
+// by about K times.
 //
-// LDS layout: p split by absolute colour (x + y) & 1, each colour with one
-// slot per (x, y-pair).  A thread owns pairs (lx, 2m), (lx, 2m+1); in a
-// half-sweep it updates the one of colour c, reading the four neighbours of
-// the other colour from consecutive slots -- no divergence, no bank
-// conflicts.  rhs and per-cell codes stay in registers.
+// Schedule inside a launch: ghosts take the colour of their cell and read
+// only fluid cells of the other colour, and a cell of colour c reads only
+// cells of the other colour.  So the red ghosts (sources black) can be
+// refreshed in the red phase and the black ghosts (sources red) in the black
+// phase, each from values that the phase does not change -- the same values
+// the start-of-sweep refresh sees.  Two barriers per sweep.
 //
-// p ping-pongs between two buffers (tiles read neighbours' halos, so a
-// launch may not write what another workgroup of the same launch reads).
-// The last workgroup of a channel (atomic ticket) folds the per-tile
-// residual partials in tile order, updates the sweep count and stops the
-// channel when sum r^2 <= tol2 or at itermax.
+// Registers and LDS: the loaded region is EX x EY cells, EY = 2*HY.  Thread
+// (strip s, pair m) owns rows [s*R, s*R + R) of the y-pair (2m, 2m+1) -- one
+// red and one black cell per row.  E / W neighbours come from its own
+// registers except at the strip ends; of N / S one is its own other cell.
+// Every new value is published to LDS (split by absolute colour, one slot
+// per (x, pair)) for threads that read it as N / S or across a strip end.
+// R is even, so a tile's colour pattern is fixed by the parity of its x
+// origin; each parity has its own compiled body without colour selects.
+//
+// No fences, no second kernel: launch L writes one residual partial per tile;
+// every workgroup of launch L + 1 folds the previous launch's partials of its
+// channel in tile order, so all take the same stop decision; the channel's
+// first tile records it.  p ping-pongs between two buffers (workgroups read
+// each other's halos).
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
@@ -38,219 +49,279 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double* wsum) {
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < NT / 64; w++) s = s + wsum[w];
+    __syncthreads();
+    return s;
+}
+
+// Everything after the load, for a tile whose (even-x, even-y) cells have
+// colour RP0.  Own cell (row r, slot q) has colour RP0 ^ (r & 1) ^ q.
+template <int K, int R, int HY, int NS, int RP0>
+struct Blk {
+    static constexpr int NT = NS * HY;
+    static constexpr int EX = NS * R;
+
+    __device__ static __forceinline__ int col(int r, int q) { return RP0 ^ (r & 1) ^ q; }
+    // own slot of colour c in row r
+    __device__ static __forceinline__ int slot(int r, int c) { return c ^ RP0 ^ (r & 1); }
+
+    // the four neighbours (colour col(r,q) ^ 1) of own cell (r, q)
+    template <int r, int q>
+    __device__ static __forceinline__ void nbrs(const double (*P)[EX * HY],
+                                                const double (&pv)[R][2], int lx0, int m,
+                                                double& pn, double& ps, double& pe, double& pw) {
+        constexpr int o = RP0 ^ (r & 1) ^ q ^ 1;
+        const int lx = lx0 + r;
+        const double x = P[o][lx * HY + m - 1 + 2 * q];
+        pn = q ? pv[r][0] : x;
+        ps = q ? x : pv[r][1];
+        if constexpr (r + 1 < R) pe = pv[r + 1][q]; else pe = P[o][(lx + 1) * HY + m];
+        if constexpr (r >= 1) pw = pv[r - 1][q]; else pw = P[o][(lx - 1) * HY + m];
+    }
+
+    // ghost pressure (src/grid/mod.rs:343-412) of own ghosts of colour g
+    template <int r, int g>
+    __device__ static __forceinline__ void refresh_row(double (*P)[EX * HY], double (&pv)[R][2],
+                                                       const uint8_t (&cc)[R][2], int lx0, int m) {
+        constexpr int q = g ^ RP0 ^ (r & 1);
+        const int e = cc[r][q] >> C_EDGE_SHIFT;
+        if (!e) return;
+        double pn, ps, pe, pw;
+        nbrs<r, q>(P, pv, lx0, m, pn, ps, pe, pw);
+        double v;
+        switch (e) {
+        case E_N: v = pn; break;
+        case E_NE: v = (pn + pe) * 0.5; break;
+        case E_E: v = pe; break;
+        case E_SE: v = (ps + pe) * 0.5; break;
+        case E_S: v = ps; break;
+        case E_SW: v = (ps + pw) * 0.5; break;
+        case E_W: v = pw; break;
+        default: v = (pn + pw) * 0.5; break;   // E_NW
+        }
+        pv[r][q] = v;
+        P[g][(lx0 + r) * HY + m] = v;
+    }
+
+    template <int g, int r = 0>
+    __device__ static __forceinline__ void refresh(double (*P)[EX * HY], double (&pv)[R][2],
+                                                   const uint8_t (&cc)[R][2], int lx0, int m) {
+        if constexpr (r < R) {
+            refresh_row<r, g>(P, pv, cc, lx0, m);
+            refresh<g, r + 1>(P, pv, cc, lx0, m);
+        }
+    }
+
+    // red-black update of own fluid cells of colour c
+    template <int c, int r = 0>
+    __device__ static __forceinline__ void update(double (*P)[EX * HY], double (&pv)[R][2],
+                                                  const double (&rr)[R][2],
+                                                  const uint8_t (&cc)[R][2], int lx0, int m,
+                                                  double omw, double mid, double rdx2,
+                                                  double rdy2) {
+        if constexpr (r < R) {
+            constexpr int q = c ^ RP0 ^ (r & 1);
+            if (cc[r][q] & C_UPDATE) {
+                double pn, ps, pe, pw;
+                nbrs<r, q>(P, pv, lx0, m, pn, ps, pe, pw);
+                const double tt = fma(pe + pw, rdx2, (ps + pn) * rdy2);
+                const double nv = fma(omw, pv[r][q], mid * (tt - rr[r][q]));
+                pv[r][q] = nv;
+                P[c][(lx0 + r) * HY + m] = nv;
+            }
+            update<c, r + 1>(P, pv, rr, cc, lx0, m, omw, mid, rdx2, rdy2);
+        }
+    }
+
+    // sum of r^2 over own fluid output cells (ghosts fresh)
+    template <int r = 0, int q = 0>
+    __device__ static __forceinline__ double resid(double (*P)[EX * HY], const double (&pv)[R][2],
+                                                   const double (&rr)[R][2],
+                                                   const uint8_t (&cc)[R][2], int lx0, int m,
+                                                   double rdx2, double rdy2, double acc) {
+        if constexpr (r < R) {
+            if ((cc[r][q] & (C_UPDATE | C_OUT)) == (C_UPDATE | C_OUT)) {
+                double pn, ps, pe, pw;
+                nbrs<r, q>(P, pv, lx0, m, pn, ps, pe, pw);
+                const double pc = pv[r][q];
+                const double a = (pe - pc) - (pc - pw);
+                const double b = (ps - pc) - (pc - pn);
+                const double res = fma(a, rdx2, b * rdy2) - rr[r][q];
+                acc = acc + res * res;
+            }
+            if constexpr (q == 0) return resid<r, 1>(P, pv, rr, cc, lx0, m, rdx2, rdy2, acc);
+            else return resid<r + 1, 0>(P, pv, rr, cc, lx0, m, rdx2, rdy2, acc);
+        } else {
+            return acc;
+        }
+    }
+
+    __device__ static __forceinline__ double run(double (*P)[EX * HY], double (&pv)[R][2],
+                                                 const double (&rr)[R][2],
+                                                 const uint8_t (&cc)[R][2], int lx0, int m,
+                                                 int nsw, bool check, const ChanDesc& d) {
+        const double omw = d.omw, mid = d.mid, rdx2 = d.rdx2, rdy2 = d.rdy2;
+#pragma unroll
+        for (int r = 0; r < R; r++)
+#pragma unroll
+            for (int q = 0; q < 2; q++) P[col(r, q)][(lx0 + r) * HY + m] = pv[r][q];
+        __syncthreads();
+        refresh<1>(P, pv, cc, lx0, m);              // black ghosts, for the first red half
+        __syncthreads();
+        for (int sw = 0; sw < nsw; sw++) {
+            update<0>(P, pv, rr, cc, lx0, m, omw, mid, rdx2, rdy2);
+            refresh<0>(P, pv, cc, lx0, m);          // red ghosts, for the black half
+            __syncthreads();
+            update<1>(P, pv, rr, cc, lx0, m, omw, mid, rdx2, rdy2);
+            if (check || sw + 1 < nsw) refresh<1>(P, pv, cc, lx0, m);   // for the next red half
+            __syncthreads();
+        }
+        double acc = 0.0;
+        if (check) {
+            refresh<0>(P, pv, cc, lx0, m);          // red ghosts from the final black
+            __syncthreads();
+            acc = resid(P, pv, rr, cc, lx0, m, rdx2, rdy2, 0.0);
+        }
+        return acc;
+    }
+};
+
 }  // namespace
 
-template <int OXM, int OYM, int K, int NT>
-__global__ void __launch_bounds__(NT)
-k_rb_tb(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
-        const TbTile* __restrict__ tiles, const uint8_t* __restrict__ flags,
-        const double* __restrict__ rhs, double* __restrict__ pa, double* __restrict__ pb,
-        double* __restrict__ part, int32_t* __restrict__ counters,
-        int32_t* __restrict__ active_out, int check)
+template <int K, int R, int HY, int NS>
+__global__ void __launch_bounds__(NS * HY)
+k_rb_strip(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
+           const TbTile* __restrict__ tiles, const uint8_t* __restrict__ flags,
+           const double* __restrict__ rhs, double* __restrict__ pa, double* __restrict__ pb,
+           const double* __restrict__ part_prev, double* __restrict__ part_cur,
+           int32_t* __restrict__ active_out, int L, int mode)
 {
-    constexpr int H = 2 * K + 3;
-    constexpr int EX = OXM + 2 * H;
-    constexpr int EYR = OYM + 2 * H;
-    constexpr int EY = EYR + (EYR & 1);
-    constexpr int HY = EY / 2;
-    constexpr int NPAIR = EX * HY;
-    constexpr int PPT = (NPAIR + NT - 1) / NT;
-    constexpr int NW = NT / 64;
+    constexpr int NT = NS * HY;
+    constexpr int H = 2 * K + 4;
+    constexpr int EX = NS * R;
+    constexpr int EY = 2 * HY;
+    static_assert(NT % 64 == 0 && NT <= 1024, "workgroup size");
+    static_assert(R % 2 == 0, "rows per thread must be even");
+    static_assert(EX > 2 * H && EY > 2 * H, "tile smaller than its halo");
 
-    __shared__ double P[2][NPAIR];
-    __shared__ double wsum[NW];
-    __shared__ int last;
+    __shared__ double P[2][EX * HY];
+    __shared__ double wsum[NT / 64];
 
     const TbTile t = tiles[blockIdx.x];
     const int c = t.chan;
     const ChanDesc& d = cd[c];
     ChanState& s = cs[c];
-    if (!s.active) return;                       // uniform per workgroup
-    const int left = d.itermax - (int)s.sor_iters;
-    const int nsw = left < K ? left : K;
-    const int par = s.pbuf;
-    const double* __restrict__ pin = par ? pb : pa;
-    double* __restrict__ pout = par ? pa : pb;
-    const int ex0 = t.x0 - H, ey0 = t.y0 - H;
+    if (s.stop_launch >= 0) return;              // uniform: the channel is done
+    const int tid = threadIdx.x;
+    const bool check = mode & TB_CHECK;
+    const double* __restrict__ pin = (L & 1) ? pb : pa;
+    double* __restrict__ pout = (L & 1) ? pa : pb;
+
+    const int ex0 = t.x0 - H, ey0 = t.y0 - H;    // ey0 is even
     const int nx = d.nx, ny = d.ny;
     const int64_t pitch = d.pitch, base = d.off;
-    const int tid = threadIdx.x;
+    const int strip = tid / HY, m = tid - (tid / HY) * HY;
+    const int lx0 = strip * R;
+    const int gy = ey0 + 2 * m;                  // even: (gy, gy+1) is one aligned pair
 
-    // ---- load: p into LDS, rhs and cell codes into registers
-    double rr[PPT][2];
-    uint8_t code[PPT][2];
+    // ---- load (issued before the stop decision so the latencies overlap)
+    double pv[R][2], rr[R][2];
+    uint8_t cc[R][2];
 #pragma unroll
-    for (int j = 0; j < PPT; j++) {
-        const int k = tid + j * NT;
-        const int lx = k / HY, m = k - (k / HY) * HY;
+    for (int r = 0; r < R; r++) {
+        const int lx = lx0 + r, gx = ex0 + lx;
+        const bool row_in = gx >= 0 && gx < nx;
+        const int64_t g = base + gx * pitch + gy;
+        double2 pp = make_double2(0.0, 0.0), rh = make_double2(0.0, 0.0);
+        uint8_t f0 = 0xff, f1 = 0xff;             // 0xff: outside the channel
+        if (row_in && gy >= 0 && gy + 1 < ny) {
+            pp = *reinterpret_cast<const double2*>(pin + g);
+            rh = *reinterpret_cast<const double2*>(rhs + g);
+            f0 = flags[g];
+            f1 = flags[g + 1];
+        } else if (row_in && gy >= 0 && gy < ny) {
+            pp.x = pin[g];
+            rh.x = rhs[g];
+            f0 = flags[g];
+        }
+        pv[r][0] = pp.x;
+        pv[r][1] = pp.y;
+        rr[r][0] = rh.x;
+        rr[r][1] = rh.y;
 #pragma unroll
         for (int q = 0; q < 2; q++) {
-            const int ly = 2 * m + q;
-            const int gx = ex0 + lx, gy = ey0 + ly;
-            const bool inside = k < NPAIR && ly < EYR && gx >= 0 && gx < nx && gy >= 0 && gy < ny;
-            double pv = 0.0, rv = 0.0;
-            uint8_t cc = 0;
-            if (inside) {
-                const int64_t g = base + gx * pitch + gy;
-                pv = pin[g];
-                rv = rhs[g];
-                const uint8_t fl = flags[g];
-                const bool inner = lx >= 1 && lx <= EX - 2 && ly >= 1 && ly <= EYR - 2;
-                if (inner && flag_fluid(fl) && gx >= 1 && gx <= nx - 2 && gy >= 1 && gy <= ny - 2)
-                    cc |= C_UPDATE;
-                if (inner && !flag_fluid(fl)) cc |= (uint8_t)(flag_edge(fl) << C_EDGE_SHIFT);
-                if (lx >= H && lx < H + t.xn && ly >= H && ly < H + t.yn) cc |= C_OUT;
+            const uint8_t fl = q ? f1 : f0;
+            const int ly = 2 * m + q, gyq = gy + q;
+            uint8_t code = 0;
+            if (fl != 0xff) {
+                const bool inner = lx >= 1 && lx <= EX - 2 && ly >= 1 && ly <= EY - 2;
+                if (inner && flag_fluid(fl) && gx >= 1 && gx <= nx - 2 && gyq >= 1 && gyq <= ny - 2)
+                    code |= C_UPDATE;
+                if (inner && !flag_fluid(fl)) code |= (uint8_t)(flag_edge(fl) << C_EDGE_SHIFT);
+                if (lx >= H && lx < H + t.xn && ly >= H && ly < H + t.yn) code |= C_OUT;
             }
-            rr[j][q] = rv;
-            code[j][q] = cc;
-            if (k < NPAIR) P[(ex0 + lx + ey0 + ly) & 1][k] = pv;
+            cc[r][q] = code;
         }
     }
-    __syncthreads();
 
-    const double omw = d.omw, mid = d.mid, rdx2 = d.rdx2, rdy2 = d.rdy2;
-
-    // ghost pressure of every refreshable boundary cell (src/grid/mod.rs:343-412)
-    auto refresh = [&]() {
-#pragma unroll
-        for (int j = 0; j < PPT; j++) {
-            const int k = tid + j * NT;
-            const int lx = k / HY, m = k - (k / HY) * HY;
-#pragma unroll
-            for (int q = 0; q < 2; q++) {
-                const int e = code[j][q] >> C_EDGE_SHIFT;
-                if (!e) continue;
-                const int col = (ex0 + lx + ey0 + 2 * m + q) & 1, o = col ^ 1;
-                const double* Q = P[o];
-                const int r = lx * HY + m;
-                double v;
-                switch (e) {
-                case E_N: v = Q[r - 1 + q]; break;
-                case E_NE: v = (Q[r - 1 + q] + Q[r + HY]) * 0.5; break;
-                case E_E: v = Q[r + HY]; break;
-                case E_SE: v = (Q[r + q] + Q[r + HY]) * 0.5; break;
-                case E_S: v = Q[r + q]; break;
-                case E_SW: v = (Q[r + q] + Q[r - HY]) * 0.5; break;
-                case E_W: v = Q[r - HY]; break;
-                default: v = (Q[r - 1 + q] + Q[r - HY]) * 0.5; break;   // E_NW
+    // ---- stop decision on the previous launch (identical in every tile)
+    const int64_t sweeps = (int64_t)L * K;
+    const int done = sweeps < d.itermax ? (int)sweeps : d.itermax;
+    if (L > 0) {
+        double acc = 0.0;
+        if (check)
+            for (int i = d.tb0 + tid; i < d.tb0 + d.ntb; i += NT) acc = acc + part_prev[i];
+        const double res2 = block_sum<NT>(acc, wsum);
+        const bool conv = check && res2 <= s.tol2;
+        if (conv || done >= d.itermax) {
+            if (blockIdx.x == (unsigned)d.tb0 && tid == 0) {
+                s.stop_launch = L - 1;
+                s.sor_iters = (uint32_t)done;
+                s.pbuf = L & 1;
+                s.active = 0;
+                s.exhausted = check && !conv;
+                if (check) {
+                    s.res2 = res2;
+                    s.norm2 = res2 / d.fluid_cells;
                 }
-                P[col][k] = v;
+                active_out[c] = 0;
             }
-        }
-        __syncthreads();
-    };
-
-    for (int sw = 0; sw < nsw; sw++) {
-        refresh();
-#pragma unroll
-        for (int colour = 0; colour < 2; colour++) {
-            const int o = colour ^ 1;
-#pragma unroll
-            for (int j = 0; j < PPT; j++) {
-                const int k = tid + j * NT;
-                const int lx = k / HY, m = k - (k / HY) * HY;
-                const int q = colour ^ ((ex0 + lx + ey0) & 1);
-                const uint8_t cc = q ? code[j][1] : code[j][0];
-                if (!(cc & C_UPDATE)) continue;
-                const double* Q = P[o];
-                const int r = lx * HY + m;
-                const double pe = Q[r + HY], pw = Q[r - HY];
-                const double pn = Q[r - 1 + q], ps = Q[r + q];
-                const double rh = q ? rr[j][1] : rr[j][0];
-                const double tt = fma(pe + pw, rdx2, (ps + pn) * rdy2);
-                P[colour][k] = fma(omw, P[colour][k], mid * (tt - rh));
-            }
-            __syncthreads();
+            return;
         }
     }
+    const int nsw = d.itermax - done < K ? d.itermax - done : K;
 
-    // ---- residual of the new state on the output cells
-    double acc = 0.0;
-    if (check) {
-        refresh();
-#pragma unroll
-        for (int j = 0; j < PPT; j++) {
-            const int k = tid + j * NT;
-            const int lx = k / HY, m = k - (k / HY) * HY;
-#pragma unroll
-            for (int q = 0; q < 2; q++) {
-                const uint8_t cc = code[j][q];
-                if ((cc & (C_UPDATE | C_OUT)) != (C_UPDATE | C_OUT)) continue;
-                const int col = (ex0 + lx + ey0 + 2 * m + q) & 1, o = col ^ 1;
-                const double* Q = P[o];
-                const int r = lx * HY + m;
-                const double pc = P[col][k];
-                const double a = (Q[r + HY] - pc) - (pc - Q[r - HY]);
-                const double b = (Q[r + q] - pc) - (pc - Q[r - 1 + q]);
-                const double res = fma(a, rdx2, b * rdy2) - rr[j][q];
-                acc = acc + res * res;
-            }
-        }
-    }
+    const double acc = (ex0 & 1) ? Blk<K, R, HY, NS, 1>::run(P, pv, rr, cc, lx0, m, nsw, check, d)
+                                 : Blk<K, R, HY, NS, 0>::run(P, pv, rr, cc, lx0, m, nsw, check, d);
 
     // ---- write the output tile
 #pragma unroll
-    for (int j = 0; j < PPT; j++) {
-        const int k = tid + j * NT;
-        const int lx = k / HY, m = k - (k / HY) * HY;
-#pragma unroll
-        for (int q = 0; q < 2; q++) {
-            if (!(code[j][q] & C_OUT)) continue;
-            const int ly = 2 * m + q;
-            const int64_t g = base + (int64_t)(ex0 + lx) * pitch + (ey0 + ly);
-            pout[g] = P[(ex0 + lx + ey0 + ly) & 1][k];
-        }
+    for (int r = 0; r < R; r++) {
+        const int64_t g = base + (int64_t)(ex0 + lx0 + r) * pitch + gy;
+        const bool o0 = cc[r][0] & C_OUT, o1 = cc[r][1] & C_OUT;
+        if (o0 && o1) *reinterpret_cast<double2*>(pout + g) = make_double2(pv[r][0], pv[r][1]);
+        else if (o0) pout[g] = pv[r][0];
+        else if (o1) pout[g + 1] = pv[r][1];
     }
-
-    // ---- per-tile partial, then the channel's last workgroup folds them
-    acc = wave_sum(acc);
-    if ((tid & 63) == 0) wsum[tid >> 6] = acc;
-    __syncthreads();
-    if (tid == 0) {
-        double b = 0.0;
-        for (int w = 0; w < NW; w++) b = b + wsum[w];
-        part[(int64_t)blockIdx.x * NPART + 2] = b;
-        __threadfence();
-        last = atomicAdd(&counters[c], 1) == d.ntb - 1;
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    double sum = 0.0;
-    for (int i = d.tb0 + tid; i < d.tb0 + d.ntb; i += NT)
-        sum = sum + __builtin_nontemporal_load(part + (int64_t)i * NPART + 2);
-    sum = wave_sum(sum);
-    if ((tid & 63) == 0) wsum[tid >> 6] = sum;
-    __syncthreads();
-    if (tid == 0) {
-        double res2 = 0.0;
-        for (int w = 0; w < NW; w++) res2 = res2 + wsum[w];
-        s.sor_iters += (uint32_t)nsw;
-        s.pbuf = par ^ 1;
-        counters[c] = 0;
-        if (check) {
-            s.res2 = res2;
-            s.norm2 = res2 / d.fluid_cells;
-            if (res2 <= s.tol2) {
-                s.active = 0;
-                s.exhausted = 0;
-            }
-        }
-        if (s.active && (int)s.sor_iters >= d.itermax) {
-            s.active = 0;
-            s.exhausted = check ? 1 : 0;
-        }
-        active_out[c] = s.active;
-    }
+    const double b = block_sum<NT>(acc, wsum);
+    if (tid == 0) part_cur[blockIdx.x] = b;
 }
 
 // After a blocked solve: channels whose latest p is in the second buffer
 // copy it back.
 __global__ void __launch_bounds__(TILE_THREADS)
 k_pbuf_fix(const ChanDesc* __restrict__ cd, const ChanState* __restrict__ cs,
-           const Tile* __restrict__ tiles, const double* __restrict__ pb, double* __restrict__ pa)
+           const stm::Tile* __restrict__ tiles, const double* __restrict__ pb,
+           double* __restrict__ pa)
 {
-    const Tile t = tiles[blockIdx.x];
+    const stm::Tile t = tiles[blockIdx.x];
     if (!cs[t.chan].pbuf) return;
     const ChanDesc& d = cd[t.chan];
     const int x = t.x0 + threadIdx.y, y = t.y0 + threadIdx.x;
@@ -266,43 +337,43 @@ __global__ void k_pbuf_reset(int nchan, ChanState* __restrict__ cs) {
 
 // ---------------------------------------------------------------- variants
 
-template <int OXM, int OYM, int K>
-static void launch_variant(hipStream_t st, int ntb, const ChanDesc* cd, ChanState* cs,
-                           const TbTile* tiles, const uint8_t* flags, const double* rhs,
-                           double* pa, double* pb, double* part, int32_t* counters,
-                           int32_t* active, int check) {
-    k_rb_tb<OXM, OYM, K, 256><<<ntb, 256, 0, st>>>(cd, cs, tiles, flags, rhs, pa, pb, part,
-                                                    counters, active, check);
-}
+// (index, K sweeps per launch, R rows per thread, HY pairs per row, NS strips)
+// loaded region (NS*R) x (2*HY); output tile (NS*R - 2H) x (2*HY - 2H) with
+// H = 2K + 4; LDS NS*R*2*HY*8 bytes; workgroup NS*HY threads.
+#define STM_TB_VARIANTS(V)   \
+    V(0, 1, 4, 32, 16)       \
+    V(1, 2, 4, 32, 16)       \
+    V(2, 2, 4, 64, 16)       \
+    V(3, 4, 4, 32, 16)       \
+    V(4, 4, 4, 64, 16)       \
+    V(5, 4, 6, 64, 16)       \
+    V(6, 8, 6, 64, 16)       \
+    V(7, 2, 2, 64, 16)
 
-// Instantiated variants: sweeps per launch K and the largest output tile.
-const TbVariant TB_VARIANTS[] = {
-    {1, 64, 64}, {2, 64, 64}, {4, 64, 64}, {8, 48, 48},
-    {2, 32, 64}, {4, 32, 64}, {4, 48, 96}, {8, 32, 64},
-};
+#define V_ENTRY(i, K, R, HY, NS) {K, NS * R - 2 * (2 * K + 4), 2 * HY - 2 * (2 * K + 4)},
+const TbVariant TB_VARIANTS[] = {STM_TB_VARIANTS(V_ENTRY)};
+#undef V_ENTRY
 const int N_TB_VARIANTS = sizeof(TB_VARIANTS) / sizeof(TB_VARIANTS[0]);
 
 void launch_rb_tb(hipStream_t st, int variant, int ntb, const ChanDesc* cd, ChanState* cs,
                   const TbTile* tiles, const uint8_t* flags, const double* rhs, double* pa,
-                  double* pb, double* part, int32_t* counters, int32_t* active, int check) {
-#define V(i, K, X, Y) \
-    case i: launch_variant<X, Y, K>(st, ntb, cd, cs, tiles, flags, rhs, pa, pb, part, counters, active, check); break;
+                  double* pb, const double* part_prev, double* part_cur, int32_t* active,
+                  int L, int mode) {
+#define V_CASE(i, K, R, HY, NS)                                                              \
+    case i:                                                                                  \
+        k_rb_strip<K, R, HY, NS><<<ntb, NS * HY, 0, st>>>(cd, cs, tiles, flags, rhs, pa, pb, \
+                                                          part_prev, part_cur, active, L,    \
+                                                          mode);                             \
+        break;
     switch (variant) {
-        V(0, 1, 64, 64)
-        V(1, 2, 64, 64)
-        V(2, 4, 64, 64)
-        V(3, 8, 48, 48)
-        V(4, 2, 32, 64)
-        V(5, 4, 32, 64)
-        V(6, 4, 48, 96)
-        V(7, 8, 32, 64)
+        STM_TB_VARIANTS(V_CASE)
     default: break;
     }
-#undef V
+#undef V_CASE
 }
 
 void launch_pbuf_fix(hipStream_t st, int ntiles, int nchan, const ChanDesc* cd, ChanState* cs,
-                     const Tile* tiles, const double* pb, double* pa) {
+                     const stm::Tile* tiles, const double* pb, double* pa) {
     k_pbuf_fix<<<ntiles, dim3(TILE_Y, TILE_X), 0, st>>>(cd, cs, tiles, pb, pa);
     k_pbuf_reset<<<(nchan + 63) / 64, 64, 0, st>>>(nchan, cs);
 }

@@ -387,6 +387,22 @@ __global__ void k_advance_time(int nchan, ChanState* __restrict__ cs)
     cs[c].iterations += 1;
 }
 
+// Plain streaming copy (16 B per lane per step): the bandwidth probe and the
+// known-size reference for hardware byte counters.
+__global__ void __launch_bounds__(256)
+k_copy(int64_t n2, const double2* __restrict__ src, double2* __restrict__ dst)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n2; k += stride)
+        dst[k] = src[k];
+}
+
+void launch_copy(hipStream_t st, int64_t bytes, const double* src, double* dst) {
+    const int64_t n2 = bytes / 16;
+    k_copy<<<256 * 16, 256, 0, st>>>(n2, reinterpret_cast<const double2*>(src),
+                                     reinterpret_cast<double2*>(dst));
+}
+
 __global__ void __launch_bounds__(256)
 k_fill(int64_t n, double* __restrict__ a, double val)
 {

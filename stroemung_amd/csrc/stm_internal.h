@@ -57,6 +57,8 @@ struct ChanState {
     int32_t active;              // red-black solve still running
     int32_t exhausted;           // last solve hit itermax
     int32_t pbuf;                // blocked solver: latest p is in the second buffer
+    int32_t stop_launch;         // blocked solver: launch whose result ended the solve, -1 running
+    int32_t pad;
 };
 
 // One boundary cell (the reference's sorted_boundary_list entry,

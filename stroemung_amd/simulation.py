@@ -156,6 +156,12 @@ class Context:
         self._check(lib().stm_solver_variant(self._ctx, variant, *[C.byref(o) for o in out]))
         return tuple(o.value for o in out)
 
+    def stream_copy(self, nbytes, reps=20):
+        """Measured device copy bandwidth, GB/s (read + write bytes)."""
+        g = C.c_double()
+        self._check(lib().stm_stream_copy(self._ctx, nbytes, reps, C.byref(g)))
+        return g.value
+
     def autotune(self, launches=8):
         chosen = C.c_int32()
         us = C.c_double()

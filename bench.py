@@ -31,6 +31,9 @@ HBM_PEAK_GBS = 8000.0
 # reads own p, other p, own rhs and writes own p at 4 B/cell, plus 1 B flags
 SWEEP_BYTES_PER_CELL = 33.0
 COPY_PROBE_BYTES = 256 << 20
+# template arguments (R, HY, NS) of each blocked-solver variant, after K
+VARIANT_SHAPES = {0: "4, 32, 16", 1: "4, 32, 16", 2: "4, 64, 16", 3: "4, 32, 16",
+                  4: "4, 64, 16", 5: "6, 64, 16", 6: "6, 64, 16", 7: "2, 64, 16"}
 
 
 def parse():
@@ -152,6 +155,13 @@ def main():
     value = cells * a.steps / elapsed
 
     kname = "rb_tb"
+    kernel_id = "k_rb_strip<%d, %s>" % (ksweeps, VARIANT_SHAPES.get(variant, "?"))
+    traffic = None
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as fh:
+            traffic = json.load(fh).get(kernel_id, {}).get("traffic_bytes_per_launch")
+    except OSError:
+        pass
     ms, timed, launches = ctx.kernel_stats(kname)
     avg_s = ms / 1e3 / max(timed, 1)
     # one launch advances every interior cell by ksweeps red-black sweeps
@@ -192,13 +202,13 @@ def main():
                                      if st["tolerance_squared"] else None),
         },
         "roofline": {
-            "kernel": "k_rb_strip (blocked red-black SOR launch)",
+            "kernel": kernel_id,
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS if achieved else None,
-            "traffic": None,
+            "traffic": traffic,
             "bytes_per_launch": bytes_per_launch,
             "avg_launch_us": avg_s * 1e6,
             "timed_launches": timed,

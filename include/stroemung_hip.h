@@ -83,7 +83,9 @@ typedef struct stm_state {
     double initial_norm_squared;
     double norm_squared;          /* value returned by the last tick */
     double residual_squared;      /* RB: sum of r^2 over interior fluid cells */
-    double tolerance_squared;     /* RB: eps^2 * sum of rhs^2 over interior fluid */
+    double tolerance_squared;     /* RB: eps^2 * sum of (rhs - mean)^2 over interior fluid */
+    double rhs_mean;              /* RB: mean rhs over interior fluid cells; the solve is for
+                                   * rhs - mean (the all-Neumann problem needs zero mean) */
     double pressure_range[2];     /* SimulationGrid::pressure_range */
     double speed_range[2];        /* SimulationGrid::speed_range */
     double umax, vmax;            /* max |u|, |v| over fluid-cell faces */

@@ -426,8 +426,10 @@ int orc_solve_sor(orc_sim *s, uint32_t *iters, double *norm2)
                                   + ((p_i_j_p1 + p_i_j_m1) / dely2)) - s->rhs[i]);
             }
         if (s->masked_stop) {
-            /* extension: the red-black solver's fluid-masked relative test */
+            /* extension: the red-black solver's fluid-masked relative test,
+             * on the unprojected rhs this solver iterates on */
             double rsum2 = 0.0;
+            s->rhs_mean = 0.0;
             double res2 = orc_masked_residual_sum(s, &rsum2);
             norm_squared = res2 / s->fluid_cells;
             s->last_res2 = norm_squared;
@@ -591,8 +593,28 @@ void orc_rb_half_sweep(orc_sim *s, int colour)
             int i = x * ny + y;
             if (s->cell_type[i] != ORC_FLUID) continue;
             double t = fma(p[i + ny] + p[i - ny], c.rdx2, (p[i + 1] + p[i - 1]) * c.rdy2);
-            p[i] = fma(c.omw, p[i], c.mid * (t - s->rhs[i]));
+            p[i] = fma(c.omw, p[i], c.mid * (t - (s->rhs[i] - s->rhs_mean)));
         }
+}
+
+/* Mean of rhs over interior fluid cells, summed x-major from 0.0.  The
+ * pressure problem has Neumann conditions everywhere (the ghost copy treats
+ * every boundary as a wall), so it is solvable only for rhs of zero mean;
+ * with inflow and outflow the mean is not zero and no iteration can drive
+ * the residual below it.  The red-black solver therefore solves for
+ * rhs - mean. */
+double orc_rhs_mean(const orc_sim *s)
+{
+    double acc = 0.0;
+    long n = 0;
+    for (int x = 1; x < s->nx - 1; x++)
+        for (int y = 1; y < s->ny - 1; y++) {
+            int i = x * s->ny + y;
+            if (s->cell_type[i] != ORC_FLUID) continue;
+            acc = acc + s->rhs[i];
+            n++;
+        }
+    return n ? acc / (double)n : 0.0;
 }
 
 /* One sweep = ghost refresh of every boundary cell, red half, black half. */
@@ -623,9 +645,10 @@ double orc_masked_residual_sum(orc_sim *s, double *rhs_sum2)
             if (s->cell_type[i] != ORC_FLUID) continue;
             double a = (p[i + ny] - p[i]) - (p[i] - p[i - ny]);
             double b = (p[i + 1] - p[i]) - (p[i] - p[i - 1]);
-            double r = fma(a, c.rdx2, b * c.rdy2) - s->rhs[i];
+            double rh = s->rhs[i] - s->rhs_mean;
+            double r = fma(a, c.rdx2, b * c.rdy2) - rh;
             acc += r * r;
-            racc += s->rhs[i] * s->rhs[i];
+            racc += rh * rh;
         }
     if (rhs_sum2) *rhs_sum2 = racc;
     return acc;
@@ -641,6 +664,7 @@ int orc_solve_rb(orc_sim *s, uint32_t *iters, double *norm2)
 {
     int every = s->check_every > 0 ? s->check_every : 1;
     double rsum2 = 0.0;
+    if (!s->fix_mean) s->rhs_mean = orc_rhs_mean(s);
     double res2 = orc_masked_residual_sum(s, &rsum2);
     double tol2 = sq(s->sor_absolute_epsilon) * rsum2;
     int it = 0;

@@ -64,6 +64,8 @@ typedef struct {
     int32_t solver;        /* ORC_SOLVER_LEX (reference) or ORC_SOLVER_RB */
     int32_t check_every;   /* RB: sweeps between residual checks */
     int32_t masked_stop;   /* LEX: stop on the RB criterion instead of the reference's */
+    int32_t fix_mean;      /* RB: use rhs_mean as given instead of computing it */
+    double rhs_mean;       /* RB: mean of rhs over interior fluid cells (projection) */
     /* fields, nx*ny each */
     double *u, *v, *p, *f, *g, *rhs;
     const uint8_t *cell_type;   /* ORC_* */
@@ -119,6 +121,7 @@ void orc_ghost_refresh(orc_sim *s, int colour);       /* colour -1: all */
 void orc_rb_half_sweep(orc_sim *s, int colour);
 void orc_rb_sweeps(orc_sim *s, int nsweeps);
 double orc_masked_residual_sum(orc_sim *s, double *rhs_sum2);
+double orc_rhs_mean(const orc_sim *s);
 int orc_solve_rb(orc_sim *s, uint32_t *iters, double *norm2);
 
 #ifdef __cplusplus

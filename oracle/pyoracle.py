@@ -54,6 +54,8 @@ class OrcSim(C.Structure):
         ("solver", C.c_int32),
         ("check_every", C.c_int32),
         ("masked_stop", C.c_int32),
+        ("fix_mean", C.c_int32),
+        ("rhs_mean", C.c_double),
         ("u", C.POINTER(C.c_double)), ("v", C.POINTER(C.c_double)),
         ("p", C.POINTER(C.c_double)), ("f", C.POINTER(C.c_double)),
         ("g", C.POINTER(C.c_double)), ("rhs", C.POINTER(C.c_double)),
@@ -117,6 +119,8 @@ def _declare(L):
     L.orc_rb_sweeps.argtypes = [P, C.c_int]
     L.orc_masked_residual_sum.argtypes = [P, _D]
     L.orc_masked_residual_sum.restype = C.c_double
+    L.orc_rhs_mean.argtypes = [P]
+    L.orc_rhs_mean.restype = C.c_double
 
 
 def _dp(a):

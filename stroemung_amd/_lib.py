@@ -48,7 +48,8 @@ class State(C.Structure):
     _fields_ = [
         ("delt", C.c_double), ("time", C.c_double), ("initial_norm_squared", C.c_double),
         ("norm_squared", C.c_double), ("residual_squared", C.c_double),
-        ("tolerance_squared", C.c_double), ("pressure_range", C.c_double * 2),
+        ("tolerance_squared", C.c_double), ("rhs_mean", C.c_double),
+        ("pressure_range", C.c_double * 2),
         ("speed_range", C.c_double * 2), ("umax", C.c_double), ("vmax", C.c_double),
         ("iterations", C.c_uint32), ("sor_iterations", C.c_uint32),
         ("has_initial_norm_squared", C.c_int32), ("exhausted", C.c_int32),

@@ -36,6 +36,7 @@ struct HostChan {
     std::vector<int8_t> edge;
     std::vector<uint32_t> blist;
     double fluid_cells = 0.0;
+    double n_inner = 0.0;          // interior fluid cells
     int32_t tile0 = 0, ntiles = 0, boff = 0;
     int32_t tb0 = 0, ntb = 0;      // blocked-solver tiles of the current variant
 };
@@ -183,6 +184,10 @@ int classify_chan(stm_ctx* ctx, HostChan& c) {
                             &c.fluid_cells, err, sizeof err);
     if (rc != STM_OK) return fail(ctx, rc, "%s", err);
     c.blist.resize(nb);
+    uint64_t inner = 0;
+    for (uint32_t x = 1; x + 1 < c.nx; x++)
+        for (uint32_t y = 1; y + 1 < c.ny; y++) inner += c.type[(size_t)x * c.ny + y] == STM_FLUID;
+    c.n_inner = (double)inner;
     return STM_OK;
 }
 
@@ -210,6 +215,7 @@ ChanDesc make_desc(const HostChan& c) {
     d.eps = q.sor_absolute_epsilon;
     d.tau = q.tau;
     d.fluid_cells = c.fluid_cells;
+    d.n_inner = c.n_inner;
     // The same expressions as the oracle (and src/simulation.rs:240-246).
     d.delx2 = q.delx * q.delx;
     d.dely2 = q.dely * q.dely;
@@ -524,7 +530,7 @@ int stage_fg(stm_ctx* ctx) {
 
 int stage_rhs(stm_ctx* ctx) {
     LAUNCH("rhs", launch_rhs(ctx->stream, ctx->ntiles, ctx->d_cd, ctx->d_cs, ctx->d_tiles,
-                             ctx->flags, ctx->f, ctx->g, ctx->rhs, ctx->d_part));
+                             ctx->f, ctx->g, ctx->rhs));
     return STM_OK;
 }
 
@@ -570,9 +576,19 @@ int blocked_solve(stm_ctx* ctx, int check) {
     return STM_OK;
 }
 
-// POISSON: sets up tol2 and the per-channel flags from the rhs^2 partials
-// left by stage_rhs, then runs each channel's solver.
+// Mean of rhs and the sum of (rhs - mean)^2 over interior fluid cells.
+int stage_rhs_stats(stm_ctx* ctx) {
+    LAUNCH("rhs_stats", launch_rhs_stats(ctx->stream, ctx->ntiles, nch(ctx), ctx->d_cd,
+                                         ctx->d_cs, ctx->d_tiles, ctx->flags, ctx->rhs,
+                                         ctx->d_part));
+    return STM_OK;
+}
+
+// POISSON: the rhs statistics, tol2 and the per-channel flags, then each
+// channel's solver.
 int stage_solve(stm_ctx* ctx) {
+    int rc = stage_rhs_stats(ctx);
+    if (rc) return rc;
     LAUNCH("solve_begin", launch_solve_begin(ctx->stream, nch(ctx), ctx->d_cd, ctx->d_cs,
                                              ctx->d_part, ctx->d_active));
     if (ctx->any_lex)
@@ -835,6 +851,7 @@ int stm_get_state(stm_ctx* ctx, uint32_t ch, stm_state* out) {
     o.norm_squared = s.norm2;
     o.residual_squared = s.res2;
     o.tolerance_squared = s.tol2;
+    o.rhs_mean = s.rhs_mean;
     o.pressure_range[0] = s.pressure_range[0];
     o.pressure_range[1] = s.pressure_range[1];
     o.speed_range[0] = s.speed_range[0];
@@ -907,8 +924,7 @@ int stm_calculate_rhs(stm_ctx* ctx) {
 int stm_solve_pressure(stm_ctx* ctx) {
     int rc = ready(ctx);
     if (rc) return rc;
-    // the rhs^2 partials of the criterion come from the RHS pass
-    if ((rc = stage_rhs(ctx)) || (rc = stage_solve(ctx))) return rc;
+    if ((rc = stage_solve(ctx))) return rc;
     if ((rc = stage_ranges(ctx, 2))) return rc;
     return read_states(ctx);
 }
@@ -966,6 +982,7 @@ int stm_rb_sweeps(stm_ctx* ctx, uint32_t n) {
     if (rc) return rc;
     if (n == 0) return STM_OK;
     std::vector<ChanState> saved;
+    if ((rc = stage_rhs_stats(ctx))) return rc;     // the projection uses the current rhs
     if ((rc = arm_channels(ctx, (int)n, false, &saved))) return rc;
     if ((rc = blocked_solve(ctx, 0))) return rc;
     // restore the scalar state, keep the new p

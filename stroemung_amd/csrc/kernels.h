@@ -12,8 +12,7 @@ void launch_bc_velocity(hipStream_t, int nchan, const ChanDesc*, const BEntry*, 
 void launch_fg(hipStream_t, int ntiles, const ChanDesc*, const ChanState*, const Tile*,
                const uint8_t* flags, const double* u, const double* v, double* f, double* g);
 void launch_rhs(hipStream_t, int ntiles, const ChanDesc*, const ChanState*, const Tile*,
-                const uint8_t* flags, const double* f, const double* g, double* rhs,
-                double* part);
+                const double* f, const double* g, double* rhs);
 void launch_adap_uv(hipStream_t, int ntiles, const ChanDesc*, const ChanState*, const Tile*,
                     const double* f, const double* g, const double* p, double* u, double* v);
 void launch_restore(hipStream_t, int nbtot, const BEntry*, const double2* restore, double* u,
@@ -29,6 +28,9 @@ void launch_copy(hipStream_t, int64_t bytes, const double* src, double* dst);
 // kernels_pressure.hip
 void launch_ghost(hipStream_t, int nbtot, const BEntry*, const ChanState*, double* p, int colour,
                   int only_active);
+// rhs_mean and the sum of (rhs - mean)^2 per channel (partials in slots 0, 1)
+void launch_rhs_stats(hipStream_t, int ntiles, int nchan, const ChanDesc*, ChanState*,
+                      const Tile*, const uint8_t* flags, const double* rhs, double* part);
 void launch_solve_begin(hipStream_t, int nchan, const ChanDesc*, ChanState*, const double* part,
                         int32_t* active);
 void launch_initial_norm(hipStream_t, int nchan, const ChanDesc*, ChanState*, const double* rhs,

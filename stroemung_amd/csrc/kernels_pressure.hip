@@ -59,14 +59,59 @@ __device__ double chan_sum(const ChanDesc& d, const double* __restrict__ part, i
     return sh[0];
 }
 
-// Start of a red-black solve: tol2 = eps^2 * sum(rhs^2) over interior fluid.
+// Per-tile sums over interior fluid cells: pass 0 sum of rhs (slot 0),
+// pass 1 sum of (rhs - mean)^2 (slot 1).
+__global__ void __launch_bounds__(TILE_THREADS)
+k_rhs_sums(const ChanDesc* __restrict__ cd, const ChanState* __restrict__ cs,
+           const Tile* __restrict__ tiles, const uint8_t* __restrict__ flags,
+           const double* __restrict__ rhs, double* __restrict__ part, int pass)
+{
+    __shared__ double sh[TILE_THREADS];
+    const Tile t = tiles[blockIdx.x];
+    const ChanDesc& d = cd[t.chan];
+    const int x = t.x0 + threadIdx.y, y = t.y0 + threadIdx.x;
+    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+    double v = 0.0;
+    if (x >= 1 && x <= d.nx - 2 && y >= 1 && y <= d.ny - 2) {
+        const int64_t i = d.off + x * (int64_t)d.pitch + y;
+        if (flag_fluid(flags[i])) {
+            const double r = pass ? rhs[i] - cs[t.chan].rhs_mean : rhs[i];
+            v = pass ? r * r : r;
+        }
+    }
+    sh[tid] = v;
+    __syncthreads();
+    for (int st = TILE_THREADS / 2; st > 0; st >>= 1) {
+        if (tid < st) sh[tid] = sh[tid] + sh[tid + st];
+        __syncthreads();
+    }
+    if (tid == 0) part[blockIdx.x * NPART + pass] = sh[0];
+}
+
+// The mean of rhs over interior fluid cells.  The pressure problem has
+// Neumann conditions all round (every boundary's ghost copies its fluid
+// neighbour), so it is solvable only for rhs of zero mean; with inflow and
+// outflow the mean is not zero and no iteration can push the residual below
+// it.  The red-black solver solves for rhs - mean.
+__global__ void __launch_bounds__(256)
+k_rhs_mean(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
+           const double* __restrict__ part)
+{
+    __shared__ double sh[256];
+    const ChanDesc& d = cd[blockIdx.x];
+    const double sum = chan_sum(d, part, 0, sh);
+    if (threadIdx.x == 0) cs[blockIdx.x].rhs_mean = d.n_inner > 0.0 ? sum / d.n_inner : 0.0;
+}
+
+// Start of a red-black solve: tol2 = eps^2 * sum((rhs - mean)^2) over
+// interior fluid cells.
 __global__ void __launch_bounds__(256)
 k_solve_begin(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
               const double* __restrict__ part, int32_t* __restrict__ active_out)
 {
     __shared__ double sh[256];
     const ChanDesc& d = cd[blockIdx.x];
-    const double rsum2 = chan_sum(d, part, 0, sh);
+    const double rsum2 = chan_sum(d, part, 1, sh);
     if (threadIdx.x == 0) {
         ChanState& s = cs[blockIdx.x];
         const bool rb = d.solver == SOLVER_RB;
@@ -201,6 +246,12 @@ void launch_ghost(hipStream_t st, int nbtot, const BEntry* be, const ChanState* 
                   int colour, int only_active) {
     if (nbtot > 0)
         k_ghost<<<(nbtot + 255) / 256, 256, 0, st>>>(nbtot, be, cs, p, colour, only_active);
+}
+void launch_rhs_stats(hipStream_t st, int ntiles, int nchan, const ChanDesc* cd, ChanState* cs,
+                      const Tile* tiles, const uint8_t* flags, const double* rhs, double* part) {
+    k_rhs_sums<<<ntiles, dim3(TILE_Y, TILE_X), 0, st>>>(cd, cs, tiles, flags, rhs, part, 0);
+    k_rhs_mean<<<nchan, 256, 0, st>>>(cd, cs, part);
+    k_rhs_sums<<<ntiles, dim3(TILE_Y, TILE_X), 0, st>>>(cd, cs, tiles, flags, rhs, part, 1);
 }
 void launch_solve_begin(hipStream_t st, int nchan, const ChanDesc* cd, ChanState* cs,
                         const double* part, int32_t* active) {

@@ -231,7 +231,9 @@ k_rb_strip(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
     const int lx0 = strip * R;
     const int gy = ey0 + 2 * m;                  // even: (gy, gy+1) is one aligned pair
 
-    // ---- load (issued before the stop decision so the latencies overlap)
+    // ---- load (issued before the stop decision so the latencies overlap);
+    // the solve is for rhs - mean (the all-Neumann problem needs zero mean)
+    const double mean = s.rhs_mean;
     double pv[R][2], rr[R][2];
     uint8_t cc[R][2];
 #pragma unroll
@@ -253,8 +255,8 @@ k_rb_strip(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
         }
         pv[r][0] = pp.x;
         pv[r][1] = pp.y;
-        rr[r][0] = rh.x;
-        rr[r][1] = rh.y;
+        rr[r][0] = rh.x - mean;
+        rr[r][1] = rh.y - mean;
 #pragma unroll
         for (int q = 0; q < 2; q++) {
             const uint8_t fl = q ? f1 : f0;

@@ -208,28 +208,20 @@ __device__ __forceinline__ double block_sum(double val, double* sh) {
     return r;
 }
 
-// src/simulation.rs:204-214: rhs on x in [1, nx-1], y in [1, ny-1].  Also the
-// per-tile sum of rhs^2 over interior fluid cells (red-black criterion).
+// src/simulation.rs:204-214: rhs on x in [1, nx-1], y in [1, ny-1].
 __global__ void __launch_bounds__(TILE_THREADS)
 k_rhs(const ChanDesc* __restrict__ cd, const ChanState* __restrict__ cs,
-      const Tile* __restrict__ tiles, const uint8_t* __restrict__ flags,
-      const double* __restrict__ f, const double* __restrict__ g, double* __restrict__ rhs,
-      double* __restrict__ part)
+      const Tile* __restrict__ tiles, const double* __restrict__ f,
+      const double* __restrict__ g, double* __restrict__ rhs)
 {
-    __shared__ double sh[TILE_THREADS];
     const Tile t = tiles[blockIdx.x];
     const ChanDesc& d = cd[t.chan];
     const int x = t.x0 + threadIdx.y, y = t.y0 + threadIdx.x;
-    double r2 = 0.0;
     if (x >= 1 && x < d.nx && y >= 1 && y < d.ny) {
         const int64_t P = d.pitch, i = d.off + x * P + y;
-        const double r = (((f[i] - f[i - P]) / d.delx) + ((g[i] - g[i - 1]) / d.dely))
-                         / cs[t.chan].delt;
-        rhs[i] = r;
-        if (x <= d.nx - 2 && y <= d.ny - 2 && flag_fluid(flags[i])) r2 = r * r;
+        rhs[i] = (((f[i] - f[i - P]) / d.delx) + ((g[i] - g[i - 1]) / d.dely))
+                 / cs[t.chan].delt;
     }
-    const double s = block_sum(r2, sh);
-    if (threadIdx.x == 0 && threadIdx.y == 0) part[blockIdx.x * NPART + 0] = s;
 }
 
 // ---------------------------------------------------------------- ADAP_UV
@@ -424,9 +416,8 @@ void launch_fg(hipStream_t st, int ntiles, const ChanDesc* cd, const ChanState* 
     k_fg<<<ntiles, TB, 0, st>>>(cd, cs, tiles, flags, u, v, f, g);
 }
 void launch_rhs(hipStream_t st, int ntiles, const ChanDesc* cd, const ChanState* cs,
-                const Tile* tiles, const uint8_t* flags, const double* f, const double* g,
-                double* rhs, double* part) {
-    k_rhs<<<ntiles, TB, 0, st>>>(cd, cs, tiles, flags, f, g, rhs, part);
+                const Tile* tiles, const double* f, const double* g, double* rhs) {
+    k_rhs<<<ntiles, TB, 0, st>>>(cd, cs, tiles, f, g, rhs);
 }
 void launch_adap_uv(hipStream_t st, int ntiles, const ChanDesc* cd, const ChanState* cs,
                     const Tile* tiles, const double* f, const double* g, const double* p,

@@ -35,6 +35,7 @@ struct ChanDesc {
     int32_t tb0, ntb;            // slice of the blocked-solver tile list
     double delx, dely, gamma, reynolds, omega, eps, tau;
     double fluid_cells;
+    double n_inner;              // interior fluid cells, x in [1, nx-2], y in [1, ny-2]
     // derived solver constants, computed on the host exactly as the oracle does
     double delx2, dely2;         // sq(delx), sq(dely)
     double omw, mid;             // 1 - omega, omega / ((2/dx2) + (2/dy2))
@@ -49,6 +50,7 @@ struct ChanState {
     double init_norm2;           // Simulation::initial_norm_squared
     double norm2;                // last solve's returned norm
     double res2, tol2, rhs2;     // red-black: sums over interior fluid cells
+    double rhs_mean;             // red-black: mean rhs over interior fluid cells
     double umax, vmax;           // max |u|, |v| over fluid faces (for delt)
     double pressure_range[2], speed_range[2];
     uint32_t iterations;         // ticks done

@@ -202,6 +202,12 @@ def test_rb_sweeps_bit_exact(seed, nx, ny, variant):
     assert ctx.solver_variant(variant)[0] == variant
     assert osim.call("rebuild_boundary_list") == 0
     ctx.stage("rb_sweeps", 7)
+    # the sweeps use rhs - mean; the GPU's mean is a tree sum, the oracle's a
+    # sequential one: equal to rounding, and the sweeps are compared on the
+    # GPU's value
+    mean = ctx.state(ch)["rhs_mean"]
+    assert mean == pytest.approx(O.lib().orc_rhs_mean(osim.s), rel=1e-12, abs=1e-15)
+    osim.set(rhs_mean=mean)
     osim.call("rb_sweeps", 7)
     _cmp(ctx, ch, osim, ("p",), "7 sweeps")
     ctx.stage("rb_sweeps", 12)
@@ -256,3 +262,46 @@ def test_batch_equals_single_channels():
         a, b = batch.download(k), c.download(0)
         for n in a:
             assert_bits_equal(a[n], b[n], "channel %d %s" % (k, n))
+
+
+# ---------------------------------------------- trajectories at equal residual
+
+def _rel(a, b, mask):
+    return float(np.linalg.norm((a - b)[mask]) / np.linalg.norm(b[mask]))
+
+
+TRAJ = [
+    ("square obstacle channel 100x20", lambda: presets.square_obstacle(), 20,
+     dict(delx=0.1, dely=0.1, delt=0.005, gamma=0.9, reynolds=100.0, omega=1.7)),
+    ("lid cavity 64x64, adaptive step", lambda: presets.lid_cavity(64, 64), 10,
+     dict(delx=1 / 64, dely=1 / 64, delt=0.0, tau=0.5, gamma=0.9, reynolds=1000.0, omega=1.8)),
+]
+
+
+@pytest.mark.parametrize("name,grid,steps,kw", TRAJ, ids=[t[0] for t in TRAJ])
+def test_rb_trajectory_within_1e6_of_converged_oracle(name, grid, steps, kw):
+    """u, v and p after N ticks of the GPU red-black path at eps = 1e-6 match
+    the FP64 oracle solved to 1e-12 within 1e-6 relative L2 (p up to its
+    fluid mean; the pressure problem is pure Neumann)."""
+    say("test: trajectory", name)
+    ct, vel = grid()
+    fl = ct == O.FLUID
+    params = make_params(kw["delx"], kw["dely"], kw["delt"], kw["gamma"], kw["reynolds"], 1e-6,
+                         200000, kw["omega"], tau=kw.get("tau", 0.0), solver="rb",
+                         check_every=8)
+    ctx = Context(0)
+    ch = ctx.add_grid(params, ct, vel)
+    ctx.construct()
+    gpu_its = [ctx.tick()[0][ch] for _ in range(steps)]
+    got = ctx.download(ch)
+    truth = O.Sim(ct, vel, sor_absolute_epsilon=1e-12, max_iterations=400000,
+                  solver=O.SOLVER_RB, check_every=8, **kw).construct()
+    for _ in range(steps):
+        it, _ = truth.tick()
+        assert it < 400000
+    assert max(gpu_its) < 200000
+    pm = lambda p: np.where(fl, p - p[fl].mean(), 0.0)  # noqa: E731
+    errs = (_rel(got["u"], truth.u, fl), _rel(got["v"], truth.v, fl),
+            _rel(pm(got["p"]), pm(truth.p), fl))
+    say("  rel L2 vs 1e-12 oracle: u %.2e v %.2e p %.2e; sweeps/tick %s" % (errs + (gpu_its,)))
+    assert max(errs) < 1e-6

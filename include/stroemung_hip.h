@@ -192,6 +192,78 @@ int stm_kernel_stats(stm_ctx *ctx, const char *name, double *total_ms, uint64_t 
                      uint64_t *launches);
 int stm_reset_stats(stm_ctx *ctx);
 
+/* ---- the pressure network (no reference counterpart)
+ * A 4-level U-Net (32 base channels, 3x3 convolutions, ReLU, 2x2 max pooling,
+ * nearest upsampling with skip concatenation, 1x1 head) in BF16 with FP32
+ * accumulation on the matrix cores and FP32 master weights trained with Adam.
+ * It maps the normalised residual of the pressure equation (channel 0) and the
+ * interior-fluid mask (channel 1) of an H x W grid to a correction of p.  H
+ * and W are multiples of 8; interior cell (x, y) of a channel is pixel
+ * (x - 1, y - 1), the rest of the H x W frame is zero.  The network lives on
+ * the context's device and stream. */
+typedef struct stm_net stm_net;
+
+typedef struct stm_net_stats_t {
+    double forward_ms;            /* event time of the forward passes inside solves */
+    double train_ms;              /* event time of stm_net_train */
+    double flops_forward_per_sample;
+    uint64_t forward_calls, forward_samples;
+    uint64_t train_steps, train_samples;
+    uint64_t params;
+    uint32_t dataset_size;
+    uint32_t pad;
+} stm_net_stats_t;
+
+/* batch = capacity: the largest batch of any call (and the channel count of a
+ * context that uses the net).  seed drives the He initialisation. */
+int stm_net_create(stm_ctx *ctx, uint32_t height, uint32_t width, uint32_t batch, uint64_t seed,
+                   stm_net **out);
+void stm_net_destroy(stm_net *net);
+const char *stm_net_last_error(const stm_net *net);
+int stm_net_param_count(stm_net *net, uint64_t *count);
+/* FP32 master parameters: per convolution W[cout][9][cin] (tap 3*(dh+1) +
+ * (dw+1) weighs the input pixel at offset dh along x and dw along y) then
+ * b[cout], padded to a multiple of 8; then the head's 32 weights and its
+ * bias (padded to 8). */
+int stm_net_get_params(stm_net *net, float *out);
+int stm_net_set_params(stm_net *net, const float *in);
+/* x: nb x H x W x 2 (fp32) -> y: nb x H x W (fp32) */
+int stm_net_forward(stm_net *net, uint32_t nb, const float *x, float *y);
+/* The masked relative squared error of the forward pass against t
+ * (nb x H x W): loss = mean_s sum m (y - t)^2 / sum m t^2 with m = x[..., 1],
+ * and its gradient with respect to every parameter (layout as above). */
+int stm_net_gradients(stm_net *net, uint32_t nb, const float *x, const float *t, float *loss,
+                      float *grads);
+/* Device-resident training set of `capacity` samples, filled by capture. */
+int stm_net_dataset_reserve(stm_net *net, uint32_t capacity);
+int stm_net_dataset_size(stm_net *net, uint32_t *size);
+/* `steps` Adam steps on random batches of the training set; *last_loss is the
+ * mean loss of the last 8 steps. */
+int stm_net_train(stm_net *net, uint32_t steps, uint32_t batch, float lr, uint64_t seed,
+                  float *last_loss);
+int stm_net_stats(stm_net *net, stm_net_stats_t *out);
+/* Forward passes (and training steps when train_steps > 0) on the buffers
+ * already resident, for timing only: ms per forward of nb samples and per
+ * training step of nb samples. */
+int stm_net_benchmark(stm_net *net, uint32_t nb, uint32_t reps, double *fwd_ms,
+                      double *train_ms);
+
+/* ---- learned pressure solve (no reference counterpart)
+ * With a net set, every red-black solve of the context runs `cycles` learned
+ * cycles first -- residual, network correction p += a y with the a that
+ * minimises the residual norm, then `sweeps` red-black sweeps -- and then the
+ * plain red-black solve to the same eps.  The stopping test and the FP64
+ * arithmetic of the sweeps are those of the plain solver, so the result meets
+ * the same residual.  net = NULL or cycles = 0 turns it off.  The net must
+ * come from this context and hold at least (nx - 2) x (ny - 2) pixels and
+ * one batch entry per channel. */
+int stm_set_learned(stm_ctx *ctx, stm_net *net, uint32_t cycles, uint32_t sweeps);
+/* With a net set, every red-black solve also stores two training pairs per
+ * channel in the net's training set (ring order): the residual at the start
+ * of the solve and after `sweeps` sweeps, each with the normalised pressure
+ * change the solve then made.  net = NULL turns it off. */
+int stm_set_capture(stm_ctx *ctx, stm_net *net, uint32_t sweeps);
+
 #ifdef __cplusplus
 }
 #endif

@@ -56,7 +56,19 @@ class State(C.Structure):
     ]
 
 
+class NetStats(C.Structure):
+    _fields_ = [
+        ("forward_ms", C.c_double), ("train_ms", C.c_double),
+        ("flops_forward_per_sample", C.c_double),
+        ("forward_calls", C.c_uint64), ("forward_samples", C.c_uint64),
+        ("train_steps", C.c_uint64), ("train_samples", C.c_uint64),
+        ("params", C.c_uint64), ("dataset_size", C.c_uint32), ("pad", C.c_uint32),
+    ]
+
+
 _D = C.POINTER(C.c_double)
+_F = C.POINTER(C.c_float)
+_NET = C.c_void_p
 _U8 = C.POINTER(C.c_uint8)
 _CTX = C.c_void_p
 
@@ -94,6 +106,22 @@ SIGNATURES = {
     "stm_kernel_stats": (C.c_int, [_CTX, C.c_char_p, _D, C.POINTER(C.c_uint64),
                                    C.POINTER(C.c_uint64)]),
     "stm_reset_stats": (C.c_int, [_CTX]),
+    "stm_net_create": (C.c_int, [_CTX, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64,
+                                 C.POINTER(C.c_void_p)]),
+    "stm_net_destroy": (None, [_NET]),
+    "stm_net_last_error": (C.c_char_p, [_NET]),
+    "stm_net_param_count": (C.c_int, [_NET, C.POINTER(C.c_uint64)]),
+    "stm_net_get_params": (C.c_int, [_NET, _F]),
+    "stm_net_set_params": (C.c_int, [_NET, _F]),
+    "stm_net_forward": (C.c_int, [_NET, C.c_uint32, _F, _F]),
+    "stm_net_gradients": (C.c_int, [_NET, C.c_uint32, _F, _F, _F, _F]),
+    "stm_net_dataset_reserve": (C.c_int, [_NET, C.c_uint32]),
+    "stm_net_dataset_size": (C.c_int, [_NET, C.POINTER(C.c_uint32)]),
+    "stm_net_train": (C.c_int, [_NET, C.c_uint32, C.c_uint32, C.c_float, C.c_uint64, _F]),
+    "stm_net_stats": (C.c_int, [_NET, C.POINTER(NetStats)]),
+    "stm_net_benchmark": (C.c_int, [_NET, C.c_uint32, C.c_uint32, _D, _D]),
+    "stm_set_learned": (C.c_int, [_CTX, _NET, C.c_uint32, C.c_uint32]),
+    "stm_set_capture": (C.c_int, [_CTX, _NET, C.c_uint32]),
 }
 
 _lib = None
@@ -114,10 +142,13 @@ def lib():
     return _lib
 
 
-def check(rc, ctx=None):
+def check(rc, ctx=None, net=None):
     if rc == STM_OK:
         return
-    msg = lib().stm_last_error(ctx).decode(errors="replace")
+    if net is not None:
+        msg = lib().stm_net_last_error(net).decode(errors="replace")
+    else:
+        msg = lib().stm_last_error(ctx).decode(errors="replace")
     if rc == -1:
         raise NoDeviceError(rc, msg)
     if rc == -3:

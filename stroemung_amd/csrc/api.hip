@@ -19,6 +19,7 @@
 
 #include "../../include/stroemung_hip.h"
 #include "kernels.h"
+#include "nn_net.h"
 #include "stm_internal.h"
 
 using namespace stm;
@@ -80,6 +81,15 @@ struct stm_ctx {
     int variant = -1;              // blocked-solver variant in use
     ChanState* h_cs = nullptr;     // pinned
     int be_capacity = 0;
+    // learned correction and capture of training pairs
+    stm_net* lnet = nullptr;       // learned cycles before the plain solve
+    int lcycles = 0, lsweeps = 0;
+    stm_net* cnet = nullptr;       // capture into this net's training set
+    int csweeps = 0;
+    LearnChan* d_lc = nullptr;
+    LearnChan* h_lc = nullptr;     // pinned
+    double *lres = nullptr, *lcf = nullptr, *lp0 = nullptr, *lpk = nullptr;
+    hipEvent_t nn_ev[2] = {nullptr, nullptr};
     // profiling
     uint32_t sample = 0;
     std::unordered_map<std::string, KStat> stats;
@@ -584,6 +594,156 @@ int stage_rhs_stats(stm_ctx* ctx) {
     return STM_OK;
 }
 
+// ------------------------------------------------------------- learned solve
+
+// Ghosts of p, the residual, and per channel: converged?  the input scale.
+int learn_prepare(stm_ctx* ctx, int first) {
+    LAUNCH("learn_ghost", launch_ghost(ctx->stream, ctx->nbtot, ctx->d_be, ctx->d_cs, ctx->p, -1,
+                                       0));
+    LAUNCH("learn_residual", launch_learn_residual(ctx->stream, ctx->ntiles, ctx->d_cd, ctx->d_cs,
+                                                   ctx->d_tiles, ctx->flags, ctx->rhs, ctx->p,
+                                                   ctx->lres, ctx->d_part));
+    LAUNCH("learn_scale", launch_learn_scale(ctx->stream, nch(ctx), ctx->d_cd, ctx->d_cs,
+                                             ctx->d_lc, ctx->d_part, first));
+    return STM_OK;
+}
+
+// Blocked red-black launches on the channels still running, `budget` sweeps
+// each (0 = up to itermax); final = 1 ends the solve.
+int learn_phase(stm_ctx* ctx, int budget, int final_phase) {
+    LAUNCH("learn_arm", launch_learn_arm(ctx->stream, nch(ctx), ctx->d_cd, ctx->d_cs, ctx->d_lc,
+                                         ctx->d_active, budget));
+    int rc = blocked_solve(ctx, 1);
+    if (rc) return rc;
+    LAUNCH("learn_collect", launch_learn_collect(ctx->stream, nch(ctx), ctx->d_cs, ctx->d_lc,
+                                                 final_phase));
+    return STM_OK;
+}
+
+int any_running(stm_ctx* ctx, bool* any) {
+    HIPCHK(ctx, hipMemcpyAsync(ctx->h_lc, ctx->d_lc, sizeof(LearnChan) * nch(ctx),
+                               hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    *any = false;
+    for (int k = 0; k < nch(ctx); k++) *any = *any || ctx->h_lc[k].run;
+    return check_launch(ctx);
+}
+
+// `lcycles` cycles of network correction and `lsweeps` sweeps, then the
+// plain solve to eps.
+int learned_solve(stm_ctx* ctx) {
+    Net& net = ctx->lnet->net;
+    const int n = nch(ctx);
+    int rc;
+    for (int cyc = 0; cyc < ctx->lcycles; cyc++) {
+        if ((rc = learn_prepare(ctx, cyc == 0))) return rc;
+        LAUNCH("learn_pack", launch_learn_pack(ctx->stream, n, ctx->d_cd, ctx->d_lc, ctx->flags,
+                                               ctx->lres, net.H, net.W, net.input(), nullptr, 0,
+                                               1));
+        HIPCHK(ctx, hipEventRecord(ctx->nn_ev[0], ctx->stream));
+        LAUNCH("nn_forward", net.forward(n));
+        HIPCHK(ctx, hipEventRecord(ctx->nn_ev[1], ctx->stream));
+        LAUNCH("learn_unpack", launch_learn_unpack(ctx->stream, ctx->ntiles, ctx->d_cd,
+                                                   ctx->d_tiles, ctx->d_lc, ctx->flags, net.y,
+                                                   net.H, net.W, ctx->lcf));
+        LAUNCH("learn_ghost", launch_ghost(ctx->stream, ctx->nbtot, ctx->d_be, ctx->d_cs,
+                                           ctx->lcf, -1, 0));
+        LAUNCH("learn_dots", launch_learn_dots(ctx->stream, ctx->ntiles, ctx->d_cd, ctx->d_tiles,
+                                               ctx->flags, ctx->lres, ctx->lcf, ctx->d_part));
+        LAUNCH("learn_alpha", launch_learn_alpha(ctx->stream, n, ctx->d_cd, ctx->d_lc,
+                                                 ctx->d_part));
+        LAUNCH("learn_apply", launch_learn_apply(ctx->stream, ctx->ntiles, ctx->d_cd,
+                                                 ctx->d_tiles, ctx->d_lc, ctx->flags, ctx->lcf,
+                                                 ctx->p));
+        if ((rc = learn_phase(ctx, ctx->lsweeps, 0))) return rc;
+        float ms = 0.f;
+        HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->nn_ev[0], ctx->nn_ev[1]));
+        net.fwd_ms += ms;
+        net.fwd_calls += 1;
+        net.fwd_samples += (uint64_t)n;
+        bool any = false;
+        if ((rc = any_running(ctx, &any))) return rc;
+        if (!any) break;
+    }
+    return learn_phase(ctx, 0, 1);
+}
+
+// A plain solve split in two phases that stores two training pairs per
+// channel: the residual at the start and after `csweeps` sweeps, with the
+// normalised change of p from there to the converged solution.
+int capture_solve(stm_ctx* ctx) {
+    Net& net = ctx->cnet->net;
+    const int n = nch(ctx);
+    const size_t bytes = sizeof(double) * (size_t)ctx->total;
+    const int slot0 = net.dwrite, slot1 = (net.dwrite + n) % net.dcap;
+    int rc;
+    if ((rc = learn_prepare(ctx, 1))) return rc;
+    LAUNCH("learn_pack", launch_learn_pack(ctx->stream, n, ctx->d_cd, ctx->d_lc, ctx->flags,
+                                           ctx->lres, net.H, net.W, nullptr, net.dx, slot0,
+                                           net.dcap));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->lp0, ctx->p, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    if ((rc = learn_phase(ctx, ctx->csweeps, 0))) return rc;
+    if ((rc = learn_prepare(ctx, 0))) return rc;
+    LAUNCH("learn_pack", launch_learn_pack(ctx->stream, n, ctx->d_cd, ctx->d_lc, ctx->flags,
+                                           ctx->lres, net.H, net.W, nullptr, net.dx, slot1,
+                                           net.dcap));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->lpk, ctx->p, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    if ((rc = learn_phase(ctx, 0, 1))) return rc;
+    LAUNCH("learn_target", launch_learn_target(ctx->stream, ctx->ntiles, n, ctx->d_cd,
+                                               ctx->d_tiles, ctx->d_lc, ctx->flags, ctx->p,
+                                               ctx->lp0, ctx->d_part, net.H, net.W, net.dt,
+                                               slot0, net.dcap));
+    LAUNCH("learn_target", launch_learn_target(ctx->stream, ctx->ntiles, n, ctx->d_cd,
+                                               ctx->d_tiles, ctx->d_lc, ctx->flags, ctx->p,
+                                               ctx->lpk, ctx->d_part, net.H, net.W, net.dt,
+                                               slot1, net.dcap));
+    net.dwrite = (net.dwrite + 2 * n) % net.dcap;
+    net.dsize = std::min(net.dsize + 2 * n, net.dcap);
+    return STM_OK;
+}
+
+// Buffers of the learned solve (and of capture when `capture`).
+int learn_buffers(stm_ctx* ctx, bool capture) {
+    const size_t bytes = sizeof(double) * (size_t)ctx->total;
+    if (!ctx->d_lc) {
+        HIPCHK(ctx, hipMalloc(&ctx->d_lc, sizeof(LearnChan) * nch(ctx)));
+        HIPCHK(ctx, hipMemset(ctx->d_lc, 0, sizeof(LearnChan) * nch(ctx)));
+        HIPCHK(ctx, hipHostMalloc(&ctx->h_lc, sizeof(LearnChan) * nch(ctx), hipHostMallocDefault));
+        HIPCHK(ctx, hipMalloc(&ctx->lres, bytes));
+        HIPCHK(ctx, hipMalloc(&ctx->lcf, bytes));
+        HIPCHK(ctx, hipMemset(ctx->lres, 0, bytes));
+        HIPCHK(ctx, hipMemset(ctx->lcf, 0, bytes));
+        HIPCHK(ctx, hipEventCreate(&ctx->nn_ev[0]));
+        HIPCHK(ctx, hipEventCreate(&ctx->nn_ev[1]));
+    }
+    if (capture && !ctx->lp0) {
+        HIPCHK(ctx, hipMalloc(&ctx->lp0, bytes));
+        HIPCHK(ctx, hipMalloc(&ctx->lpk, bytes));
+    }
+    return STM_OK;
+}
+
+// The net fits the context: same device and stream, every channel's
+// interior fits the net's H x W, a batch entry per channel, all channels
+// red-black.
+int check_net(stm_ctx* ctx, stm_net* net) {
+    const Net& m = net->net;
+    if (m.ctx != ctx) return fail(ctx, STM_ERR_INVALID_ARG, "the net belongs to another context");
+    if (m.cap < nch(ctx))
+        return fail(ctx, STM_ERR_INVALID_ARG, "net batch %d below the channel count %d", m.cap,
+                    nch(ctx));
+    for (size_t k = 0; k < ctx->chans.size(); k++) {
+        const HostChan& c = ctx->chans[k];
+        if ((int)c.nx - 2 > m.H || (int)c.ny - 2 > m.W)
+            return fail(ctx, STM_ERR_INVALID_ARG,
+                        "channel %zu (interior %ux%u) exceeds the net's %dx%d", k, c.nx - 2,
+                        c.ny - 2, m.H, m.W);
+        if (c.prm.solver != STM_SOLVER_RB)
+            return fail(ctx, STM_ERR_INVALID_ARG, "channel %zu is not red-black", k);
+    }
+    return STM_OK;
+}
+
 // POISSON: the rhs statistics, tol2 and the per-channel flags, then each
 // channel's solver.
 int stage_solve(stm_ctx* ctx) {
@@ -596,6 +756,8 @@ int stage_solve(stm_ctx* ctx) {
                                              ctx->d_be, ctx->flags, ctx->rhs, ctx->p,
                                              ctx->scratch));
     if (!ctx->any_rb) return STM_OK;
+    if (ctx->cnet) return capture_solve(ctx);
+    if (ctx->lnet && ctx->lcycles > 0) return learned_solve(ctx);
     return blocked_solve(ctx, 1);
 }
 
@@ -704,11 +866,15 @@ void stm_destroy(stm_ctx* ctx) {
     for (hipEvent_t e : ctx->free_events) hipEventDestroy(e);
     void* bufs[] = {ctx->u, ctx->v, ctx->p, ctx->f, ctx->g, ctx->rhs, ctx->scratch, ctx->p2,
                     ctx->flags, ctx->d_cd, ctx->d_cs, ctx->d_be, ctx->d_restore, ctx->d_tiles,
-                    ctx->d_part, ctx->d_active, ctx->d_tbpart, ctx->d_tb};
+                    ctx->d_part, ctx->d_active, ctx->d_tbpart, ctx->d_tb, ctx->d_lc, ctx->lres,
+                    ctx->lcf, ctx->lp0, ctx->lpk};
     for (void* b : bufs)
         if (b) hipFree(b);
     if (ctx->h_active) hipHostFree(ctx->h_active);
     if (ctx->h_cs) hipHostFree(ctx->h_cs);
+    if (ctx->h_lc) hipHostFree(ctx->h_lc);
+    for (hipEvent_t e : ctx->nn_ev)
+        if (e) hipEventDestroy(e);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -962,6 +1128,7 @@ int arm_channels(stm_ctx* ctx, int itermax_override, bool never_converge,
         s.stop_launch = s.active ? -1 : 0;
         s.sor_iters = 0;
         s.pbuf = 0;
+        s.budget = 0;
         if (never_converge) s.tol2 = -1.0;
     }
     HIPCHK(ctx, hipMemcpyAsync(ctx->d_cd, cd.data(), sizeof(ChanDesc) * cd.size(),
@@ -1133,6 +1300,42 @@ int stm_kernel_stats(stm_ctx* ctx, const char* name, double* total_ms, uint64_t*
     return STM_OK;
 }
 
+int stm_set_learned(stm_ctx* ctx, stm_net* net, uint32_t cycles, uint32_t sweeps) {
+    int rc = ready(ctx);
+    if (rc) return rc;
+    if (!net || cycles == 0) {
+        ctx->lnet = nullptr;
+        ctx->lcycles = 0;
+        return STM_OK;
+    }
+    if (sweeps == 0) return fail(ctx, STM_ERR_INVALID_ARG, "sweeps per cycle must be positive");
+    if ((rc = check_net(ctx, net))) return rc;
+    if ((rc = learn_buffers(ctx, false))) return rc;
+    ctx->lnet = net;
+    ctx->lcycles = (int)cycles;
+    ctx->lsweeps = (int)sweeps;
+    return STM_OK;
+}
+
+int stm_set_capture(stm_ctx* ctx, stm_net* net, uint32_t sweeps) {
+    int rc = ready(ctx);
+    if (rc) return rc;
+    if (!net) {
+        ctx->cnet = nullptr;
+        return STM_OK;
+    }
+    if (sweeps == 0) return fail(ctx, STM_ERR_INVALID_ARG, "sweeps must be positive");
+    if ((rc = check_net(ctx, net))) return rc;
+    if (net->net.dcap < 2 * nch(ctx))
+        return fail(ctx, STM_ERR_INVALID_ARG,
+                    "training set of %d samples cannot take the %d pairs of one solve",
+                    net->net.dcap, 2 * nch(ctx));
+    if ((rc = learn_buffers(ctx, true))) return rc;
+    ctx->cnet = net;
+    ctx->csweeps = (int)sweeps;
+    return STM_OK;
+}
+
 int stm_reset_stats(stm_ctx* ctx) {
     if (!ctx) return fail(nullptr, STM_ERR_INVALID_ARG, "null context");
     int rc = drain(ctx);
@@ -1142,3 +1345,8 @@ int stm_reset_stats(stm_ctx* ctx) {
 }
 
 }  // extern "C"
+
+namespace stm {
+hipStream_t ctx_stream(stm_ctx* ctx) { return ctx->stream; }
+int ctx_device(stm_ctx* ctx) { return ctx->device; }
+}  // namespace stm

@@ -55,4 +55,27 @@ void launch_rb_tb(hipStream_t, int variant, int ntb, const ChanDesc*, ChanState*
 void launch_pbuf_fix(hipStream_t, int ntiles, int nchan, const ChanDesc*, ChanState*,
                      const Tile*, const double* pb, double* pa);
 
+// kernels_learn.hip: the FP64 side of the learned correction and of capture
+void launch_learn_residual(hipStream_t, int ntiles, const ChanDesc*, const ChanState*,
+                           const Tile*, const uint8_t* flags, const double* rhs, const double* p,
+                           double* res, double* part);
+void launch_learn_scale(hipStream_t, int nch, const ChanDesc*, ChanState*, LearnChan*,
+                        const double* part, int first);
+void launch_learn_pack(hipStream_t, int nch, const ChanDesc*, const LearnChan*,
+                       const uint8_t* flags, const double* res, int H, int W, __bf16* a0,
+                       __bf16* dx, int slot0, int dcap);
+void launch_learn_unpack(hipStream_t, int ntiles, const ChanDesc*, const Tile*, const LearnChan*,
+                         const uint8_t* flags, const float* y, int H, int W, double* cf);
+void launch_learn_dots(hipStream_t, int ntiles, const ChanDesc*, const Tile*,
+                       const uint8_t* flags, const double* res, const double* cf, double* part);
+void launch_learn_alpha(hipStream_t, int nch, const ChanDesc*, LearnChan*, const double* part);
+void launch_learn_apply(hipStream_t, int ntiles, const ChanDesc*, const Tile*, const LearnChan*,
+                        const uint8_t* flags, const double* cf, double* p);
+void launch_learn_arm(hipStream_t, int nch, const ChanDesc*, ChanState*, const LearnChan*,
+                      int32_t* active, int budget);
+void launch_learn_collect(hipStream_t, int nch, ChanState*, LearnChan*, int final_phase);
+void launch_learn_target(hipStream_t, int ntiles, int nch, const ChanDesc*, const Tile*,
+                         LearnChan*, const uint8_t* flags, const double* p, const double* p0,
+                         double* part, int H, int W, float* dt, int slot0, int dcap);
+
 }  // namespace stm

@@ -122,6 +122,7 @@ k_solve_begin(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
         s.active = (rb && d.itermax > 0) ? 1 : 0;
         s.stop_launch = s.active ? -1 : 0;
         s.pbuf = 0;
+        s.budget = 0;
         active_out[blockIdx.x] = s.active;
     }
 }

@@ -274,15 +274,16 @@ k_rb_strip(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
     }
 
     // ---- stop decision on the previous launch (identical in every tile)
+    const int itermax = (s.budget > 0 && s.budget < d.itermax) ? s.budget : d.itermax;
     const int64_t sweeps = (int64_t)L * K;
-    const int done = sweeps < d.itermax ? (int)sweeps : d.itermax;
+    const int done = sweeps < itermax ? (int)sweeps : itermax;
     if (L > 0) {
         double acc = 0.0;
         if (check)
             for (int i = d.tb0 + tid; i < d.tb0 + d.ntb; i += NT) acc = acc + part_prev[i];
         const double res2 = block_sum<NT>(acc, wsum);
         const bool conv = check && res2 <= s.tol2;
-        if (conv || done >= d.itermax) {
+        if (conv || done >= itermax) {
             if (blockIdx.x == (unsigned)d.tb0 && tid == 0) {
                 s.stop_launch = L - 1;
                 s.sor_iters = (uint32_t)done;
@@ -298,7 +299,7 @@ k_rb_strip(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
             return;
         }
     }
-    const int nsw = d.itermax - done < K ? d.itermax - done : K;
+    const int nsw = itermax - done < K ? itermax - done : K;
 
     const double acc = (ex0 & 1) ? Blk<K, R, HY, NS, 1>::run(P, pv, rr, cc, lx0, m, nsw, check, d)
                                  : Blk<K, R, HY, NS, 0>::run(P, pv, rr, cc, lx0, m, nsw, check, d);

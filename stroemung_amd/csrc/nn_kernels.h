@@ -1,0 +1,44 @@
+// Launchers of the pressure-network kernels (nn_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace stm {
+namespace nn {
+
+typedef __bf16 bf16;
+
+// 3x3 convolution, padding 1, NHWC bf16, fp32 accumulation on the matrix
+// cores.  w is [cout][9][cin] bf16; flags 1 = add bias (fp32), 2 = ReLU.
+// Writes bf16 `out` and/or fp32 `out32` (either may be null).
+void conv3x3(hipStream_t, const bf16* in, int cin, const bf16* w, const float* bias, bf16* out,
+             float* out32, int cout, int nb, int H, int W, int flags);
+// w'[ci][8 - tap][co] = w[co][tap][ci]: the backward-input weights
+void weight_flip(hipStream_t, const bf16* w, bf16* wt, int cin, int cout);
+// dw32[co][tap][ci] += sum_pix gz[pix][co] * in[pix + off(tap)][ci]
+void conv3x3_dw(hipStream_t, const bf16* gz, int cout, const bf16* in, int cin, float* dw32, int nb,
+                int H, int W);
+void relu_bwd(hipStream_t, int64_t n, const bf16* y, bf16* g);
+void bias_grad(hipStream_t, const bf16* g, int c, int64_t npix, float* db);
+void maxpool(hipStream_t, const bf16* x, bf16* y, int nb, int H, int W, int c);
+// accumulates into gx
+void maxpool_bwd(hipStream_t, const bf16* x, const bf16* y, const bf16* gy, bf16* gx, int nb,
+                 int H, int W, int c);
+// x (nb, H/2, W/2, c) -> channels [c0, c0 + c) of y (nb, H, W, cy)
+void upsample(hipStream_t, const bf16* x, bf16* y, int nb, int H, int W, int c, int cy, int c0);
+void upsample_bwd(hipStream_t, const bf16* gy, bf16* gx, int nb, int H, int W, int c, int cy,
+                  int c0);
+void channel_copy(hipStream_t, const bf16* src, int cs, int s0, bf16* dst, int cd, int d0, int c,
+                  int64_t npix, int add);
+void head(hipStream_t, const bf16* x, int c, const float* w, const float* b, float* y,
+          int64_t npix);
+void head_bwd(hipStream_t, const bf16* x, int c, const float* w, const float* gy, bf16* gx,
+              float* dw, float* db, int64_t npix);
+void loss(hipStream_t, const float* y, const float* t, const float* m, float* denom, float* gy,
+          float* loss_out, int64_t npix_per, int nb);
+void adam(hipStream_t, int64_t n, float* p, const float* g, float* m, float* v, bf16* pb, float lr,
+          float b1, float b2, float eps, int step, float clip);
+void to_bf16(hipStream_t, int64_t n, const float* x, bf16* y);
+
+}  // namespace nn
+}  // namespace stm

@@ -1,0 +1,478 @@
+// gfx950 kernels of the pressure network: a U-Net of 3x3 convolutions in
+// BF16 with FP32 accumulation on the matrix cores.
+//
+// Tensors are NHWC (channels contiguous), bf16, with channel counts that are
+// multiples of 32.  A 3x3 convolution with padding 1 is an implicit GEMM:
+//   forward      out[pix, co]  = sum_{tap, ci} in[pix + off(tap), ci] * w[co, tap, ci]
+//   backward-in  gin[pix, ci]  = sum_{tap, co} gz[pix - off(tap), co] * w[co, tap, ci]
+//                              = forward with w'[ci, 8 - tap, co] (rotated, transposed)
+//   backward-w   dw[co, tap, ci] = sum_pix gz[pix, co] * in[pix + off(tap), ci]
+// v_mfma_f32_16x16x32_bf16 operands (checked on the device): lane l holds row
+// l % 16 of A and column l % 16 of B, each with k = 8 * (l / 16) + 0..7, and
+// column l % 16, rows 4 * (l / 16) + 0..3 of the f32 result.
+#include <hip/hip_runtime.h>
+
+#include "nn_kernels.h"
+
+namespace stm {
+namespace nn {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ bf16x8 load8(const bf16* p) {
+    return *reinterpret_cast<const bf16x8*>(p);
+}
+__device__ __forceinline__ bf16x8 zero8() {
+    bf16x8 z;
+    for (int i = 0; i < 8; i++) z[i] = (bf16)0.0f;
+    return z;
+}
+
+// ------------------------------------------------------------ convolution
+
+// Block: 256 threads = 4 waves in a 2 x 2 grid; block tile BM pixels x 64
+// output channels, each wave (BM/2) x 32 as (BM/32) x 2 MFMA tiles.
+// flags: 1 = add bias, 2 = ReLU.
+template <int BM>
+__global__ void __launch_bounds__(256)
+k_conv3x3(const bf16* __restrict__ in, int cin, const bf16* __restrict__ w,
+          const float* __restrict__ bias, bf16* __restrict__ out, float* __restrict__ out32,
+          int cout, int nb, int H, int W, int flags)
+{
+    constexpr int MT = BM / 32;          // M tiles per wave
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int npix = nb * H * W;
+    const int m_base = blockIdx.x * BM + wm * (BM / 2);
+    const int n_base = blockIdx.y * 64 + wn * 32;
+    if (n_base >= cout) return;          // 32-channel layers use half the tile (no barriers here)
+    const int row = lane & 15, kq = (lane >> 4) * 8;
+
+    // the pixels this lane loads for each M tile
+    int pb[MT], ph[MT], pw[MT];
+    bool pv[MT];
+#pragma unroll
+    for (int t = 0; t < MT; t++) {
+        const int pix = m_base + t * 16 + row;
+        pv[t] = pix < npix;
+        const int pp = pv[t] ? pix : 0;
+        pb[t] = pp / (H * W);
+        const int r = pp - pb[t] * H * W;
+        ph[t] = r / W;
+        pw[t] = r - ph[t] * W;
+    }
+    f32x4 acc[MT][2];
+#pragma unroll
+    for (int t = 0; t < MT; t++)
+        for (int u = 0; u < 2; u++) acc[t][u] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    for (int tap = 0; tap < 9; tap++) {
+        const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+        const bf16* arow[MT];
+        bool aok[MT];
+#pragma unroll
+        for (int t = 0; t < MT; t++) {
+            const int hh = ph[t] + dy, ww = pw[t] + dx;
+            aok[t] = pv[t] && hh >= 0 && hh < H && ww >= 0 && ww < W;
+            arow[t] = in + ((int64_t)(pb[t] * H + (aok[t] ? hh : 0)) * W + (aok[t] ? ww : 0)) * cin;
+        }
+        const bf16* brow[2];
+#pragma unroll
+        for (int u = 0; u < 2; u++) brow[u] = w + ((int64_t)(n_base + u * 16 + row) * 9 + tap) * cin;
+        for (int c0 = 0; c0 < cin; c0 += 32) {
+            bf16x8 a[MT], b[2];
+#pragma unroll
+            for (int t = 0; t < MT; t++) a[t] = aok[t] ? load8(arow[t] + c0 + kq) : zero8();
+#pragma unroll
+            for (int u = 0; u < 2; u++) b[u] = load8(brow[u] + c0 + kq);
+#pragma unroll
+            for (int t = 0; t < MT; t++)
+#pragma unroll
+                for (int u = 0; u < 2; u++)
+                    acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[t], b[u], acc[t][u], 0, 0, 0);
+        }
+    }
+    // epilogue: lane holds column n = l % 16, rows 4 * (l / 16) + i
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+        const int n = n_base + u * 16 + row;
+        const float bv = (flags & 1) ? bias[n] : 0.f;
+#pragma unroll
+        for (int t = 0; t < MT; t++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int pix = m_base + t * 16 + (lane >> 4) * 4 + i;
+                if (pix >= npix) continue;
+                float v = acc[t][u][i] + bv;
+                if (flags & 2) v = v > 0.f ? v : 0.f;
+                if (out) out[(int64_t)pix * cout + n] = (bf16)v;
+                if (out32) out32[(int64_t)pix * cout + n] = v;
+            }
+    }
+}
+
+// w'[ci][8 - tap][co] = w[co][tap][ci]
+__global__ void k_weight_flip(const bf16* __restrict__ w, bf16* __restrict__ wt, int cin, int cout)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = (int64_t)cin * 9 * cout;
+    if (k >= n) return;
+    const int ci = (int)(k % cin);
+    const int tap = (int)((k / cin) % 9);
+    const int co = (int)(k / ((int64_t)cin * 9));
+    wt[((int64_t)ci * 9 + (8 - tap)) * cout + co] = w[k];
+}
+
+// dw[co][tap][ci] += sum over a pixel chunk of gz[pix][co] * in[pix+off][ci].
+// Block: 256 threads, tile 64 co x 64 ci for one tap, pixel chunks of 32
+// staged transposed in LDS ([channel][pixel]) so each lane's 8 consecutive
+// pixels are one 16-byte read.  Partial sums go to dw32 with atomics.
+__global__ void __launch_bounds__(256)
+k_conv3x3_dw(const bf16* __restrict__ gz, int cout, const bf16* __restrict__ in, int cin,
+             float* __restrict__ dw32, int nb, int H, int W, int pix_per_block)
+{
+    __shared__ __attribute__((aligned(16))) bf16 sa[64][40];   // [co][pix], padded rows
+    __shared__ __attribute__((aligned(16))) bf16 sb[64][40];   // [ci][pix]
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int tap = blockIdx.z % 9;
+    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+    const int co0 = blockIdx.y * 64;
+    const int ci0 = blockIdx.x * 64;
+    const int npix = nb * H * W;
+    const int p_begin = (blockIdx.z / 9) * pix_per_block;
+    const int p_end = min(npix, p_begin + pix_per_block);
+    const int ci_n = cin - ci0 < 64 ? cin - ci0 : 64;    // 32 for 32-channel layers
+    const int co_n = cout - co0 < 64 ? cout - co0 : 64;
+    f32x4 acc[2][2];
+    for (int t = 0; t < 2; t++)
+        for (int u = 0; u < 2; u++) acc[t][u] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int row = lane & 15, kq = (lane >> 4) * 8;
+    // staging: thread -> (pixel j = tid / 8, channel group g = tid % 8 of 8 channels)
+    const int sj = threadIdx.x >> 3, sg = (threadIdx.x & 7) * 8;
+    for (int p0 = p_begin; p0 < p_end; p0 += 32) {
+        const int pix = p0 + sj;
+        bf16x8 va = zero8(), vb = zero8();
+        if (pix < p_end) {
+            if (sg < co_n) va = load8(gz + (int64_t)pix * cout + co0 + sg);
+            const int b = pix / (H * W), r = pix - b * H * W;
+            const int hh = r / W + dy, ww = r % W + dx;
+            if (hh >= 0 && hh < H && ww >= 0 && ww < W && sg < ci_n)
+                vb = load8(in + ((int64_t)(b * H + hh) * W + ww) * cin + ci0 + sg);
+        }
+        __syncthreads();
+        for (int i = 0; i < 8; i++) {
+            sa[sg + i][sj] = va[i];
+            sb[sg + i][sj] = vb[i];
+        }
+        __syncthreads();
+        bf16x8 a[2], b[2];
+        for (int t = 0; t < 2; t++) a[t] = load8(&sa[wm * 32 + t * 16 + row][kq]);
+        for (int u = 0; u < 2; u++) b[u] = load8(&sb[wn * 32 + u * 16 + row][kq]);
+        for (int t = 0; t < 2; t++)
+            for (int u = 0; u < 2; u++)
+                acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[t], b[u], acc[t][u], 0, 0, 0);
+    }
+    for (int t = 0; t < 2; t++)
+        for (int u = 0; u < 2; u++) {
+            const int ci = ci0 + wn * 32 + u * 16 + row;
+            if (ci >= ci0 + ci_n) continue;
+            for (int i = 0; i < 4; i++) {
+                const int co = co0 + wm * 32 + t * 16 + (lane >> 4) * 4 + i;
+                if (co >= co0 + co_n) continue;
+                atomicAdd(&dw32[((int64_t)co * 9 + tap) * cin + ci], acc[t][u][i]);
+            }
+        }
+}
+
+// ------------------------------------------------------------ elementwise
+
+// g = g * (y > 0): gradient through a ReLU whose output is y
+__global__ void k_relu_bwd(int64_t n, const bf16* __restrict__ y, bf16* __restrict__ g)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n && !((float)y[k] > 0.f)) g[k] = (bf16)0.f;
+}
+
+// db[c] += sum over pixels of g[pix][c]
+__global__ void k_bias_grad(const bf16* __restrict__ g, int c, int npix, float* __restrict__ db)
+{
+    const int ch = blockIdx.y * blockDim.x + threadIdx.x;
+    if (ch >= c) return;
+    const int per = (npix + gridDim.x - 1) / gridDim.x;
+    const int p0 = blockIdx.x * per, p1 = min(npix, p0 + per);
+    float s = 0.f;
+    for (int p = p0; p < p1; p++) s += (float)g[(int64_t)p * c + ch];
+    atomicAdd(&db[ch], s);
+}
+
+// 2x2 max pooling, H and W even
+__global__ void k_maxpool(const bf16* __restrict__ x, bf16* __restrict__ y, int nb, int H, int W, int c)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int Ho = H / 2, Wo = W / 2;
+    const int64_t n = (int64_t)nb * Ho * Wo * c;
+    if (k >= n) return;
+    const int ch = (int)(k % c);
+    const int64_t q = k / c;
+    const int wo = (int)(q % Wo), ho = (int)((q / Wo) % Ho), b = (int)(q / ((int64_t)Wo * Ho));
+    const int64_t base = ((int64_t)(b * H + 2 * ho) * W + 2 * wo) * c + ch;
+    float m = (float)x[base];
+    m = fmaxf(m, (float)x[base + c]);
+    m = fmaxf(m, (float)x[base + (int64_t)W * c]);
+    m = fmaxf(m, (float)x[base + (int64_t)W * c + c]);
+    y[k] = (bf16)m;
+}
+
+// gradient of max pooling: to the first maximal input of each window
+__global__ void k_maxpool_bwd(const bf16* __restrict__ x, const bf16* __restrict__ y,
+                              const bf16* __restrict__ gy, bf16* __restrict__ gx, int nb, int H,
+                              int W, int c)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int Ho = H / 2, Wo = W / 2;
+    const int64_t n = (int64_t)nb * Ho * Wo * c;
+    if (k >= n) return;
+    const int ch = (int)(k % c);
+    const int64_t q = k / c;
+    const int wo = (int)(q % Wo), ho = (int)((q / Wo) % Ho), b = (int)(q / ((int64_t)Wo * Ho));
+    const int64_t base = ((int64_t)(b * H + 2 * ho) * W + 2 * wo) * c + ch;
+    const int64_t off[4] = {0, c, (int64_t)W * c, (int64_t)W * c + c};
+    const float m = (float)y[k];
+    int hit = 0;
+    for (int i = 0; i < 4; i++) {
+        const bool take = !hit && (float)x[base + off[i]] == m;
+        if (take) gx[base + off[i]] = (bf16)((float)gx[base + off[i]] + (float)gy[k]);
+        hit |= take;
+    }
+}
+
+// nearest 2x upsampling of x (nb, H/2, W/2, c) into channels [c0, c0 + c) of
+// y (nb, H, W, cy) -- the concatenation with the skip happens in place
+__global__ void k_upsample(const bf16* __restrict__ x, bf16* __restrict__ y, int nb, int H, int W,
+                           int c, int cy, int c0)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = (int64_t)nb * H * W * c;
+    if (k >= n) return;
+    const int ch = (int)(k % c);
+    const int64_t q = k / c;
+    const int w = (int)(q % W), h = (int)((q / W) % H), b = (int)(q / ((int64_t)W * H));
+    y[q * cy + c0 + ch] = x[((int64_t)(b * (H / 2) + h / 2) * (W / 2) + w / 2) * c + ch];
+}
+
+__global__ void k_upsample_bwd(const bf16* __restrict__ gy, bf16* __restrict__ gx, int nb, int H,
+                               int W, int c, int cy, int c0)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int Ho = H / 2, Wo = W / 2;
+    const int64_t n = (int64_t)nb * Ho * Wo * c;
+    if (k >= n) return;
+    const int ch = (int)(k % c);
+    const int64_t q = k / c;
+    const int wo = (int)(q % Wo), ho = (int)((q / Wo) % Ho), b = (int)(q / ((int64_t)Wo * Ho));
+    float s = 0.f;
+    for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 2; j++)
+            s += (float)gy[((int64_t)(b * H + 2 * ho + i) * W + 2 * wo + j) * cy + c0 + ch];
+    gx[k] = (bf16)s;
+}
+
+// copy channels: y[:, c0:c0+c] = x (forward of the skip half of a concat) or
+// x = y[:, c0:c0+c] (its backward), add = accumulate into x
+__global__ void k_channel_copy(const bf16* __restrict__ src, int cs, int s0, bf16* __restrict__ dst,
+                               int cd, int d0, int c, int64_t npix, int add)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= npix * c) return;
+    const int ch = (int)(k % c);
+    const int64_t p = k / c;
+    const bf16 v = src[p * cs + s0 + ch];
+    bf16& d = dst[p * cd + d0 + ch];
+    d = add ? (bf16)((float)d + (float)v) : v;
+}
+
+// 1x1 head: y[pix] = b + sum_c x[pix, c] * w[c]   (fp32 out)
+__global__ void k_head(const bf16* __restrict__ x, int c, const float* __restrict__ w,
+                       const float* __restrict__ b, float* __restrict__ y, int64_t npix)
+{
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npix) return;
+    float s = b[0];
+    for (int i = 0; i < c; i++) s += (float)x[p * c + i] * w[i];
+    y[p] = s;
+}
+
+// head backward: gx[pix, c] = gy[pix] * w[c]; dw[c] += sum gy * x; db += sum gy
+__global__ void k_head_bwd(const bf16* __restrict__ x, int c, const float* __restrict__ w,
+                           const float* __restrict__ gy, bf16* __restrict__ gx,
+                           float* __restrict__ dw, float* __restrict__ db, int64_t npix)
+{
+    __shared__ float red[256];
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const float g = p < npix ? gy[p] : 0.f;
+    if (p < npix)
+        for (int i = 0; i < c; i++) gx[p * c + i] = (bf16)(g * w[i]);
+    for (int i = -1; i < c; i++) {
+        red[threadIdx.x] = i < 0 ? g : (p < npix ? g * (float)x[p * c + i] : 0.f);
+        __syncthreads();
+        for (int s = 128; s > 0; s >>= 1) {
+            if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) atomicAdd(i < 0 ? db : &dw[i], red[0]);
+        __syncthreads();
+    }
+}
+
+// masked MSE, normalised per sample: loss_s = sum m (y-t)^2 / sum m t^2;
+// gy = 2 m (y - t) / (denominator * nb).  A sample whose target is zero (a
+// solve that made no change) has weight zero.
+__global__ void k_loss(const float* __restrict__ y, const float* __restrict__ t,
+                       const float* __restrict__ m, const float* __restrict__ denom,
+                       float* __restrict__ gy, float* __restrict__ loss, int64_t npix_per,
+                       int nb)
+{
+    __shared__ float red[256];
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = npix_per * nb;
+    float l = 0.f;
+    if (k < n) {
+        const int s = (int)(k / npix_per);
+        const float d = y[k] - t[k];
+        const float mm = denom[s] > 1e-12f ? m[k] : 0.f;
+        l = mm * d * d / denom[s];
+        gy[k] = 2.f * mm * d / (denom[s] * nb);
+    }
+    red[threadIdx.x] = l / nb;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(loss, red[0]);
+}
+
+// per-sample sum of m * t^2 (+1e-20)
+__global__ void k_target_norm(const float* __restrict__ t, const float* __restrict__ m,
+                              float* __restrict__ denom, int64_t npix_per)
+{
+    __shared__ float red[256];
+    const int s = blockIdx.y;
+    float acc = 0.f;
+    for (int64_t k = blockIdx.x * 256 + threadIdx.x; k < npix_per; k += (int64_t)gridDim.x * 256) {
+        const float v = t[s * npix_per + k];
+        acc += m[s * npix_per + k] * v * v;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(&denom[s], red[0] + (blockIdx.x == 0 ? 1e-30f : 0.f));
+}
+
+// Adam on fp32 master weights; refreshes the bf16 copy used by the kernels
+__global__ void k_adam(int64_t n, float* __restrict__ p, const float* __restrict__ g,
+                       float* __restrict__ m, float* __restrict__ v, bf16* __restrict__ pb,
+                       float lr, float b1, float b2, float eps, float c1, float c2, float clip)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    float gk = g[k] * clip;
+    const float mk = b1 * m[k] + (1.f - b1) * gk;
+    const float vk = b2 * v[k] + (1.f - b2) * gk * gk;
+    m[k] = mk;
+    v[k] = vk;
+    const float upd = lr * (mk / c1) / (sqrtf(vk / c2) + eps);
+    p[k] -= upd;
+    if (pb) pb[k] = (bf16)p[k];
+}
+
+__global__ void k_to_bf16(int64_t n, const float* __restrict__ x, bf16* __restrict__ y)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) y[k] = (bf16)x[k];
+}
+
+// ------------------------------------------------------------ launchers
+
+static inline unsigned nblk(int64_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
+
+void conv3x3(hipStream_t st, const bf16* in, int cin, const bf16* w, const float* bias, bf16* out,
+             float* out32, int cout, int nb, int H, int W, int flags) {
+    const int64_t npix = (int64_t)nb * H * W;
+    if (npix >= 64 * 256) {
+        dim3 grid(nblk(npix, 128), (cout + 63) / 64);
+        k_conv3x3<128><<<grid, 256, 0, st>>>(in, cin, w, bias, out, out32, cout, nb, H, W, flags);
+    } else {
+        dim3 grid(nblk(npix, 64), (cout + 63) / 64);
+        k_conv3x3<64><<<grid, 256, 0, st>>>(in, cin, w, bias, out, out32, cout, nb, H, W, flags);
+    }
+}
+void weight_flip(hipStream_t st, const bf16* w, bf16* wt, int cin, int cout) {
+    const int64_t n = (int64_t)cin * 9 * cout;
+    k_weight_flip<<<nblk(n), 256, 0, st>>>(w, wt, cin, cout);
+}
+void conv3x3_dw(hipStream_t st, const bf16* gz, int cout, const bf16* in, int cin, float* dw32,
+                int nb, int H, int W) {
+    const int64_t npix = (int64_t)nb * H * W;
+    const int ci_blocks = (cin + 63) / 64;
+    int per = 4096;
+    while (npix / per > 64 && per < (1 << 20)) per *= 2;
+    const int chunks = (int)((npix + per - 1) / per);
+    dim3 grid(ci_blocks, (cout + 63) / 64, 9 * chunks);
+    k_conv3x3_dw<<<grid, 256, 0, st>>>(gz, cout, in, cin, dw32, nb, H, W, per);
+}
+void relu_bwd(hipStream_t st, int64_t n, const bf16* y, bf16* g) {
+    k_relu_bwd<<<nblk(n), 256, 0, st>>>(n, y, g);
+}
+void bias_grad(hipStream_t st, const bf16* g, int c, int64_t npix, float* db) {
+    dim3 grid(64, (c + 255) / 256);
+    k_bias_grad<<<grid, 256, 0, st>>>(g, c, (int)npix, db);
+}
+void maxpool(hipStream_t st, const bf16* x, bf16* y, int nb, int H, int W, int c) {
+    k_maxpool<<<nblk((int64_t)nb * H * W * c / 4), 256, 0, st>>>(x, y, nb, H, W, c);
+}
+void maxpool_bwd(hipStream_t st, const bf16* x, const bf16* y, const bf16* gy, bf16* gx, int nb,
+                 int H, int W, int c) {
+    k_maxpool_bwd<<<nblk((int64_t)nb * H * W * c / 4), 256, 0, st>>>(x, y, gy, gx, nb, H, W, c);
+}
+void upsample(hipStream_t st, const bf16* x, bf16* y, int nb, int H, int W, int c, int cy, int c0) {
+    k_upsample<<<nblk((int64_t)nb * H * W * c), 256, 0, st>>>(x, y, nb, H, W, c, cy, c0);
+}
+void upsample_bwd(hipStream_t st, const bf16* gy, bf16* gx, int nb, int H, int W, int c, int cy,
+                  int c0) {
+    k_upsample_bwd<<<nblk((int64_t)nb * H * W * c / 4), 256, 0, st>>>(gy, gx, nb, H, W, c, cy, c0);
+}
+void channel_copy(hipStream_t st, const bf16* src, int cs, int s0, bf16* dst, int cd, int d0, int c,
+                  int64_t npix, int add) {
+    k_channel_copy<<<nblk(npix * c), 256, 0, st>>>(src, cs, s0, dst, cd, d0, c, npix, add);
+}
+void head(hipStream_t st, const bf16* x, int c, const float* w, const float* b, float* y,
+          int64_t npix) {
+    k_head<<<nblk(npix), 256, 0, st>>>(x, c, w, b, y, npix);
+}
+void head_bwd(hipStream_t st, const bf16* x, int c, const float* w, const float* gy, bf16* gx,
+              float* dw, float* db, int64_t npix) {
+    k_head_bwd<<<nblk(npix), 256, 0, st>>>(x, c, w, gy, gx, dw, db, npix);
+}
+void loss(hipStream_t st, const float* y, const float* t, const float* m, float* denom, float* gy,
+          float* loss_out, int64_t npix_per, int nb) {
+    hipMemsetAsync(denom, 0, sizeof(float) * nb, st);
+    k_target_norm<<<dim3(64, nb), 256, 0, st>>>(t, m, denom, npix_per);
+    k_loss<<<nblk(npix_per * nb), 256, 0, st>>>(y, t, m, denom, gy, loss_out, npix_per, nb);
+}
+void adam(hipStream_t st, int64_t n, float* p, const float* g, float* m, float* v, bf16* pb,
+          float lr, float b1, float b2, float eps, int step, float clip) {
+    const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
+    k_adam<<<nblk(n), 256, 0, st>>>(n, p, g, m, v, pb, lr, b1, b2, eps, c1, c2, clip);
+}
+void to_bf16(hipStream_t st, int64_t n, const float* x, bf16* y) {
+    k_to_bf16<<<nblk(n), 256, 0, st>>>(n, x, y);
+}
+
+}  // namespace nn
+}  // namespace stm

@@ -60,7 +60,17 @@ struct ChanState {
     int32_t exhausted;           // last solve hit itermax
     int32_t pbuf;                // blocked solver: latest p is in the second buffer
     int32_t stop_launch;         // blocked solver: launch whose result ended the solve, -1 running
-    int32_t pad;
+    int32_t budget;              // blocked solver: sweep limit below itermax, 0 = none
+};
+
+// Per-channel state of the learned correction (device copy in LearnChan[]).
+struct LearnChan {
+    double scale;                // 1 / rms of the residual given to the network
+    double alpha;                // step length of the network's correction
+    double dmean, dscale;        // capture: mean and 1 / rms of the pressure change
+    int32_t run;                 // the channel has not converged yet
+    int32_t sweeps;              // sweeps summed over the phases of one solve
+    int32_t pad0, pad1;
 };
 
 // One boundary cell (the reference's sorted_boundary_list entry,

@@ -182,6 +182,14 @@ class Context:
     def reset_stats(self):
         self._check(lib().stm_reset_stats(self._ctx))
 
+    def set_learned(self, net, cycles=1, sweeps=32):
+        """Learned cycles before every red-black solve (net None: off)."""
+        self._check(lib().stm_set_learned(self._ctx, net._net if net else None, cycles, sweeps))
+
+    def set_capture(self, net, sweeps=32):
+        """Store training pairs from every solve in net's training set (None: off)."""
+        self._check(lib().stm_set_capture(self._ctx, net._net if net else None, sweeps))
+
 
 class Simulation:
     """The reference's `Simulation` on one GPU channel.

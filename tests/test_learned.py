@@ -1,0 +1,283 @@
+"""The pressure network and the learned solve.
+
+CPU: the PyTorch statement of the network (tests/unet_ref.py) is pinned --
+its convolution follows the documented tap order (checked against a direct
+loop), its loss has the closed-form values, and its gradient agrees with a
+central finite difference in FP64.
+
+GPU: the library's BF16 matrix-core forward and backward agree with that
+statement to BF16 accuracy; capture fills the training set; training lowers
+the loss; a learned solve ends at the same residual as the plain solve.
+"""
+import numpy as np
+import pytest
+import torch
+
+from tests import unet_ref as R
+
+# ------------------------------------------------------------------ CPU pins
+
+
+def test_ref_conv_follows_the_documented_tap_order():
+    rng = np.random.default_rng(0)
+    nb, cin, cout, H, W = 2, 3, 4, 5, 6
+    t = rng.standard_normal((nb, cin, H, W))
+    w = rng.standard_normal((cout, 9, cin))
+    b = rng.standard_normal(cout)
+    got = R.conv3x3(torch.tensor(t), torch.tensor(w), torch.tensor(b)).numpy()
+    want = np.zeros((nb, cout, H, W))
+    for n in range(nb):
+        for co in range(cout):
+            for h in range(H):
+                for x in range(W):
+                    acc = b[co]
+                    for tap in range(9):
+                        dh, dw = tap // 3 - 1, tap % 3 - 1
+                        if 0 <= h + dh < H and 0 <= x + dw < W:
+                            acc += np.dot(t[n, :, h + dh, x + dw], w[co, tap])
+                    want[n, co, h, x] = acc
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-12)
+
+
+def test_ref_loss_closed_forms():
+    rng = np.random.default_rng(1)
+    x = torch.tensor(rng.standard_normal((2, 8, 8, 2)))
+    x[..., 1] = torch.tensor(rng.random((2, 8, 8)) > 0.3, dtype=torch.float64)
+    t = torch.tensor(rng.standard_normal((2, 8, 8)))
+    assert float(R.loss(t.clone(), x, t)) == 0.0
+    assert abs(float(R.loss(torch.zeros_like(t), x, t)) - 1.0) < 1e-14
+    # only masked cells count
+    y = t.clone()
+    y[x[..., 1] < 0.5] += 100.0
+    assert float(R.loss(y, x, t)) == 0.0
+
+
+def _inputs(nb, H, W, seed):
+    rng = np.random.default_rng(seed)
+    x = np.zeros((nb, H, W, 2), np.float32)
+    x[..., 1] = 1.0
+    for n in range(nb):   # a few rectangles of solid cells
+        for _ in range(2):
+            h0, w0 = rng.integers(0, H - 3), rng.integers(0, W - 3)
+            x[n, h0:h0 + 3, w0:w0 + 3, 1] = 0.0
+    x[..., 0] = rng.standard_normal((nb, H, W)) * x[..., 1]
+    t = (rng.standard_normal((nb, H, W)) * x[..., 1]).astype(np.float32)
+    return x, t
+
+
+def _params(seed, scale_bias=0.1):
+    """He-scaled weights, non-zero biases."""
+    rng = np.random.default_rng(seed)
+    lay, n = R.layout()
+    p = np.zeros(n, np.float32)
+    for name, off, shape in lay:
+        size = int(np.prod(shape))
+        if name.endswith(".w") and name.startswith("conv"):
+            fan = 9 * shape[2]
+            p[off:off + size] = rng.standard_normal(size) * np.sqrt(2.0 / fan)
+        elif name == "head.w":
+            p[off:off + size] = rng.standard_normal(size) * np.sqrt(1.0 / 32)
+        else:
+            p[off:off + size] = rng.standard_normal(size) * scale_bias
+    return p
+
+
+def test_ref_gradient_matches_finite_difference():
+    x, t = _inputs(1, 8, 8, 2)
+    p = _params(3)
+    L0, g = R.gradients(p.astype(np.float64), x, t, dtype=torch.float64)
+    rng = np.random.default_rng(4)
+    d = rng.standard_normal(p.size)
+    lay, _ = R.layout()
+    # the library keeps conv weights in BF16: differentiate the FP64 function
+    # of the rounded weights, i.e. perturb around the rounded point
+    pr = R.unpack(p, dtype=torch.float64)
+    flat = np.zeros(p.size)
+    for name, off, shape in lay:
+        flat[off:off + int(np.prod(shape))] = pr[name].numpy().ravel()
+
+    def f(q):
+        P = R.unpack(q, dtype=torch.float64, bf16_weights=False)
+        xx = R.to_bf16(x).double()
+        return float(R.loss(R.forward(P, xx), xx, torch.tensor(t).double()))
+
+    eps = 1e-6
+    fd = (f(flat + eps * d) - f(flat - eps * d)) / (2 * eps)
+    assert abs(fd - float(g @ d)) <= 1e-6 * max(1.0, abs(fd)), (fd, float(g @ d))
+
+
+# ------------------------------------------------------------------ GPU
+
+
+def _gpu_ctx():
+    from stroemung_amd.simulation import Context
+    return Context(0)
+
+
+@pytest.mark.gpu
+def test_net_layout_and_forward_match_reference():
+    from stroemung_amd.learned import PressureNet
+    ctx = _gpu_ctx()
+    H, W, nb = 32, 24, 3
+    net = PressureNet(ctx, H, W, nb, seed=5)
+    lay, n = R.layout()
+    assert net.nparam == n
+    assert [(a, b, c) for a, b, c in net.layout()] == lay
+    p = _params(6)
+    net.set_params(p)
+    np.testing.assert_array_equal(net.get_params(), p)
+    x, _ = _inputs(nb, H, W, 7)
+    y = net.forward(x)
+    P = R.unpack(p)
+    with torch.no_grad():
+        want = R.forward(P, R.to_bf16(x)).numpy()
+        emu = R.forward(P, R.to_bf16(x), bf16_acts=True).numpy()
+        emu64 = R.forward(R.unpack(p, dtype=torch.float64), R.to_bf16(x).double(),
+                          bf16_acts=True).numpy()
+    n = np.linalg.norm
+    err, err16 = n(y - want) / n(want), n(y - emu) / n(emu)
+    # the noise floor of BF16 storage: the same storage with FP32 or FP64
+    # accumulation already differs by this much (rounding flips propagate)
+    floor = n(emu - emu64) / n(emu64)
+    print("forward relative L2 error: %.3e vs fp32, %.3e vs bf16-stored (floor %.3e)"
+          % (err, err16, floor))
+    assert err < 3e-2
+    assert err16 < 2 * floor + 1e-3
+    net.close()
+    ctx.close()
+
+
+@pytest.mark.gpu
+def test_net_gradients_match_reference():
+    from stroemung_amd.learned import PressureNet
+    ctx = _gpu_ctx()
+    H, W, nb = 32, 32, 2
+    net = PressureNet(ctx, H, W, nb, seed=8)
+    p = _params(9)
+    net.set_params(p)
+    x, t = _inputs(nb, H, W, 10)
+    loss, g = net.gradients(x, t)
+    Lr, gr = R.gradients(p, x, t)
+    print("loss gpu %.6f ref fp32 %.6f" % (loss, Lr))
+    assert abs(loss - Lr) <= 2e-2 * abs(Lr)
+    for name, off, shape in R.layout()[0]:
+        size = int(np.prod(shape))
+        cos, rel = _cmp(g[off:off + size], gr[off:off + size])
+        print("%-10s cos %.5f rel %.3e" % (name, cos, rel))
+        # BF16 storage of ~20 activations and of their gradients: ReLU and
+        # pooling decisions flip and the flips propagate, hence a loose bound
+        assert cos > 0.98 and rel < 0.2, name
+    # the padding of the first layer's input channels gets no gradient
+    w0 = g[0:32 * 9 * 32].reshape(32, 9, 32)
+    assert np.all(w0[:, :, 2:] == 0.0)
+    net.close()
+    ctx.close()
+
+
+def _cmp(a, b):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    cos = float(a @ b / (np.linalg.norm(a) * np.linalg.norm(b) + 1e-300))
+    return cos, float(np.linalg.norm(a - b) / (np.linalg.norm(b) + 1e-300))
+
+
+def _sparse_params(seed):
+    """Each output channel reads one (tap, input channel) with weight +-1 and
+    an integer bias; the head has weights in {0, +-1, 1/2}.  With integer
+    inputs every activation is a small integer, exact in BF16, so the forward
+    pass has no rounding at all and must match bit for bit."""
+    rng = np.random.default_rng(seed)
+    lay, n = R.layout()
+    p = np.zeros(n, np.float32)
+    real_cin = {0: 2}
+    for name, off, shape in lay:
+        size = int(np.prod(shape))
+        if name.startswith("conv") and name.endswith(".w"):
+            k = int(name[4:-2])
+            cout, _, cin = shape
+            w = np.zeros(shape, np.float32)
+            for co in range(cout):
+                w[co, rng.integers(0, 9), rng.integers(0, real_cin.get(k, cin))] = \
+                    1.0 if rng.random() < 0.8 else -1.0
+            p[off:off + size] = w.ravel()
+        elif name.startswith("conv"):
+            p[off:off + size] = rng.integers(-1, 2, size)
+        elif name == "head.w":
+            p[off:off + size] = rng.choice([0.0, 1.0, -1.0, 0.5], size)
+        else:
+            p[off:off + size] = 0.5
+    return p
+
+
+@pytest.mark.gpu
+def test_net_exact_arithmetic_forward_is_bit_identical_and_gradients_tight():
+    """Integer activations: no rounding in the forward pass, so any index
+    mistake (tap, channel, pooling window, upsampling, concatenation order,
+    head) shows as a difference; the backward pass then rounds only its
+    gradients to BF16, at the places the emulated reference does."""
+    from stroemung_amd.learned import PressureNet
+    ctx = _gpu_ctx()
+    H, W, nb = 32, 40, 2
+    net = PressureNet(ctx, H, W, nb, seed=1)
+    p = _sparse_params(13)
+    net.set_params(p)
+    rng = np.random.default_rng(14)
+    x, _ = _inputs(nb, H, W, 15)
+    x[..., 0] = rng.integers(-4, 5, (nb, H, W)) * x[..., 1]
+    y = net.forward(x)
+    with torch.no_grad():
+        want = R.forward(R.unpack(p), torch.tensor(x)).numpy()
+    assert np.count_nonzero(want) > want.size // 4
+    np.testing.assert_array_equal(y, want)
+    t = (rng.integers(-3, 4, (nb, H, W)) * x[..., 1]).astype(np.float32)
+    loss, g = net.gradients(x, t)
+    L16, g16 = R.gradients(p, x, t, bf16_acts=True)
+    assert abs(loss - L16) <= 1e-5 * abs(L16)
+    for name, off, shape in R.layout()[0]:
+        size = int(np.prod(shape))
+        cos, rel = _cmp(g[off:off + size], g16[off:off + size])
+        print("%-10s cos %.7f rel %.3e" % (name, cos, rel))
+        assert rel < 1e-3, name
+
+
+def _cavity_context(count, n, seed, eps=1e-6):
+    from stroemung_amd import workloads
+    from stroemung_amd.simulation import Context
+    ctx = Context(0)
+    for ct, vel, prm in workloads.random_cavities(count, n=n, seed=seed, eps=eps):
+        ctx.add_grid(prm, ct, vel)
+    ctx.construct()
+    return ctx
+
+
+@pytest.mark.gpu
+def test_capture_train_and_learned_solve_reach_the_same_residual():
+    from stroemung_amd.learned import PressureNet
+    n, count = 64, 4
+    ctx = _cavity_context(count, n, seed=11)
+    net = PressureNet(ctx, n, n, 32, seed=12)
+    net.reserve(64)
+    ctx.set_capture(net, sweeps=16)
+    for _ in range(6):
+        its, _ = ctx.tick()
+        for c in range(count):
+            s = ctx.state(c)
+            assert s["residual_squared"] <= s["tolerance_squared"] and not s["exhausted"]
+    ctx.set_capture(None)
+    assert net.dataset_size() == 6 * 2 * count
+    first = net.train(20, 16, lr=1e-3, seed=1)
+    last = net.train(200, 16, lr=1e-3, seed=2)
+    print("training loss %.4f -> %.4f" % (first, last))
+    assert last < first
+    # learned solve: same stopping test, so the same residual bound
+    ctx.set_learned(net, cycles=2, sweeps=16)
+    for _ in range(3):
+        its, _ = ctx.tick()
+        for c in range(count):
+            s = ctx.state(c)
+            assert s["residual_squared"] <= s["tolerance_squared"] and not s["exhausted"]
+            assert s["sor_iterations"] == its[c]
+    st = net.stats()
+    assert st["forward_calls"] >= 3 and st["forward_ms"] > 0
+    ctx.set_learned(None)
+    net.close()
+    ctx.close()

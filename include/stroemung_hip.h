@@ -197,10 +197,10 @@ int stm_reset_stats(stm_ctx *ctx);
  * nearest upsampling with skip concatenation, 1x1 head) in BF16 with FP32
  * accumulation on the matrix cores and FP32 master weights trained with Adam.
  * It maps the normalised residual of the pressure equation (channel 0) and the
- * interior-fluid mask (channel 1) of an H x W grid to a correction of p.  H
- * and W are multiples of 8; interior cell (x, y) of a channel is pixel
- * (x - 1, y - 1), the rest of the H x W frame is zero.  The network lives on
- * the context's device and stream. */
+ * fluid fraction (channel 1) of an H x W grid to a correction of p (see
+ * stm_set_learned for how a channel's grid maps onto the frame).  H and W
+ * are multiples of 8.  The network lives on the context's device and
+ * stream. */
 typedef struct stm_net stm_net;
 
 typedef struct stm_net_stats_t {
@@ -230,13 +230,15 @@ int stm_net_set_params(stm_net *net, const float *in);
 /* x: nb x H x W x 2 (fp32) -> y: nb x H x W (fp32) */
 int stm_net_forward(stm_net *net, uint32_t nb, const float *x, float *y);
 /* The masked relative squared error of the forward pass against t
- * (nb x H x W): loss = mean_s sum m (y - t)^2 / sum m t^2 with m = x[..., 1],
+ * (nb x H x W): loss = mean_s sum m (y - t)^2 / sum m t^2 with m = x[..., 1] > 0,
  * and its gradient with respect to every parameter (layout as above). */
 int stm_net_gradients(stm_net *net, uint32_t nb, const float *x, const float *t, float *loss,
                       float *grads);
 /* Device-resident training set of `capacity` samples, filled by capture. */
 int stm_net_dataset_reserve(stm_net *net, uint32_t capacity);
 int stm_net_dataset_size(stm_net *net, uint32_t *size);
+/* Sample `index` of the training set: x (H x W x 2) and t (H x W), fp32. */
+int stm_net_dataset_get(stm_net *net, uint32_t index, float *x, float *t);
 /* `steps` Adam steps on random batches of the training set; *last_loss is the
  * mean loss of the last 8 steps. */
 int stm_net_train(stm_net *net, uint32_t steps, uint32_t batch, float lr, uint64_t seed,
@@ -249,20 +251,35 @@ int stm_net_benchmark(stm_net *net, uint32_t nb, uint32_t reps, double *fwd_ms,
                       double *train_ms);
 
 /* ---- learned pressure solve (no reference counterpart)
- * With a net set, every red-black solve of the context runs `cycles` learned
- * cycles first -- residual, network correction p += a y with the a that
- * minimises the residual norm, then `sweeps` red-black sweeps -- and then the
- * plain red-black solve to the same eps.  The stopping test and the FP64
- * arithmetic of the sweeps are those of the plain solver, so the result meets
- * the same residual.  net = NULL or cycles = 0 turns it off.  The net must
- * come from this context and hold at least (nx - 2) x (ny - 2) pixels and
- * one batch entry per channel. */
-int stm_set_learned(stm_ctx *ctx, stm_net *net, uint32_t cycles, uint32_t sweeps);
-/* With a net set, every red-black solve also stores two training pairs per
- * channel in the net's training set (ring order): the residual at the start
- * of the solve and after `sweeps` sweeps, each with the normalised pressure
- * change the solve then made.  net = NULL turns it off. */
-int stm_set_capture(stm_ctx *ctx, stm_net *net, uint32_t sweeps);
+ * With a net set, every red-black solve of the context runs `cycles`
+ * two-grid cycles first and then the plain red-black solve to the same eps.
+ * A cycle restricts the residual r to a grid coarser by `coarsen` (1, 2, 4
+ * or 8: pixel = mean of r over its coarsen^2 interior cells), normalises it,
+ * runs the network there, prolongs its output y bilinearly to a correction z,
+ * sets p += a z and runs `sweeps` red-black sweeps.  The network is trained
+ * to output the restricted error times rms(R r) / h_c^2 (h_c^2 = coarsen^2
+ * dx dy); step STM_STEP_NETWORK takes that magnitude (a = h_c^2 / rms(R r)),
+ * STM_STEP_ENERGY the a that minimises the energy norm of the new error
+ * (a = <r, z> / <z, -Lap z>).  The stopping test and the FP64 arithmetic of
+ * the sweeps are those of the plain solver, so the result meets the same
+ * residual.  net = NULL or cycles = 0 turns it off.  The net must come from
+ * this context, hold one batch entry per channel and an H x W frame with
+ * coarsen * H >= nx - 2 and coarsen * W >= ny - 2 (interior cell (x, y)
+ * falls in pixel ((x - 1) / coarsen, (y - 1) / coarsen)). */
+#define STM_STEP_ENERGY 0
+#define STM_STEP_NETWORK 1
+int stm_set_learned(stm_ctx *ctx, stm_net *net, uint32_t cycles, uint32_t sweeps, int32_t step,
+                    uint32_t coarsen);
+/* With a net set, every red-black solve also stores `points` (1..8)
+ * training pairs per channel in the net's training set (ring order): the
+ * residual, restricted as for stm_set_learned, at `points` moments of the
+ * solve, each with the restricted mean-free change of p from that moment to
+ * the converged solution, in the units of the network's output.  A plain
+ * solve is sampled at 0, s, 3 s, 7 s, ... sweeps (s = `sweeps`); a learned
+ * solve (same net and coarsening) at the start of its first `points`
+ * cycles, before the network's correction.  net = NULL turns it off. */
+int stm_set_capture(stm_ctx *ctx, stm_net *net, uint32_t points, uint32_t sweeps,
+                    uint32_t coarsen);
 
 #ifdef __cplusplus
 }

@@ -117,11 +117,12 @@ SIGNATURES = {
     "stm_net_gradients": (C.c_int, [_NET, C.c_uint32, _F, _F, _F, _F]),
     "stm_net_dataset_reserve": (C.c_int, [_NET, C.c_uint32]),
     "stm_net_dataset_size": (C.c_int, [_NET, C.POINTER(C.c_uint32)]),
+    "stm_net_dataset_get": (C.c_int, [_NET, C.c_uint32, _F, _F]),
     "stm_net_train": (C.c_int, [_NET, C.c_uint32, C.c_uint32, C.c_float, C.c_uint64, _F]),
     "stm_net_stats": (C.c_int, [_NET, C.POINTER(NetStats)]),
     "stm_net_benchmark": (C.c_int, [_NET, C.c_uint32, C.c_uint32, _D, _D]),
-    "stm_set_learned": (C.c_int, [_CTX, _NET, C.c_uint32, C.c_uint32]),
-    "stm_set_capture": (C.c_int, [_CTX, _NET, C.c_uint32]),
+    "stm_set_learned": (C.c_int, [_CTX, _NET, C.c_uint32, C.c_uint32, C.c_int32, C.c_uint32]),
+    "stm_set_capture": (C.c_int, [_CTX, _NET, C.c_uint32, C.c_uint32, C.c_uint32]),
 }
 
 _lib = None

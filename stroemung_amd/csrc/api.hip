@@ -83,12 +83,16 @@ struct stm_ctx {
     int be_capacity = 0;
     // learned correction and capture of training pairs
     stm_net* lnet = nullptr;       // learned cycles before the plain solve
-    int lcycles = 0, lsweeps = 0;
+    int lcycles = 0, lsweeps = 0, lstep = 0, lcoarsen = 1;
     stm_net* cnet = nullptr;       // capture into this net's training set
-    int csweeps = 0;
+    int csweeps = 0, ccoarsen = 1, cpoints = 2;
     LearnChan* d_lc = nullptr;
     LearnChan* h_lc = nullptr;     // pinned
-    double *lres = nullptr, *lcf = nullptr, *lp0 = nullptr, *lpk = nullptr;
+    double *lres = nullptr, *lcf = nullptr;
+    double* lsnap = nullptr;       // capture: LEARN_POINTS snapshots of p
+    float *lrc = nullptr, *lmc = nullptr;   // coarse residual and fluid fraction, nch frames
+    double2* d_cpart = nullptr;
+    int64_t lframe = 0;            // pixels per frame the coarse buffers hold
     hipEvent_t nn_ev[2] = {nullptr, nullptr};
     // profiling
     uint32_t sample = 0;
@@ -596,25 +600,44 @@ int stage_rhs_stats(stm_ctx* ctx) {
 
 // ------------------------------------------------------------- learned solve
 
-// Ghosts of p, the residual, and per channel: converged?  the input scale.
+// Ghosts of p, the fine residual, and per channel: converged?
 int learn_prepare(stm_ctx* ctx, int first) {
     LAUNCH("learn_ghost", launch_ghost(ctx->stream, ctx->nbtot, ctx->d_be, ctx->d_cs, ctx->p, -1,
                                        0));
     LAUNCH("learn_residual", launch_learn_residual(ctx->stream, ctx->ntiles, ctx->d_cd, ctx->d_cs,
                                                    ctx->d_tiles, ctx->flags, ctx->rhs, ctx->p,
                                                    ctx->lres, ctx->d_part));
-    LAUNCH("learn_scale", launch_learn_scale(ctx->stream, nch(ctx), ctx->d_cd, ctx->d_cs,
+    LAUNCH("learn_check", launch_learn_check(ctx->stream, nch(ctx), ctx->d_cd, ctx->d_cs,
                                              ctx->d_lc, ctx->d_part, first));
     return STM_OK;
 }
 
+// The coarse residual of every channel in the net's frame and its scale.
+int learn_restrict(stm_ctx* ctx, const Net& net, int f, int save) {
+    LAUNCH("learn_restrict", launch_learn_restrict(ctx->stream, nch(ctx), ctx->d_cd, ctx->d_lc,
+                                                   ctx->flags, ctx->lres, net.H, net.W, f,
+                                                   ctx->lrc, ctx->lmc, ctx->d_cpart, save));
+    return STM_OK;
+}
+
 // Blocked red-black launches on the channels still running, `budget` sweeps
-// each (0 = up to itermax); final = 1 ends the solve.
+// each (0 = up to itermax); final = 1 ends the solve.  A phase with a budget
+// needs no polling: ceil(budget / K) launches sweep and one more decides, and
+// launches on a stopped channel return at once.
 int learn_phase(stm_ctx* ctx, int budget, int final_phase) {
     LAUNCH("learn_arm", launch_learn_arm(ctx->stream, nch(ctx), ctx->d_cd, ctx->d_cs, ctx->d_lc,
                                          ctx->d_active, budget));
-    int rc = blocked_solve(ctx, 1);
-    if (rc) return rc;
+    if (budget > 0) {
+        const int K = TB_VARIANTS[ctx->variant].k;
+        const int nl = (budget + K - 1) / K + 1;
+        for (int L = 0; L < nl; L++)
+            LAUNCH("rb_tb", launch_blocked(ctx, ctx->variant, L, TB_CHECK));
+        LAUNCH("pbuf_fix", launch_pbuf_fix(ctx->stream, ctx->ntiles, nch(ctx), ctx->d_cd,
+                                           ctx->d_cs, ctx->d_tiles, ctx->p2, ctx->p));
+    } else {
+        int rc = blocked_solve(ctx, 1);
+        if (rc) return rc;
+    }
     LAUNCH("learn_collect", launch_learn_collect(ctx->stream, nch(ctx), ctx->d_cs, ctx->d_lc,
                                                  final_phase));
     return STM_OK;
@@ -629,81 +652,93 @@ int any_running(stm_ctx* ctx, bool* any) {
     return check_launch(ctx);
 }
 
-// `lcycles` cycles of network correction and `lsweeps` sweeps, then the
-// plain solve to eps.
-int learned_solve(stm_ctx* ctx) {
-    Net& net = ctx->lnet->net;
+// The network's correction of p: forward on the packed coarse residual,
+// prolongation, ghosts, step length, update.
+int learn_correct(stm_ctx* ctx, Net& net, int f) {
     const int n = nch(ctx);
-    int rc;
-    for (int cyc = 0; cyc < ctx->lcycles; cyc++) {
-        if ((rc = learn_prepare(ctx, cyc == 0))) return rc;
-        LAUNCH("learn_pack", launch_learn_pack(ctx->stream, n, ctx->d_cd, ctx->d_lc, ctx->flags,
-                                               ctx->lres, net.H, net.W, net.input(), nullptr, 0,
-                                               1));
-        HIPCHK(ctx, hipEventRecord(ctx->nn_ev[0], ctx->stream));
-        LAUNCH("nn_forward", net.forward(n));
-        HIPCHK(ctx, hipEventRecord(ctx->nn_ev[1], ctx->stream));
-        LAUNCH("learn_unpack", launch_learn_unpack(ctx->stream, ctx->ntiles, ctx->d_cd,
-                                                   ctx->d_tiles, ctx->d_lc, ctx->flags, net.y,
-                                                   net.H, net.W, ctx->lcf));
-        LAUNCH("learn_ghost", launch_ghost(ctx->stream, ctx->nbtot, ctx->d_be, ctx->d_cs,
-                                           ctx->lcf, -1, 0));
-        LAUNCH("learn_dots", launch_learn_dots(ctx->stream, ctx->ntiles, ctx->d_cd, ctx->d_tiles,
-                                               ctx->flags, ctx->lres, ctx->lcf, ctx->d_part));
-        LAUNCH("learn_alpha", launch_learn_alpha(ctx->stream, n, ctx->d_cd, ctx->d_lc,
-                                                 ctx->d_part));
-        LAUNCH("learn_apply", launch_learn_apply(ctx->stream, ctx->ntiles, ctx->d_cd,
-                                                 ctx->d_tiles, ctx->d_lc, ctx->flags, ctx->lcf,
-                                                 ctx->p));
-        if ((rc = learn_phase(ctx, ctx->lsweeps, 0))) return rc;
-        float ms = 0.f;
-        HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->nn_ev[0], ctx->nn_ev[1]));
-        net.fwd_ms += ms;
-        net.fwd_calls += 1;
-        net.fwd_samples += (uint64_t)n;
-        bool any = false;
-        if ((rc = any_running(ctx, &any))) return rc;
-        if (!any) break;
-    }
-    return learn_phase(ctx, 0, 1);
-}
-
-// A plain solve split in two phases that stores two training pairs per
-// channel: the residual at the start and after `csweeps` sweeps, with the
-// normalised change of p from there to the converged solution.
-int capture_solve(stm_ctx* ctx) {
-    Net& net = ctx->cnet->net;
-    const int n = nch(ctx);
-    const size_t bytes = sizeof(double) * (size_t)ctx->total;
-    const int slot0 = net.dwrite, slot1 = (net.dwrite + n) % net.dcap;
-    int rc;
-    if ((rc = learn_prepare(ctx, 1))) return rc;
-    LAUNCH("learn_pack", launch_learn_pack(ctx->stream, n, ctx->d_cd, ctx->d_lc, ctx->flags,
-                                           ctx->lres, net.H, net.W, nullptr, net.dx, slot0,
-                                           net.dcap));
-    HIPCHK(ctx, hipMemcpyAsync(ctx->lp0, ctx->p, bytes, hipMemcpyDeviceToDevice, ctx->stream));
-    if ((rc = learn_phase(ctx, ctx->csweeps, 0))) return rc;
-    if ((rc = learn_prepare(ctx, 0))) return rc;
-    LAUNCH("learn_pack", launch_learn_pack(ctx->stream, n, ctx->d_cd, ctx->d_lc, ctx->flags,
-                                           ctx->lres, net.H, net.W, nullptr, net.dx, slot1,
-                                           net.dcap));
-    HIPCHK(ctx, hipMemcpyAsync(ctx->lpk, ctx->p, bytes, hipMemcpyDeviceToDevice, ctx->stream));
-    if ((rc = learn_phase(ctx, 0, 1))) return rc;
-    LAUNCH("learn_target", launch_learn_target(ctx->stream, ctx->ntiles, n, ctx->d_cd,
-                                               ctx->d_tiles, ctx->d_lc, ctx->flags, ctx->p,
-                                               ctx->lp0, ctx->d_part, net.H, net.W, net.dt,
-                                               slot0, net.dcap));
-    LAUNCH("learn_target", launch_learn_target(ctx->stream, ctx->ntiles, n, ctx->d_cd,
-                                               ctx->d_tiles, ctx->d_lc, ctx->flags, ctx->p,
-                                               ctx->lpk, ctx->d_part, net.H, net.W, net.dt,
-                                               slot1, net.dcap));
-    net.dwrite = (net.dwrite + 2 * n) % net.dcap;
-    net.dsize = std::min(net.dsize + 2 * n, net.dcap);
+    LAUNCH("learn_pack", launch_learn_pack(ctx->stream, n, ctx->d_lc, ctx->lrc, ctx->lmc, net.H,
+                                           net.W, net.input(), nullptr, 0, 1));
+    HIPCHK(ctx, hipEventRecord(ctx->nn_ev[0], ctx->stream));
+    LAUNCH("nn_forward", net.forward(n));
+    HIPCHK(ctx, hipEventRecord(ctx->nn_ev[1], ctx->stream));
+    LAUNCH("learn_prolong", launch_learn_prolong(ctx->stream, ctx->ntiles, ctx->d_cd,
+                                                 ctx->d_tiles, ctx->d_lc, ctx->flags, net.y,
+                                                 ctx->lmc, net.H, net.W, f, ctx->lcf));
+    LAUNCH("learn_ghost", launch_ghost(ctx->stream, ctx->nbtot, ctx->d_be, ctx->d_cs, ctx->lcf,
+                                       -1, 0));
+    LAUNCH("learn_dots", launch_learn_dots(ctx->stream, ctx->ntiles, ctx->d_cd, ctx->d_tiles,
+                                           ctx->flags, ctx->lres, ctx->lcf, ctx->d_part));
+    LAUNCH("learn_alpha", launch_learn_alpha(ctx->stream, n, ctx->d_cd, ctx->d_lc, ctx->d_part,
+                                             ctx->lstep, f));
+    LAUNCH("learn_apply", launch_learn_apply(ctx->stream, ctx->ntiles, ctx->d_cd, ctx->d_tiles,
+                                             ctx->d_lc, ctx->flags, ctx->lcf, ctx->p));
     return STM_OK;
 }
 
-// Buffers of the learned solve (and of capture when `capture`).
-int learn_buffers(stm_ctx* ctx, bool capture) {
+// Red-black solve in phases.  Phase k starts with the residual and the
+// convergence test; when capturing, its first `cpoints` phases each store
+// a training input and a snapshot of p; with the learned solve on, each
+// applies the network's correction.  The learned solve runs `lcycles`
+// phases of `lsweeps` sweeps, a plain capture `cpoints` phases of csweeps,
+// 2 csweeps, 4 csweeps, ...; a final phase runs to eps.  Captured targets
+// are the restricted change from each snapshot to the converged p.
+int phased_solve(stm_ctx* ctx) {
+    Net* L = (ctx->lnet && ctx->lcycles > 0) ? &ctx->lnet->net : nullptr;
+    Net* C = ctx->cnet ? &ctx->cnet->net : nullptr;
+    if (L && C && (L != C || ctx->lcoarsen != ctx->ccoarsen))
+        return fail(ctx, STM_ERR_STATE,
+                    "capture during a learned solve needs the same net and coarsening");
+    const int n = nch(ctx);
+    const int f = L ? ctx->lcoarsen : ctx->ccoarsen;
+    const int phases = L ? ctx->lcycles : ctx->cpoints;
+    const size_t bytes = sizeof(double) * (size_t)ctx->total;
+    const int slot_base = C ? C->dwrite : 0;
+    int captured = 0, rc;
+    for (int k = 0; k < phases; k++) {
+        if ((rc = learn_prepare(ctx, k == 0))) return rc;
+        const bool cap = C && captured < ctx->cpoints;
+        if ((rc = learn_restrict(ctx, L ? *L : *C, f, cap ? captured : -1))) return rc;
+        if (cap) {
+            LAUNCH("learn_pack", launch_learn_pack(ctx->stream, n, ctx->d_lc, ctx->lrc, ctx->lmc,
+                                                   C->H, C->W, nullptr, C->dx,
+                                                   slot_base + captured * n, C->dcap));
+            HIPCHK(ctx, hipMemcpyAsync(ctx->lsnap + (size_t)captured * ctx->total, ctx->p, bytes,
+                                       hipMemcpyDeviceToDevice, ctx->stream));
+            captured++;
+        }
+        if (L && (rc = learn_correct(ctx, *L, f))) return rc;
+        const int budget = L ? ctx->lsweeps : ctx->csweeps << k;
+        if ((rc = learn_phase(ctx, budget, 0))) return rc;
+        if (L) {
+            float ms = 0.f;
+            HIPCHK(ctx, hipEventSynchronize(ctx->nn_ev[1]));
+            HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->nn_ev[0], ctx->nn_ev[1]));
+            L->fwd_ms += ms;
+            L->fwd_calls += 1;
+            L->fwd_samples += (uint64_t)n;
+        }
+        if (L && k % 4 == 3) {
+            bool any = false;
+            if ((rc = any_running(ctx, &any))) return rc;
+            if (!any) break;
+        }
+    }
+    if ((rc = learn_phase(ctx, 0, 1))) return rc;
+    for (int j = 0; j < captured; j++)
+        LAUNCH("learn_target", launch_learn_target(ctx->stream, ctx->ntiles, n, ctx->d_cd,
+                                                   ctx->d_tiles, ctx->d_lc, ctx->flags, ctx->p,
+                                                   ctx->lsnap + (size_t)j * ctx->total,
+                                                   ctx->d_part, C->H, C->W, f, C->dt,
+                                                   slot_base + j * n, C->dcap, j));
+    if (C) {
+        C->dwrite = (C->dwrite + captured * n) % C->dcap;
+        C->dsize = std::min(C->dsize + captured * n, C->dcap);
+    }
+    return STM_OK;
+}
+
+// Buffers of the learned solve for a net of H x W (and of capture).
+int learn_buffers(stm_ctx* ctx, const Net& net, bool capture) {
     const size_t bytes = sizeof(double) * (size_t)ctx->total;
     if (!ctx->d_lc) {
         HIPCHK(ctx, hipMalloc(&ctx->d_lc, sizeof(LearnChan) * nch(ctx)));
@@ -716,28 +751,41 @@ int learn_buffers(stm_ctx* ctx, bool capture) {
         HIPCHK(ctx, hipEventCreate(&ctx->nn_ev[0]));
         HIPCHK(ctx, hipEventCreate(&ctx->nn_ev[1]));
     }
-    if (capture && !ctx->lp0) {
-        HIPCHK(ctx, hipMalloc(&ctx->lp0, bytes));
-        HIPCHK(ctx, hipMalloc(&ctx->lpk, bytes));
+    const int64_t frame = (int64_t)net.H * net.W;
+    if (frame > ctx->lframe) {
+        if (ctx->lrc) hipFree(ctx->lrc);
+        if (ctx->lmc) hipFree(ctx->lmc);
+        if (ctx->d_cpart) hipFree(ctx->d_cpart);
+        ctx->lrc = ctx->lmc = nullptr;
+        ctx->d_cpart = nullptr;
+        ctx->lframe = 0;
+        HIPCHK(ctx, hipMalloc(&ctx->lrc, sizeof(float) * frame * nch(ctx)));
+        HIPCHK(ctx, hipMalloc(&ctx->lmc, sizeof(float) * frame * nch(ctx)));
+        HIPCHK(ctx, hipMalloc(&ctx->d_cpart, sizeof(double2) * learn_coarse_blocks(net.H, net.W) *
+                                                 (size_t)nch(ctx)));
+        ctx->lframe = frame;
     }
+    if (capture && !ctx->lsnap) HIPCHK(ctx, hipMalloc(&ctx->lsnap, bytes * LEARN_POINTS));
     return STM_OK;
 }
 
 // The net fits the context: same device and stream, every channel's
-// interior fits the net's H x W, a batch entry per channel, all channels
-// red-black.
-int check_net(stm_ctx* ctx, stm_net* net) {
+// interior fits f times the net's H x W, a batch entry per channel, all
+// channels red-black.
+int check_net(stm_ctx* ctx, stm_net* net, uint32_t f) {
     const Net& m = net->net;
     if (m.ctx != ctx) return fail(ctx, STM_ERR_INVALID_ARG, "the net belongs to another context");
+    if (f != 1 && f != 2 && f != 4 && f != 8)
+        return fail(ctx, STM_ERR_INVALID_ARG, "coarsening %u is not 1, 2, 4 or 8", f);
     if (m.cap < nch(ctx))
         return fail(ctx, STM_ERR_INVALID_ARG, "net batch %d below the channel count %d", m.cap,
                     nch(ctx));
     for (size_t k = 0; k < ctx->chans.size(); k++) {
         const HostChan& c = ctx->chans[k];
-        if ((int)c.nx - 2 > m.H || (int)c.ny - 2 > m.W)
+        if ((int64_t)c.nx - 2 > (int64_t)f * m.H || (int64_t)c.ny - 2 > (int64_t)f * m.W)
             return fail(ctx, STM_ERR_INVALID_ARG,
-                        "channel %zu (interior %ux%u) exceeds the net's %dx%d", k, c.nx - 2,
-                        c.ny - 2, m.H, m.W);
+                        "channel %zu (interior %ux%u) exceeds %u x the net's %dx%d", k, c.nx - 2,
+                        c.ny - 2, f, m.H, m.W);
         if (c.prm.solver != STM_SOLVER_RB)
             return fail(ctx, STM_ERR_INVALID_ARG, "channel %zu is not red-black", k);
     }
@@ -756,8 +804,7 @@ int stage_solve(stm_ctx* ctx) {
                                              ctx->d_be, ctx->flags, ctx->rhs, ctx->p,
                                              ctx->scratch));
     if (!ctx->any_rb) return STM_OK;
-    if (ctx->cnet) return capture_solve(ctx);
-    if (ctx->lnet && ctx->lcycles > 0) return learned_solve(ctx);
+    if (ctx->cnet || (ctx->lnet && ctx->lcycles > 0)) return phased_solve(ctx);
     return blocked_solve(ctx, 1);
 }
 
@@ -867,7 +914,7 @@ void stm_destroy(stm_ctx* ctx) {
     void* bufs[] = {ctx->u, ctx->v, ctx->p, ctx->f, ctx->g, ctx->rhs, ctx->scratch, ctx->p2,
                     ctx->flags, ctx->d_cd, ctx->d_cs, ctx->d_be, ctx->d_restore, ctx->d_tiles,
                     ctx->d_part, ctx->d_active, ctx->d_tbpart, ctx->d_tb, ctx->d_lc, ctx->lres,
-                    ctx->lcf, ctx->lp0, ctx->lpk};
+                    ctx->lcf, ctx->lsnap, ctx->lrc, ctx->lmc, ctx->d_cpart};
     for (void* b : bufs)
         if (b) hipFree(b);
     if (ctx->h_active) hipHostFree(ctx->h_active);
@@ -1300,7 +1347,8 @@ int stm_kernel_stats(stm_ctx* ctx, const char* name, double* total_ms, uint64_t*
     return STM_OK;
 }
 
-int stm_set_learned(stm_ctx* ctx, stm_net* net, uint32_t cycles, uint32_t sweeps) {
+int stm_set_learned(stm_ctx* ctx, stm_net* net, uint32_t cycles, uint32_t sweeps, int32_t step,
+                    uint32_t coarsen) {
     int rc = ready(ctx);
     if (rc) return rc;
     if (!net || cycles == 0) {
@@ -1309,15 +1357,20 @@ int stm_set_learned(stm_ctx* ctx, stm_net* net, uint32_t cycles, uint32_t sweeps
         return STM_OK;
     }
     if (sweeps == 0) return fail(ctx, STM_ERR_INVALID_ARG, "sweeps per cycle must be positive");
-    if ((rc = check_net(ctx, net))) return rc;
-    if ((rc = learn_buffers(ctx, false))) return rc;
+    if (step != STM_STEP_ENERGY && step != STM_STEP_NETWORK)
+        return fail(ctx, STM_ERR_INVALID_ARG, "unknown step rule %d", step);
+    if ((rc = check_net(ctx, net, coarsen))) return rc;
+    if ((rc = learn_buffers(ctx, net->net, false))) return rc;
+    ctx->lstep = step;
+    ctx->lcoarsen = (int)coarsen;
     ctx->lnet = net;
     ctx->lcycles = (int)cycles;
     ctx->lsweeps = (int)sweeps;
     return STM_OK;
 }
 
-int stm_set_capture(stm_ctx* ctx, stm_net* net, uint32_t sweeps) {
+int stm_set_capture(stm_ctx* ctx, stm_net* net, uint32_t points, uint32_t sweeps,
+                    uint32_t coarsen) {
     int rc = ready(ctx);
     if (rc) return rc;
     if (!net) {
@@ -1325,14 +1378,18 @@ int stm_set_capture(stm_ctx* ctx, stm_net* net, uint32_t sweeps) {
         return STM_OK;
     }
     if (sweeps == 0) return fail(ctx, STM_ERR_INVALID_ARG, "sweeps must be positive");
-    if ((rc = check_net(ctx, net))) return rc;
-    if (net->net.dcap < 2 * nch(ctx))
+    if (points == 0 || points > (uint32_t)LEARN_POINTS)
+        return fail(ctx, STM_ERR_INVALID_ARG, "points must be 1..%d", LEARN_POINTS);
+    if ((rc = check_net(ctx, net, coarsen))) return rc;
+    if (net->net.dcap < (int)points * nch(ctx))
         return fail(ctx, STM_ERR_INVALID_ARG,
                     "training set of %d samples cannot take the %d pairs of one solve",
-                    net->net.dcap, 2 * nch(ctx));
-    if ((rc = learn_buffers(ctx, true))) return rc;
+                    net->net.dcap, (int)points * nch(ctx));
+    if ((rc = learn_buffers(ctx, net->net, true))) return rc;
     ctx->cnet = net;
     ctx->csweeps = (int)sweeps;
+    ctx->ccoarsen = (int)coarsen;
+    ctx->cpoints = (int)points;
     return STM_OK;
 }
 

@@ -59,16 +59,22 @@ void launch_pbuf_fix(hipStream_t, int ntiles, int nchan, const ChanDesc*, ChanSt
 void launch_learn_residual(hipStream_t, int ntiles, const ChanDesc*, const ChanState*,
                            const Tile*, const uint8_t* flags, const double* rhs, const double* p,
                            double* res, double* part);
-void launch_learn_scale(hipStream_t, int nch, const ChanDesc*, ChanState*, LearnChan*,
+void launch_learn_check(hipStream_t, int nch, const ChanDesc*, ChanState*, LearnChan*,
                         const double* part, int first);
-void launch_learn_pack(hipStream_t, int nch, const ChanDesc*, const LearnChan*,
-                       const uint8_t* flags, const double* res, int H, int W, __bf16* a0,
-                       __bf16* dx, int slot0, int dcap);
-void launch_learn_unpack(hipStream_t, int ntiles, const ChanDesc*, const Tile*, const LearnChan*,
-                         const uint8_t* flags, const float* y, int H, int W, double* cf);
+// pixel blocks of an H x W frame (the size of cpart per channel)
+int learn_coarse_blocks(int H, int W);
+void launch_learn_restrict(hipStream_t, int nch, const ChanDesc*, LearnChan*,
+                           const uint8_t* flags, const double* res, int H, int W, int f,
+                           float* rc, float* mc, double2* cpart, int save);
+void launch_learn_pack(hipStream_t, int nch, const LearnChan*, const float* rc, const float* mc,
+                       int H, int W, __bf16* a0, __bf16* dx, int slot0, int dcap);
+void launch_learn_prolong(hipStream_t, int ntiles, const ChanDesc*, const Tile*,
+                          const LearnChan*, const uint8_t* flags, const float* y,
+                          const float* mc, int H, int W, int f, double* cf);
 void launch_learn_dots(hipStream_t, int ntiles, const ChanDesc*, const Tile*,
                        const uint8_t* flags, const double* res, const double* cf, double* part);
-void launch_learn_alpha(hipStream_t, int nch, const ChanDesc*, LearnChan*, const double* part);
+void launch_learn_alpha(hipStream_t, int nch, const ChanDesc*, LearnChan*, const double* part,
+                        int step, int f);
 void launch_learn_apply(hipStream_t, int ntiles, const ChanDesc*, const Tile*, const LearnChan*,
                         const uint8_t* flags, const double* cf, double* p);
 void launch_learn_arm(hipStream_t, int nch, const ChanDesc*, ChanState*, const LearnChan*,
@@ -76,6 +82,7 @@ void launch_learn_arm(hipStream_t, int nch, const ChanDesc*, ChanState*, const L
 void launch_learn_collect(hipStream_t, int nch, ChanState*, LearnChan*, int final_phase);
 void launch_learn_target(hipStream_t, int ntiles, int nch, const ChanDesc*, const Tile*,
                          LearnChan*, const uint8_t* flags, const double* p, const double* p0,
-                         double* part, int H, int W, float* dt, int slot0, int dcap);
+                         double* part, int H, int W, int f, float* dt, int slot0, int dcap,
+                         int which);
 
 }  // namespace stm

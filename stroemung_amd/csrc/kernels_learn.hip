@@ -1,16 +1,34 @@
 // gfx950 kernels joining the FP64 pressure solve and the BF16 pressure network.
 //
-// A learned cycle on channel c:
-//   r  = Lap(p) - (rhs - mean)              interior fluid cells, ghosts refreshed first
-//   x  = -r / rms(r)                        network input, channel 0 (channel 1: fluid mask)
-//   y  = net(x)                             fp32, one value per cell
-//   q  = Lap(y)                             y with the same Neumann ghosts as p
-//   a  = -<r, q> / <q, q>                   minimises |r + a q|, so the step never raises
-//   p += a y                                the residual norm
-// after which budgeted red-black sweeps smooth what the network left.  The
-// same kernels build training pairs from plain solves: the residual at the
-// start of a solve (and after a few sweeps) with the normalised pressure
-// change that the solve went on to make.
+// With A = -Lap (symmetric positive definite on mean-free fields, Neumann
+// ghosts) the pressure equation is A p = b, b = -(rhs - mean), and the error
+// e = p* - p solves A e = r with r = b - A p = Lap(p) - (rhs - mean).
+//
+// The network works on a grid coarsened by f (1, 2, 4 or 8) and acts as the
+// coarse solver of a two-grid cycle whose smoother is red-black SOR:
+//   r   = Lap(p) - (rhs - mean)             fine interior fluid cells, ghosts current
+//   rc  = R r                               coarse pixel = sum of r over its f x f
+//                                           fine cells / f^2 (solid cells add 0)
+//   x   = -rc / rms(rc)                     network input channel 0; channel 1 is the
+//                                           fluid fraction of the pixel
+//   y   = net(x)                            trained to approximate e_c rms(rc) / h_c^2,
+//                                           h_c^2 = f^2 dx dy: the grid-unit inverse
+//                                           Laplacian of x on the coarse grid
+//   z   = P y                               bilinear between coarse pixel centres,
+//                                           weighted by fluid fraction
+//   p  += a z                               a = h_c^2 / rms(rc) (the network's own
+//                                           magnitude), or the a minimising the energy
+//                                           norm of the new error,
+//                                           a = <r, z> / <z, A z> = -<r, z> / <z, Lap z>
+// after which budgeted red-black sweeps remove what the coarse grid cannot
+// carry.  SOR with omega near 2 damps every mode shorter than a few dozen
+// cells at a rate near omega - 1 per sweep and only the long modes slowly;
+// those are what a coarse grid resolves and the network's receptive field
+// covers.  f = 1 is the network on the fine grid itself.
+//
+// The same kernels build training pairs from plain solves: the coarse
+// residual at the start of a solve (and after a few sweeps) with R of the
+// mean-free pressure change the solve went on to make, in the units of y.
 //
 // Lap is the five-point operator of the red-black solver with the same
 // operation order, so a residual computed here equals the solver's own.
@@ -33,7 +51,7 @@ __device__ __forceinline__ double lap(const double* __restrict__ a, int64_t i, i
     return fma((a[i + P] - c) - (c - a[i - P]), rdx2, ((a[i + 1] - c) - (c - a[i - 1])) * rdy2);
 }
 
-// Sum of one block's values into part[blockIdx.x * NPART + slot].
+// Sum of one tile block's values into part[blockIdx.x * NPART + slot].
 __device__ void tile_sum(double v, double* sh, double* __restrict__ part, int slot) {
     const int tid = threadIdx.y * blockDim.x + threadIdx.x;
     sh[tid] = v;
@@ -46,16 +64,12 @@ __device__ void tile_sum(double v, double* sh, double* __restrict__ part, int sl
     __syncthreads();
 }
 
-// One channel's partials of `slot` summed in a fixed order (256 threads).
-__device__ double chan_total(const ChanDesc& d, const double* __restrict__ part, int slot,
-                             double* sh) {
+// Block sum (256 threads, 1-D) returned to every thread.
+__device__ double block_total(double v, double* sh) {
     const int tid = threadIdx.x;
-    double acc = 0.0;
-    for (int k = d.tile0 + tid; k < d.tile0 + d.ntiles; k += blockDim.x)
-        acc = acc + part[(int64_t)k * NPART + slot];
-    sh[tid] = acc;
+    sh[tid] = v;
     __syncthreads();
-    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+    for (int st = 128; st > 0; st >>= 1) {
         if (tid < st) sh[tid] = sh[tid] + sh[tid + st];
         __syncthreads();
     }
@@ -64,7 +78,41 @@ __device__ double chan_total(const ChanDesc& d, const double* __restrict__ part,
     return r;
 }
 
-constexpr int S_RES2 = 2, S_RQ = 3, S_QQ = 4, S_DSUM = 5, S_DSQ = 6;
+// One channel's partials of `slot` summed in a fixed order (256 threads).
+__device__ double chan_total(const ChanDesc& d, const double* __restrict__ part, int slot,
+                             double* sh) {
+    double acc = 0.0;
+    for (int k = d.tile0 + threadIdx.x; k < d.tile0 + d.ntiles; k += blockDim.x)
+        acc = acc + part[(int64_t)k * NPART + slot];
+    return block_total(acc, sh);
+}
+
+constexpr int S_RES2 = 2, S_RZ = 3, S_ZQ = 4, S_DSUM = 5;
+
+// Sum over the interior fluid cells of coarse pixel (hc, wc) of a[g], or of
+// (a[g] - a0[g]) - shift when a0 is given; *nf = the number of those cells.
+__device__ __forceinline__ double block_fine_sum(const ChanDesc& d, const uint8_t* __restrict__ flags,
+                                                 const double* __restrict__ a,
+                                                 const double* __restrict__ a0, double shift,
+                                                 int hc, int wc, int f, int* nf) {
+    double sum = 0.0;
+    int n = 0;
+    for (int i = 0; i < f; i++) {
+        const int x = hc * f + i + 1;
+        if (x > d.nx - 2) break;
+        for (int j = 0; j < f; j++) {
+            const int y = wc * f + j + 1;
+            if (y > d.ny - 2) break;
+            const int64_t g = d.off + x * (int64_t)d.pitch + y;
+            if (flag_fluid(flags[g])) {
+                sum += a0 ? (a[g] - a0[g]) - shift : a[g];
+                n++;
+            }
+        }
+    }
+    *nf = n;
+    return sum;
+}
 
 }  // namespace
 
@@ -93,10 +141,9 @@ k_learn_residual(const ChanDesc* __restrict__ cd, const ChanState* __restrict__ 
     tile_sum(r2, sh, part, S_RES2);
 }
 
-// Per channel: has the solve converged, and the input scale.  first = 1 at
-// the start of a solve.
+// Per channel: has the solve converged?  first = 1 at the start of a solve.
 __global__ void __launch_bounds__(256)
-k_learn_scale(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
+k_learn_check(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
               LearnChan* __restrict__ lc, const double* __restrict__ part, int first)
 {
     __shared__ double sh[256];
@@ -116,34 +163,74 @@ k_learn_scale(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
         s.norm2 = res2 / d.fluid_cells;
         s.exhausted = 0;
     }
-    l.scale = (l.run && res2 > 0.0) ? sqrt(d.n_inner / res2) : 0.0;
     l.alpha = 0.0;
 }
 
-// Network input of channel b (batch entry b): channel 0 = -r * scale,
-// channel 1 = interior fluid mask, channels 2..31 zero.  Interior cell
-// (x, y) is pixel (x - 1, y - 1); pixels beyond the interior are zero.
-// Optionally also a compact 2-channel dataset sample in slot (slot0 + b) % dcap.
+// Coarse residual rc and fluid fraction mc of every pixel of an H x W frame
+// (grid: pixel blocks x channels); per-block (sum rc^2, fluid pixels) into
+// cpart[c * gridDim.x + blockIdx.x].
 __global__ void __launch_bounds__(256)
-k_learn_pack(const ChanDesc* __restrict__ cd, const LearnChan* __restrict__ lc,
-             const uint8_t* __restrict__ flags, const double* __restrict__ res, int nch, int H,
-             int W, __bf16* __restrict__ a0, __bf16* __restrict__ dx, int slot0, int dcap)
+k_learn_restrict(const ChanDesc* __restrict__ cd, const LearnChan* __restrict__ lc,
+                 const uint8_t* __restrict__ flags, const double* __restrict__ res, int H, int W,
+                 int f, float* __restrict__ rc, float* __restrict__ mc,
+                 double2* __restrict__ cpart)
+{
+    __shared__ double sh[256];
+    const int c = blockIdx.y;
+    const int64_t hw = (int64_t)H * W;
+    const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const ChanDesc& d = cd[c];
+    double v = 0.0, fr = 0.0;
+    if (q < hw) {
+        if (lc[c].run) {
+            const int hc = (int)(q / W), wc = (int)(q - (int64_t)hc * W);
+            int nf = 0;
+            const double sum = block_fine_sum(d, flags, res, nullptr, 0.0, hc, wc, f, &nf);
+            v = sum / (double)(f * f);
+            fr = (double)nf / (double)(f * f);
+        }
+        rc[c * hw + q] = (float)v;
+        mc[c * hw + q] = (float)fr;
+    }
+    const double s2 = block_total(v * v, sh);
+    const double n = block_total(fr > 0.0 ? 1.0 : 0.0, sh);
+    if (threadIdx.x == 0) cpart[(int64_t)c * gridDim.x + blockIdx.x] = make_double2(s2, n);
+}
+
+// Network input scale 1 / rms(rc) over the channel's fluid pixels; save >= 0
+// keeps it for captured sample `save`.
+__global__ void __launch_bounds__(256)
+k_learn_nscale(LearnChan* __restrict__ lc, const double2* __restrict__ cpart, int nblk, int save)
+{
+    __shared__ double sh[256];
+    const int c = blockIdx.x;
+    double s2 = 0.0, n = 0.0;
+    for (int k = threadIdx.x; k < nblk; k += 256) {
+        const double2 v = cpart[(int64_t)c * nblk + k];
+        s2 = s2 + v.x;
+        n = n + v.y;
+    }
+    s2 = block_total(s2, sh);
+    n = block_total(n, sh);
+    if (threadIdx.x) return;
+    LearnChan& l = lc[c];
+    l.scale = (l.run && s2 > 0.0) ? sqrt(n / s2) : 0.0;
+    if (save >= 0) l.cscale[save] = l.scale;
+}
+
+// Network input (nch x H x W x 32 bf16: -rc * scale, fluid fraction, zeros)
+// and/or a compact 2-channel dataset sample in slot (slot0 + b) % dcap.
+__global__ void __launch_bounds__(256)
+k_learn_pack(const LearnChan* __restrict__ lc, const float* __restrict__ rc,
+             const float* __restrict__ mc, int nch, int H, int W, __bf16* __restrict__ a0,
+             __bf16* __restrict__ dx, int slot0, int dcap)
 {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t hw = (int64_t)H * W;
     if (k >= hw * nch) return;
     const int b = (int)(k / hw);
     const int64_t q = k - (int64_t)b * hw;
-    const int h = (int)(q / W), w = (int)(q - (int64_t)h * W);
-    const ChanDesc& d = cd[b];
-    float v0 = 0.f, v1 = 0.f;
-    if (h <= d.nx - 3 && w <= d.ny - 3) {
-        const int64_t i = d.off + (h + 1) * (int64_t)d.pitch + (w + 1);
-        if (flag_fluid(flags[i])) {
-            v0 = (float)(-res[i] * lc[b].scale);
-            v1 = 1.f;
-        }
-    }
+    const float v0 = (float)(-(double)rc[k] * lc[b].scale), v1 = mc[k];
     if (a0) {
         typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
         bf16x8 z;
@@ -164,12 +251,15 @@ k_learn_pack(const ChanDesc* __restrict__ cd, const LearnChan* __restrict__ lc,
     }
 }
 
-// Correction field: y at interior fluid cells of running channels, 0 at the
-// other cells (boundary ghosts are set afterwards).
+// Correction z = P y at interior fluid cells of running channels, 0 at the
+// other cells (boundary ghosts are set afterwards).  Fine interior index u
+// lies at coarse coordinate (u + 1/2) / f - 1/2 (pixel centres at integers);
+// the four surrounding pixels are weighted bilinearly and by fluid fraction.
 __global__ void __launch_bounds__(TILE_THREADS)
-k_learn_unpack(const ChanDesc* __restrict__ cd, const Tile* __restrict__ tiles,
-               const LearnChan* __restrict__ lc, const uint8_t* __restrict__ flags,
-               const float* __restrict__ y, int H, int W, double* __restrict__ cf)
+k_learn_prolong(const ChanDesc* __restrict__ cd, const Tile* __restrict__ tiles,
+                const LearnChan* __restrict__ lc, const uint8_t* __restrict__ flags,
+                const float* __restrict__ y, const float* __restrict__ mc, int H, int W, int f,
+                double* __restrict__ cf)
 {
     const Tile t = tiles[blockIdx.x];
     const ChanDesc& d = cd[t.chan];
@@ -177,12 +267,26 @@ k_learn_unpack(const ChanDesc* __restrict__ cd, const Tile* __restrict__ tiles,
     if (x >= d.nx || yy >= d.ny) return;
     const int64_t i = d.off + x * (int64_t)d.pitch + yy;
     double v = 0.0;
-    if (lc[t.chan].run && interior(d, x, yy) && flag_fluid(flags[i]))
-        v = (double)y[((int64_t)t.chan * H + (x - 1)) * W + (yy - 1)];
+    if (lc[t.chan].run && interior(d, x, yy) && flag_fluid(flags[i])) {
+        const int hc_n = (d.nx - 2 + f - 1) / f, wc_n = (d.ny - 2 + f - 1) / f;
+        const double xi = fmin(fmax(((x - 1) + 0.5) / f - 0.5, 0.0), (double)(hc_n - 1));
+        const double eta = fmin(fmax(((yy - 1) + 0.5) / f - 0.5, 0.0), (double)(wc_n - 1));
+        const int i0 = (int)xi, j0 = (int)eta;
+        const int i1 = i0 + 1 < hc_n ? i0 + 1 : i0, j1 = j0 + 1 < wc_n ? j0 + 1 : j0;
+        const double tx = xi - i0, ty = eta - j0;
+        const int64_t base = (int64_t)t.chan * H * W;
+        const int64_t k00 = base + (int64_t)i0 * W + j0, k01 = base + (int64_t)i0 * W + j1;
+        const int64_t k10 = base + (int64_t)i1 * W + j0, k11 = base + (int64_t)i1 * W + j1;
+        const double w00 = (1.0 - tx) * (1.0 - ty) * mc[k00], w01 = (1.0 - tx) * ty * mc[k01];
+        const double w10 = tx * (1.0 - ty) * mc[k10], w11 = tx * ty * mc[k11];
+        const double ws = (w00 + w01) + (w10 + w11);
+        if (ws > 0.0)
+            v = ((w00 * y[k00] + w01 * y[k01]) + (w10 * y[k10] + w11 * y[k11])) / ws;
+    }
     cf[i] = v;
 }
 
-// q = Lap(cf) at interior fluid cells: slot 3 = sum r q, slot 4 = sum q q.
+// q = Lap(z) at interior fluid cells: slot 3 = sum r z, slot 4 = sum z q.
 __global__ void __launch_bounds__(TILE_THREADS)
 k_learn_dots(const ChanDesc* __restrict__ cd, const Tile* __restrict__ tiles,
              const uint8_t* __restrict__ flags, const double* __restrict__ res,
@@ -192,34 +296,40 @@ k_learn_dots(const ChanDesc* __restrict__ cd, const Tile* __restrict__ tiles,
     const Tile t = tiles[blockIdx.x];
     const ChanDesc& d = cd[t.chan];
     const int x = t.x0 + threadIdx.y, y = t.y0 + threadIdx.x;
-    double rq = 0.0, qq = 0.0;
+    double rz = 0.0, zq = 0.0;
     if (x < d.nx && y < d.ny && interior(d, x, y)) {
         const int64_t i = d.off + x * (int64_t)d.pitch + y;
         if (flag_fluid(flags[i])) {
             const double q = lap(cf, i, d.pitch, d.rdx2, d.rdy2);
-            rq = res[i] * q;
-            qq = q * q;
+            rz = res[i] * cf[i];
+            zq = cf[i] * q;
         }
     }
-    tile_sum(rq, sh, part, S_RQ);
-    tile_sum(qq, sh, part, S_QQ);
+    tile_sum(rz, sh, part, S_RZ);
+    tile_sum(zq, sh, part, S_ZQ);
 }
 
+// step 0: energy-norm line search; step 1: the network's own magnitude.
 __global__ void __launch_bounds__(256)
 k_learn_alpha(const ChanDesc* __restrict__ cd, LearnChan* __restrict__ lc,
-              const double* __restrict__ part)
+              const double* __restrict__ part, int step, int f)
 {
     __shared__ double sh[256];
     const int c = blockIdx.x;
     const ChanDesc& d = cd[c];
-    const double rq = chan_total(d, part, S_RQ, sh);
-    const double qq = chan_total(d, part, S_QQ, sh);
+    const double rz = chan_total(d, part, S_RZ, sh);
+    const double zq = chan_total(d, part, S_ZQ, sh);
     if (threadIdx.x) return;
     LearnChan& l = lc[c];
-    l.alpha = (l.run && qq > 0.0) ? -rq / qq : 0.0;
+    double a = 0.0;
+    if (l.run && l.scale > 0.0) {
+        if (step == 1) a = (double)(f * f) * d.delx * d.dely / l.scale;
+        else if (zq < 0.0) a = -rz / zq;    // <z, Lap z> < 0 for any z not constant
+    }
+    l.alpha = a;
 }
 
-// p += alpha * cf at interior fluid cells.
+// p += alpha * z at interior fluid cells.
 __global__ void __launch_bounds__(TILE_THREADS)
 k_learn_apply(const ChanDesc* __restrict__ cd, const Tile* __restrict__ tiles,
               const LearnChan* __restrict__ lc, const uint8_t* __restrict__ flags,
@@ -281,13 +391,11 @@ __global__ void k_learn_collect(ChanState* __restrict__ cs, LearnChan* __restric
     }
 }
 
-// Capture: pass 0 sums d = p - p0 (slot 5), pass 1 sums (d - mean)^2 (slot 6),
-// over interior fluid cells.
+// Capture: slot 5 = sum of p - p0 over interior fluid cells.
 __global__ void __launch_bounds__(TILE_THREADS)
-k_learn_delta_sums(const ChanDesc* __restrict__ cd, const Tile* __restrict__ tiles,
-                   const LearnChan* __restrict__ lc, const uint8_t* __restrict__ flags,
-                   const double* __restrict__ p, const double* __restrict__ p0,
-                   double* __restrict__ part, int pass)
+k_learn_delta_sum(const ChanDesc* __restrict__ cd, const Tile* __restrict__ tiles,
+                  const uint8_t* __restrict__ flags, const double* __restrict__ p,
+                  const double* __restrict__ p0, double* __restrict__ part)
 {
     __shared__ double sh[TILE_THREADS];
     const Tile t = tiles[blockIdx.x];
@@ -296,49 +404,42 @@ k_learn_delta_sums(const ChanDesc* __restrict__ cd, const Tile* __restrict__ til
     double v = 0.0;
     if (x < d.nx && y < d.ny && interior(d, x, y)) {
         const int64_t i = d.off + x * (int64_t)d.pitch + y;
-        if (flag_fluid(flags[i])) {
-            const double dl = p[i] - p0[i];
-            v = pass ? (dl - lc[t.chan].dmean) * (dl - lc[t.chan].dmean) : dl;
-        }
+        if (flag_fluid(flags[i])) v = p[i] - p0[i];
     }
-    tile_sum(v, sh, part, pass ? S_DSQ : S_DSUM);
+    tile_sum(v, sh, part, S_DSUM);
 }
 
 __global__ void __launch_bounds__(256)
-k_learn_delta_stats(const ChanDesc* __restrict__ cd, LearnChan* __restrict__ lc,
-                    const double* __restrict__ part, int pass)
+k_learn_delta_mean(const ChanDesc* __restrict__ cd, LearnChan* __restrict__ lc,
+                   const double* __restrict__ part)
 {
     __shared__ double sh[256];
     const int c = blockIdx.x;
     const ChanDesc& d = cd[c];
-    const double sum = chan_total(d, part, pass ? S_DSQ : S_DSUM, sh);
-    if (threadIdx.x) return;
-    LearnChan& l = lc[c];
-    if (!pass) l.dmean = d.n_inner > 0.0 ? sum / d.n_inner : 0.0;
-    else l.dscale = sum > 0.0 ? sqrt(d.n_inner / sum) : 0.0;
+    const double sum = chan_total(d, part, S_DSUM, sh);
+    if (threadIdx.x == 0) lc[c].dmean = d.n_inner > 0.0 ? sum / d.n_inner : 0.0;
 }
 
-// Capture: the normalised pressure change as the training target of the
-// sample in slot (slot0 + b) % dcap; zero outside interior fluid cells.
+// Capture: the training target of the sample in slot (slot0 + b) % dcap --
+// R of the mean-free pressure change times scale / h_c^2 with the scale of
+// the stored residual, i.e. in the units of the network's output.
 __global__ void __launch_bounds__(256)
 k_learn_target(const ChanDesc* __restrict__ cd, const LearnChan* __restrict__ lc,
                const uint8_t* __restrict__ flags, const double* __restrict__ p,
-               const double* __restrict__ p0, int nch, int H, int W, float* __restrict__ dt,
-               int slot0, int dcap)
+               const double* __restrict__ p0, int nch, int H, int W, int f,
+               float* __restrict__ dt, int slot0, int dcap, int which)
 {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t hw = (int64_t)H * W;
     if (k >= hw * nch) return;
     const int b = (int)(k / hw);
     const int64_t q = k - (int64_t)b * hw;
-    const int h = (int)(q / W), w = (int)(q - (int64_t)h * W);
+    const int hc = (int)(q / W), wc = (int)(q - (int64_t)hc * W);
     const ChanDesc& d = cd[b];
-    float v = 0.f;
-    if (h <= d.nx - 3 && w <= d.ny - 3) {
-        const int64_t i = d.off + (h + 1) * (int64_t)d.pitch + (w + 1);
-        if (flag_fluid(flags[i])) v = (float)((p[i] - p0[i] - lc[b].dmean) * lc[b].dscale);
-    }
-    dt[((slot0 + b) % dcap) * hw + q] = v;
+    int nf = 0;
+    const double sum = block_fine_sum(d, flags, p, p0, lc[b].dmean, hc, wc, f, &nf);
+    const double h2 = (double)(f * f) * d.delx * d.dely;
+    dt[((slot0 + b) % dcap) * hw + q] = (float)(sum / (double)(f * f) * (lc[b].cscale[which] / h2));
 }
 
 // ------------------------------------------------------------ launchers
@@ -346,33 +447,42 @@ k_learn_target(const ChanDesc* __restrict__ cd, const LearnChan* __restrict__ lc
 static inline unsigned nb256(int64_t n) { return (unsigned)((n + 255) / 256); }
 static const dim3 TB(TILE_Y, TILE_X);
 
+int learn_coarse_blocks(int H, int W) { return (int)nb256((int64_t)H * W); }
+
 void launch_learn_residual(hipStream_t st, int ntiles, const ChanDesc* cd, const ChanState* cs,
                            const Tile* tiles, const uint8_t* flags, const double* rhs,
                            const double* p, double* res, double* part) {
     k_learn_residual<<<ntiles, TB, 0, st>>>(cd, cs, tiles, flags, rhs, p, res, part);
 }
-void launch_learn_scale(hipStream_t st, int nch, const ChanDesc* cd, ChanState* cs, LearnChan* lc,
+void launch_learn_check(hipStream_t st, int nch, const ChanDesc* cd, ChanState* cs, LearnChan* lc,
                         const double* part, int first) {
-    k_learn_scale<<<nch, 256, 0, st>>>(cd, cs, lc, part, first);
+    k_learn_check<<<nch, 256, 0, st>>>(cd, cs, lc, part, first);
 }
-void launch_learn_pack(hipStream_t st, int nch, const ChanDesc* cd, const LearnChan* lc,
-                       const uint8_t* flags, const double* res, int H, int W, __bf16* a0,
-                       __bf16* dx, int slot0, int dcap) {
-    k_learn_pack<<<nb256((int64_t)H * W * nch), 256, 0, st>>>(cd, lc, flags, res, nch, H, W, a0,
-                                                              dx, slot0, dcap);
+void launch_learn_restrict(hipStream_t st, int nch, const ChanDesc* cd, LearnChan* lc,
+                           const uint8_t* flags, const double* res, int H, int W, int f,
+                           float* rc, float* mc, double2* cpart, int save) {
+    const int nblk = learn_coarse_blocks(H, W);
+    k_learn_restrict<<<dim3(nblk, nch), 256, 0, st>>>(cd, lc, flags, res, H, W, f, rc, mc, cpart);
+    k_learn_nscale<<<nch, 256, 0, st>>>(lc, cpart, nblk, save);
 }
-void launch_learn_unpack(hipStream_t st, int ntiles, const ChanDesc* cd, const Tile* tiles,
-                         const LearnChan* lc, const uint8_t* flags, const float* y, int H, int W,
-                         double* cf) {
-    k_learn_unpack<<<ntiles, TB, 0, st>>>(cd, tiles, lc, flags, y, H, W, cf);
+void launch_learn_pack(hipStream_t st, int nch, const LearnChan* lc, const float* rc,
+                       const float* mc, int H, int W, __bf16* a0, __bf16* dx, int slot0,
+                       int dcap) {
+    k_learn_pack<<<nb256((int64_t)H * W * nch), 256, 0, st>>>(lc, rc, mc, nch, H, W, a0, dx,
+                                                              slot0, dcap);
+}
+void launch_learn_prolong(hipStream_t st, int ntiles, const ChanDesc* cd, const Tile* tiles,
+                          const LearnChan* lc, const uint8_t* flags, const float* y,
+                          const float* mc, int H, int W, int f, double* cf) {
+    k_learn_prolong<<<ntiles, TB, 0, st>>>(cd, tiles, lc, flags, y, mc, H, W, f, cf);
 }
 void launch_learn_dots(hipStream_t st, int ntiles, const ChanDesc* cd, const Tile* tiles,
                        const uint8_t* flags, const double* res, const double* cf, double* part) {
     k_learn_dots<<<ntiles, TB, 0, st>>>(cd, tiles, flags, res, cf, part);
 }
 void launch_learn_alpha(hipStream_t st, int nch, const ChanDesc* cd, LearnChan* lc,
-                        const double* part) {
-    k_learn_alpha<<<nch, 256, 0, st>>>(cd, lc, part);
+                        const double* part, int step, int f) {
+    k_learn_alpha<<<nch, 256, 0, st>>>(cd, lc, part, step, f);
 }
 void launch_learn_apply(hipStream_t st, int ntiles, const ChanDesc* cd, const Tile* tiles,
                         const LearnChan* lc, const uint8_t* flags, const double* cf, double* p) {
@@ -387,14 +497,12 @@ void launch_learn_collect(hipStream_t st, int nch, ChanState* cs, LearnChan* lc,
 }
 void launch_learn_target(hipStream_t st, int ntiles, int nch, const ChanDesc* cd,
                          const Tile* tiles, LearnChan* lc, const uint8_t* flags, const double* p,
-                         const double* p0, double* part, int H, int W, float* dt, int slot0,
-                         int dcap) {
-    k_learn_delta_sums<<<ntiles, TB, 0, st>>>(cd, tiles, lc, flags, p, p0, part, 0);
-    k_learn_delta_stats<<<nch, 256, 0, st>>>(cd, lc, part, 0);
-    k_learn_delta_sums<<<ntiles, TB, 0, st>>>(cd, tiles, lc, flags, p, p0, part, 1);
-    k_learn_delta_stats<<<nch, 256, 0, st>>>(cd, lc, part, 1);
+                         const double* p0, double* part, int H, int W, int f, float* dt,
+                         int slot0, int dcap, int which) {
+    k_learn_delta_sum<<<ntiles, TB, 0, st>>>(cd, tiles, flags, p, p0, part);
+    k_learn_delta_mean<<<nch, 256, 0, st>>>(cd, lc, part);
     k_learn_target<<<nb256((int64_t)H * W * nch), 256, 0, st>>>(cd, lc, flags, p, p0, nch, H, W,
-                                                                dt, slot0, dcap);
+                                                                f, dt, slot0, dcap, which);
 }
 
 }  // namespace stm

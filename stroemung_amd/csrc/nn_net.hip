@@ -250,7 +250,7 @@ __global__ void k_pack_host(const float* __restrict__ x, const float* __restrict
     for (int c = 0; c < 32; c++) a0[p * 32 + c] = (bf16)(c < 2 ? x[p * 2 + c] : 0.f);
     if (t) {
         tt[p] = t[p];
-        m[p] = x[p * 2 + 1] > 0.5f ? 1.f : 0.f;
+        m[p] = x[p * 2 + 1] > 0.f ? 1.f : 0.f;
     }
 }
 
@@ -269,7 +269,7 @@ __global__ void k_pack_batch(const bf16* __restrict__ dx, const float* __restric
     o[1] = x1;
     for (int c = 2; c < 32; c++) o[c] = (bf16)0.f;
     tt[k] = dt[src];
-    m[k] = (float)x1 > 0.5f ? 1.f : 0.f;
+    m[k] = (float)x1 > 0.f ? 1.f : 0.f;
 }
 
 int fail_net(stm_net* n, int code, const char* msg) {
@@ -407,6 +407,22 @@ int stm_net_dataset_reserve(stm_net* n, uint32_t capacity) {
 int stm_net_dataset_size(stm_net* n, uint32_t* size) {
     if (!n || !size) return fail_net(n, STM_ERR_INVALID_ARG, "null argument");
     *size = (uint32_t)n->net.dsize;
+    return STM_OK;
+}
+
+int stm_net_dataset_get(stm_net* n, uint32_t index, float* x, float* t) {
+    if (!n || !x || !t) return fail_net(n, STM_ERR_INVALID_ARG, "null argument");
+    Net& net = n->net;
+    std::string& err = net.err;
+    if ((int)index >= net.dsize) return fail_net(n, STM_ERR_INVALID_ARG, "sample out of range");
+    const size_t hw = (size_t)net.H * net.W;
+    std::vector<bf16> xb(hw * 2);
+    NCHK(hipMemcpyAsync(xb.data(), net.dx + hw * 2 * index, sizeof(bf16) * hw * 2,
+                        hipMemcpyDeviceToHost, net.stream));
+    NCHK(hipMemcpyAsync(t, net.dt + hw * index, sizeof(float) * hw, hipMemcpyDeviceToHost,
+                        net.stream));
+    NCHK(hipStreamSynchronize(net.stream));
+    for (size_t k = 0; k < hw * 2; k++) x[k] = (float)xb[k];
     return STM_OK;
 }
 

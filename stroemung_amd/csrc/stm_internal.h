@@ -65,13 +65,16 @@ struct ChanState {
 
 // Per-channel state of the learned correction (device copy in LearnChan[]).
 struct LearnChan {
-    double scale;                // 1 / rms of the residual given to the network
+    double scale;                // network input scale: 1 / rms of the coarse residual
     double alpha;                // step length of the network's correction
-    double dmean, dscale;        // capture: mean and 1 / rms of the pressure change
+    double dmean;                // capture: mean of the pressure change
+    double cscale[8];            // capture: `scale` of each stored residual (LEARN_POINTS)
     int32_t run;                 // the channel has not converged yet
     int32_t sweeps;              // sweeps summed over the phases of one solve
-    int32_t pad0, pad1;
 };
+
+// Training pairs one solve can store per channel.
+constexpr int LEARN_POINTS = 8;
 
 // One boundary cell (the reference's sorted_boundary_list entry,
 // src/grid/mod.rs:64) with what the parallel SETBCOND needs.

@@ -96,6 +96,15 @@ class PressureNet:
         self._check(lib().stm_net_dataset_size(self._net, C.byref(n)))
         return int(n.value)
 
+    def sample(self, index):
+        """(x (H, W, 2), t (H, W)) of training sample `index`."""
+        x = np.empty((self.height, self.width, 2), np.float32)
+        t = np.empty((self.height, self.width), np.float32)
+        fp = C.POINTER(C.c_float)
+        self._check(lib().stm_net_dataset_get(self._net, index, x.ctypes.data_as(fp),
+                                              t.ctypes.data_as(fp)))
+        return x, t
+
     def train(self, steps, batch, lr=1e-3, seed=0):
         loss = C.c_float()
         self._check(lib().stm_net_train(self._net, steps, batch, lr, seed, C.byref(loss)))

@@ -182,13 +182,19 @@ class Context:
     def reset_stats(self):
         self._check(lib().stm_reset_stats(self._ctx))
 
-    def set_learned(self, net, cycles=1, sweeps=32):
-        """Learned cycles before every red-black solve (net None: off)."""
-        self._check(lib().stm_set_learned(self._ctx, net._net if net else None, cycles, sweeps))
+    def set_learned(self, net, cycles=1, sweeps=32, step="network", coarsen=1):
+        """Two-grid cycles with the network before every red-black solve (net
+        None: off).  step: "network" takes the network's magnitude, "energy"
+        a line search; coarsen: 1, 2, 4 or 8."""
+        rule = {"energy": 0, "network": 1}[step]
+        self._check(lib().stm_set_learned(self._ctx, net._net if net else None, cycles, sweeps,
+                                          rule, coarsen))
 
-    def set_capture(self, net, sweeps=32):
-        """Store training pairs from every solve in net's training set (None: off)."""
-        self._check(lib().stm_set_capture(self._ctx, net._net if net else None, sweeps))
+    def set_capture(self, net, points=2, sweeps=32, coarsen=1):
+        """Store `points` training pairs per channel from every solve in net's
+        training set (None: off)."""
+        self._check(lib().stm_set_capture(self._ctx, net._net if net else None, points, sweeps,
+                                          coarsen))
 
 
 class Simulation:

@@ -243,20 +243,80 @@ def _cavity_context(count, n, seed, eps=1e-6):
     from stroemung_amd import workloads
     from stroemung_amd.simulation import Context
     ctx = Context(0)
-    for ct, vel, prm in workloads.random_cavities(count, n=n, seed=seed, eps=eps):
+    grids = workloads.random_cavities(count, n=n, seed=seed, eps=eps)
+    for ct, vel, prm in grids:
         ctx.add_grid(prm, ct, vel)
     ctx.construct()
-    return ctx
+    return ctx, grids
+
+
+def _ghosts(ct, p):
+    """p with the boundary ghosts refreshed by the oracle's rule."""
+    from oracle import pyoracle as O
+    sim = O.Sim(ct, delx=1.0, dely=1.0, delt=0.0, gamma=0.0, reynolds=1.0,
+                sor_absolute_epsilon=1.0, max_iterations=1, omega=1.0, p=p)
+    assert sim.call("rebuild_boundary_list") == 0
+    sim.call("copy_pressure_to_boundaries")
+    return sim.p.copy()
+
+
+def _restrict(a, f):
+    n0, n1 = a.shape
+    return a.reshape(n0 // f, f, n1 // f, f).sum(axis=(1, 3)) / (f * f)
 
 
 @pytest.mark.gpu
-def test_capture_train_and_learned_solve_reach_the_same_residual():
+@pytest.mark.parametrize("f", [1, 4])
+def test_capture_pairs_follow_the_restriction(f):
+    """The stored pair of the first capture point, recomputed from the fields:
+    residual from ghost-refreshed p, restriction, scale, fluid fraction and
+    the target from the converged p."""
+    from stroemung_amd.learned import PressureNet
+    n, count = 64, 2
+    ctx, grids = _cavity_context(count, n, seed=21)
+    for _ in range(3):
+        ctx.tick()
+    p0 = [ctx.download(c, ("p",))["p"] for c in range(count)]
+    net = PressureNet(ctx, n // f, n // f, 4, seed=1)
+    net.reserve(8)
+    ctx.set_capture(net, points=2, sweeps=8, coarsen=f)
+    ctx.tick()
+    ctx.set_capture(None)
+    for c, (ct, _, prm) in enumerate(grids):
+        st = ctx.state(c)
+        fld = ctx.download(c, ("p", "rhs"))
+        pg = _ghosts(ct, p0[c])
+        fl = (ct == 0)[1:-1, 1:-1]
+        P = pg
+        pc = P[1:-1, 1:-1]
+        rdx2, rdy2 = 1.0 / prm.delx ** 2, 1.0 / prm.dely ** 2
+        lap = (((P[2:, 1:-1] - pc) - (pc - P[:-2, 1:-1])) * rdx2
+               + ((P[1:-1, 2:] - pc) - (pc - P[1:-1, :-2])) * rdy2)
+        r = np.where(fl, lap - (fld["rhs"][1:-1, 1:-1] - st["rhs_mean"]), 0.0)
+        rc = _restrict(r, f)
+        frac = _restrict(fl.astype(float), f)
+        scale = np.sqrt(np.count_nonzero(frac) / np.sum(rc * rc))
+        x, t = net.sample(c)
+        np.testing.assert_array_equal(x[..., 1], R.to_bf16(frac).numpy())
+        want_x = -rc * scale
+        assert np.max(np.abs(x[..., 0] - want_x)) <= 1e-2 * np.max(np.abs(want_x)) + 1e-6
+        d = np.where(fl, fld["p"][1:-1, 1:-1] - p0[c][1:-1, 1:-1], 0.0)
+        d = np.where(fl, d - d[fl].mean(), 0.0)
+        want_t = _restrict(d, f) * scale / (f * f * prm.delx * prm.dely)
+        np.testing.assert_allclose(t, want_t, rtol=1e-4, atol=1e-5 * np.max(np.abs(want_t)))
+    net.close()
+    ctx.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("f,step", [(1, "network"), (4, "network"), (4, "energy")])
+def test_capture_train_and_learned_solve_reach_the_same_residual(f, step):
     from stroemung_amd.learned import PressureNet
     n, count = 64, 4
-    ctx = _cavity_context(count, n, seed=11)
-    net = PressureNet(ctx, n, n, 32, seed=12)
+    ctx, _ = _cavity_context(count, n, seed=11)
+    net = PressureNet(ctx, n // f, n // f, 32, seed=12)
     net.reserve(64)
-    ctx.set_capture(net, sweeps=16)
+    ctx.set_capture(net, points=2, sweeps=16, coarsen=f)
     for _ in range(6):
         its, _ = ctx.tick()
         for c in range(count):
@@ -269,7 +329,7 @@ def test_capture_train_and_learned_solve_reach_the_same_residual():
     print("training loss %.4f -> %.4f" % (first, last))
     assert last < first
     # learned solve: same stopping test, so the same residual bound
-    ctx.set_learned(net, cycles=2, sweeps=16)
+    ctx.set_learned(net, cycles=2, sweeps=16, step=step, coarsen=f)
     for _ in range(3):
         its, _ = ctx.tick()
         for c in range(count):
@@ -278,6 +338,15 @@ def test_capture_train_and_learned_solve_reach_the_same_residual():
             assert s["sor_iterations"] == its[c]
     st = net.stats()
     assert st["forward_calls"] >= 3 and st["forward_ms"] > 0
+    # capture inside learned solves (pairs from the learned solver's own states)
+    before = net.dataset_size()
+    ctx.set_capture(net, points=2, sweeps=16, coarsen=f)
+    its, _ = ctx.tick()
+    for c in range(count):
+        s = ctx.state(c)
+        assert s["residual_squared"] <= s["tolerance_squared"] and not s["exhausted"]
+    assert net.dataset_size() == min(64, before + 2 * count)
+    ctx.set_capture(None)
     ctx.set_learned(None)
     net.close()
     ctx.close()

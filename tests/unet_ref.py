@@ -96,8 +96,8 @@ def forward(P, x, bf16_acts=False):
 
 
 def loss(y, x, t):
-    """mean over samples of sum m (y - t)^2 / sum m t^2, m = x[..., 1] > 0.5."""
-    m = (x[..., 1] > 0.5).to(y.dtype)
+    """mean over samples of sum m (y - t)^2 / sum m t^2, m = x[..., 1] > 0."""
+    m = (x[..., 1] > 0).to(y.dtype)
     num = (m * (y - t) ** 2).sum(dim=(1, 2))
     den = (m * t * t).sum(dim=(1, 2))
     return (num / den).mean()

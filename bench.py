@@ -8,13 +8,19 @@ RHS, pressure solve to the residual, velocity update, next step size).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
     python bench.py --profile        # fixed sweeps + one 256 MiB copy, for rocprofv3
+    python bench.py --workload channels           # 256 uneven channels, plain
+    python bench.py --workload learned_cavities   # 32 cavities 256x256: plain vs learned
+    python bench.py --workload learned_channels   # 64 channels 512x256: plain vs learned
 
 Prints one JSON line.  `value` = interior cells * K / wall seconds of the K
 timed ticks, inputs resident in HBM.  The roofline entry is for the dominant
 kernel (the blocked red-black SOR launch), timed with HIP events on the
 library's stream inside the timed region; the CPU baseline is the oracle's
 lexicographic SOR (the reference algorithm, one core) timed on a bounded
-sample of the same grid.
+sample of the same grid.  The default line also carries "learned": the
+plain solve against the learned two-grid solve on 32 random-obstacle lid
+cavities of 256 x 256 (the network trained first, on the GPU, from pairs the
+solver produces on other layouts), both to the same residual.
 """
 import argparse
 import json
@@ -50,10 +56,14 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile", action="store_true",
                     help="short fixed run for hardware counters, no JSON line")
-    ap.add_argument("--workload", choices=("cavity", "channels"), default="cavity",
+    ap.add_argument("--workload", choices=("cavity", "channels", "learned_cavities",
+                                           "learned_channels"), default="cavity",
                     help="cavity: the 1024x1024 lid cavity (default); channels: 256 "
-                         "independent channels of uneven size, seed 0")
+                         "independent channels of uneven size, seed 0; learned_*: plain vs "
+                         "learned solve")
     ap.add_argument("--channels", type=int, default=256)
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the learned comparison of the default run")
     return ap.parse_args()
 
 
@@ -193,6 +203,77 @@ def channels_run(a):
     print(json.dumps(out), flush=True)
 
 
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def learned_cavities(ticks=10, log=None):
+    """32 random-obstacle lid cavities of 256 x 256 cells, Re 50-1000, eps
+    1e-6: network trained on seed-0 layouts, compared on seed-100 layouts."""
+    from stroemung_amd import learned, workloads
+    log = log or (lambda *x: None)
+    train = workloads.random_cavities(32, n=256, seed=0, eps=1e-6)
+    params, info = learned.train_net(train, coarsen=4, capture_ticks=96, points=4,
+                                     capture_sweeps=32, steps=3000, batch=32, dagger_rounds=2,
+                                     dagger_ticks=32, dagger_steps=1500, cycles=16, sweeps=32,
+                                     log=log)
+    test = workloads.random_cavities(32, n=256, seed=100, eps=1e-6)
+    res = learned.compare(test, params, coarsen=4, cycles=16, sweeps=32, warm=20, ticks=ticks,
+                          log=log)
+    res["training"] = info
+    res["workload"] = ("32 lid cavities 256x256 with 1-4 random obstacles, Re 50-1000, "
+                       "red-black SOR omega=1.8 to relative residual 1e-6, tau=0.5, fp64; "
+                       "trained on seed 0, measured on seed 100 after 20 plain ticks")
+    return res
+
+
+def learned_channels(ticks, log=None):
+    """64 channels 512 x 256, 1-4 random obstacles, Re 100-1000, eps 1e-7:
+    network trained on seed-1001 layouts, compared on seed 1."""
+    from stroemung_amd import learned, workloads
+    log = log or (lambda *x: None)
+    train = workloads.fixed_channels(64, seed=1001, eps=1e-7)
+    params, info = learned.train_net(train, coarsen=4, capture_ticks=40, points=4,
+                                     capture_sweeps=32, steps=3000, batch=32, dagger_rounds=2,
+                                     dagger_ticks=24, dagger_steps=1500, cycles=16, sweeps=32,
+                                     log=log)
+    test = workloads.fixed_channels(64, seed=1, eps=1e-7)
+    res = learned.compare(test, params, coarsen=4, cycles=16, sweeps=32, warm=10, ticks=ticks,
+                          log=log)
+    res["training"] = info
+    res["workload"] = ("64 channels 512x256 with 1-4 random obstacles, Re 100-1000, inflow u=1, "
+                       "red-black SOR omega=1.7 to relative residual 1e-7, tau=0.5, fp64; "
+                       "trained on seed 1001, measured on seed 1 after 10 plain ticks")
+    return res
+
+
+def learned_line(a):
+    log = lambda *x: print(*x, file=sys.stderr, flush=True)  # noqa: E731
+    if a.workload == "learned_cavities":
+        res = learned_cavities(a.steps, log)
+    else:
+        res = learned_channels(a.steps, log)
+    lr = res["learned"]
+    out = {
+        "metric": METRIC,
+        "value": lr["cell_updates_per_s"],
+        "unit": "cell-updates/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": res["warm"],
+        "ms_per_step": lr["ms_per_tick"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 (solve), bf16 (network)", "data": "synthetic",
+        "config": {"workload": res.pop("workload"), "plain": res["plain"],
+                   "speedup_vs_plain": res["speedup"], "sweeps_saved": res["sweeps_saved"],
+                   "training": res["training"], "learned": lr},
+        "roofline": {
+            "kernel": "U-Net forward (k_conv3x3 and companions)", "bound": "mfma",
+            "achieved": lr["forward_tflops"], "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": (lr["forward_tflops"] / MFMA_BF16_PEAK_TFLOPS) if lr["forward_tflops"] else None,
+            "traffic": None,
+        },
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -204,6 +285,9 @@ def main():
         return
     if a.workload == "channels":
         channels_run(a)
+        return
+    if a.workload.startswith("learned"):
+        learned_line(a)
         return
     n = a.n
     ctx, ch = make_context(a)
@@ -293,6 +377,12 @@ def main():
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n, sweeps_per_step, a.cpu_sweeps)
     ctx.close()
+    if not a.no_learned:
+        try:
+            out["learned"] = learned_cavities(
+                10, lambda *x: print("learned:", *x, file=sys.stderr, flush=True))
+        except Exception as e:   # the plain measurement stands on its own
+            out["learned"] = {"error": "%s: %s" % (type(e).__name__, e)}
     print(json.dumps(out), flush=True)
 
 

@@ -14,6 +14,7 @@ context store the residual at its start (and after a few sweeps) with the
 normalised change of p the solve then made.
 """
 import ctypes as C
+import time
 
 import numpy as np
 
@@ -143,3 +144,153 @@ class PressureNet:
 
 def round8(n):
     return (int(n) + 7) // 8 * 8
+
+
+def frame_for(grids, coarsen):
+    """Smallest network frame (multiples of 8) holding every grid's interior
+    coarsened by `coarsen`."""
+    h = max(-(-(ct.shape[0] - 2) // coarsen) for ct, _, _ in grids)
+    w = max(-(-(ct.shape[1] - 2) // coarsen) for ct, _, _ in grids)
+    return round8(h), round8(w)
+
+
+def make_context(grids, device=0):
+    from .simulation import Context
+    ctx = Context(device)
+    for ct, vel, prm in grids:
+        ctx.add_grid(prm, ct, vel)
+    ctx.construct()
+    return ctx
+
+
+def _quiet(*_):
+    pass
+
+
+def train_net(grids, coarsen=4, capture_ticks=96, points=4, capture_sweeps=32, steps=3000,
+              batch=32, lr=1e-3, dagger_rounds=2, dagger_ticks=32, dagger_steps=1500,
+              cycles=16, sweeps=32, seed=1, log=_quiet):
+    """Train a pressure network on pairs the solver produces on `grids`:
+    `capture_ticks` plain ticks storing `points` pairs per channel, `steps`
+    Adam steps, then `dagger_rounds` rounds of learned ticks that store pairs
+    from the learned solver's own states, each followed by `dagger_steps`
+    more steps.  Returns (params, info)."""
+    H, W = frame_for(grids, coarsen)
+    n = len(grids)
+    ctx = make_context(grids)
+    net = PressureNet(ctx, H, W, max(n, batch), seed=seed)
+    t0 = time.perf_counter()
+    try:
+        net.reserve(points * n * (capture_ticks + dagger_rounds * dagger_ticks))
+        ctx.set_capture(net, points=points, sweeps=capture_sweeps, coarsen=coarsen)
+        plain_sweeps = []
+        for i in range(capture_ticks):
+            its, _ = ctx.tick()
+            plain_sweeps.append(float(np.mean(its)))
+        ctx.set_capture(None)
+        log("captured %d pairs from %d plain ticks" % (net.dataset_size(), capture_ticks))
+
+        def fit(nsteps, rate):
+            done, loss = 0, None
+            while done < nsteps:
+                k = min(500, nsteps - done)
+                loss = net.train(k, batch, lr=rate if done < 0.7 * nsteps else rate * 0.3,
+                                 seed=seed * 1000003 + done + 7919 * nsteps)
+                done += k
+            return loss
+
+        loss = fit(steps, lr)
+        log("trained %d steps, loss %.4f" % (steps, loss))
+        rounds = []
+        for r in range(dagger_rounds):
+            ctx.set_learned(net, cycles=cycles, sweeps=sweeps, step="energy", coarsen=coarsen)
+            ctx.set_capture(net, points=points, sweeps=capture_sweeps, coarsen=coarsen)
+            sw = []
+            for _ in range(dagger_ticks):
+                its, _ = ctx.tick()
+                sw.append(float(np.mean(its)))
+            ctx.set_capture(None)
+            ctx.set_learned(None)
+            loss = fit(dagger_steps, lr * 0.5)
+            rounds.append({"learned_sweeps_per_solve": float(np.mean(sw)), "loss": loss})
+            log("round %d: learned sweeps/solve %.1f, loss %.4f" % (r, np.mean(sw), loss))
+        st = net.stats()
+        info = {
+            "frame": [H, W], "coarsen": coarsen, "params": net.nparam,
+            "pairs": net.dataset_size(), "capture_ticks": capture_ticks, "points": points,
+            "plain_sweeps_per_solve_during_capture": float(np.mean(plain_sweeps)),
+            "train_steps": int(st["train_steps"]), "batch": batch, "final_loss": loss,
+            "dagger_rounds": rounds,
+            "train_ms": st["train_ms"],
+            "train_tflops": (3.0 * st["flops_forward_per_sample"] * st["train_samples"]
+                             / (st["train_ms"] * 1e-3) / 1e12) if st["train_ms"] else None,
+            "wall_s": time.perf_counter() - t0,
+        }
+        return net.get_params(), info
+    finally:
+        ctx.set_learned(None)
+        ctx.set_capture(None)
+        net.close()
+        ctx.close()
+
+
+def _run_ticks(ctx, ticks, cells):
+    ctx.synchronize()
+    sweeps = 0.0
+    t = time.perf_counter()
+    for _ in range(ticks):
+        its, _ = ctx.tick()
+        sweeps += float(np.sum(its))
+    ctx.synchronize()
+    el = time.perf_counter() - t
+    bound = True
+    for c in range(len(cells)):
+        s = ctx.state(c)
+        bound = bound and s["residual_squared"] <= s["tolerance_squared"] and not s["exhausted"]
+    return el, sweeps, bound
+
+
+def compare(grids, params, coarsen=4, cycles=16, sweeps=32, warm=20, ticks=10, log=_quiet):
+    """Plain red-black solve vs the learned solve on `grids`, each from the
+    same state after `warm` plain ticks (two contexts, same deterministic
+    warm-up).  Every channel of both runs ends each solve at its residual
+    bound.  Returns a dict of throughputs."""
+    cells = [(ct.shape[0] - 2) * (ct.shape[1] - 2) for ct, _, _ in grids]
+    H, W = frame_for(grids, coarsen)
+    out = {"channels": len(grids), "interior_cells": int(sum(cells)), "ticks": ticks,
+           "warm": warm}
+    ctx = make_context(grids)
+    for _ in range(warm):
+        ctx.tick()
+    el, sw, ok = _run_ticks(ctx, ticks, cells)
+    ctx.close()
+    out["plain"] = {"cell_updates_per_s": sum(cells) * ticks / el, "ms_per_tick": el * 1e3 / ticks,
+                    "sweeps_per_solve": sw / ticks / len(grids), "residual_bound_met": ok}
+    log("plain: %.2f ms/tick, %.1f sweeps/solve" % (el * 1e3 / ticks, sw / ticks / len(grids)))
+    ctx = make_context(grids)
+    net = PressureNet(ctx, H, W, len(grids), seed=1)
+    try:
+        net.set_params(params)
+        for _ in range(warm):
+            ctx.tick()
+        ctx.set_learned(net, cycles=cycles, sweeps=sweeps, step="energy", coarsen=coarsen)
+        el, sw, ok = _run_ticks(ctx, ticks, cells)
+        st = net.stats()
+        ctx.set_learned(None)
+    finally:
+        net.close()
+        ctx.close()
+    fwd_flops = st["flops_forward_per_sample"] * st["forward_samples"]
+    out["learned"] = {
+        "cell_updates_per_s": sum(cells) * ticks / el, "ms_per_tick": el * 1e3 / ticks,
+        "sweeps_per_solve": sw / ticks / len(grids), "residual_bound_met": ok,
+        "cycles": cycles, "sweeps_per_cycle": sweeps, "coarsen": coarsen, "frame": [H, W],
+        "forward_ms_per_tick": st["forward_ms"] / ticks,
+        "forward_tflops": fwd_flops / (st["forward_ms"] * 1e-3) / 1e12 if st["forward_ms"] else None,
+    }
+    out["speedup"] = out["learned"]["cell_updates_per_s"] / out["plain"]["cell_updates_per_s"]
+    out["sweeps_saved"] = 1.0 - out["learned"]["sweeps_per_solve"] / out["plain"]["sweeps_per_solve"]
+    log("learned: %.2f ms/tick, %.1f sweeps/solve, %.2fx" % (el * 1e3 / ticks,
+                                                             sw / ticks / len(grids),
+                                                             out["speedup"]))
+    return out

@@ -4,6 +4,7 @@ square_channel  -- flow past a 4x4 square (imax 100, jmax 20, 102x22 cells with 
 cavity          -- lid-driven cavity, n x n cells on the unit square
 random_channels -- many independent channels of uneven size with random obstacles
 random_cavities -- lid cavities with random obstacles (training data for the network)
+fixed_channels  -- many channels of one size with random obstacles (learned + batched)
 
 All random draws come from SplitMix64 (presets.SplitMix64) with the stated seed, so
 the same seed gives the same layouts and parameters.
@@ -59,5 +60,22 @@ def random_cavities(count, n=256, seed=0, re=(50.0, 1000.0), eps=1e-6, omega=1.8
         ct, vel = presets.random_cavity(rng, n, n)
         prm = make_params(1.0 / n, 1.0 / n, 0.0, 0.9, r, eps, itermax, omega, tau=tau,
                           solver="rb", check_every=check_every)
+        out.append((ct, vel, prm))
+    return out
+
+
+def fixed_channels(count=64, seed=1, imax=512, jmax=256, re=(100.0, 1000.0), eps=1e-7,
+                   omega=1.7, tau=0.5, check_every=0, itermax=10_000_000):
+    """`count` channels of imax x jmax interior cells with 1-4 random NoSlip
+    rectangles (as random_channels), Re uniform in `re`, inflow u = 1 west,
+    outflow east, 1 unit high, red-black SOR to relative residual eps."""
+    rng = presets.SplitMix64(seed)
+    out = []
+    for _ in range(count):
+        r = re[0] + rng.uniform() * (re[1] - re[0])
+        ct, vel = presets.random_channel(rng, imax, jmax)
+        h = 1.0 / jmax
+        prm = make_params(h, h, 0.0, 0.9, r, eps, itermax, omega, tau=tau, solver="rb",
+                          check_every=check_every)
         out.append((ct, vel, prm))
     return out

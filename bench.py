@@ -57,7 +57,7 @@ def parse():
     ap.add_argument("--profile", action="store_true",
                     help="short fixed run for hardware counters, no JSON line")
     ap.add_argument("--workload", choices=("cavity", "channels", "learned_cavities",
-                                           "learned_channels"), default="cavity",
+                                           "learned_channels", "net"), default="cavity",
                     help="cavity: the 1024x1024 lid cavity (default); channels: 256 "
                          "independent channels of uneven size, seed 0; learned_*: plain vs "
                          "learned solve")
@@ -232,18 +232,51 @@ def learned_channels(ticks, log=None):
     from stroemung_amd import learned, workloads
     log = log or (lambda *x: None)
     train = workloads.fixed_channels(64, seed=1001, eps=1e-7)
-    params, info = learned.train_net(train, coarsen=4, capture_ticks=40, points=4,
+    params, info = learned.train_net(train, coarsen=8, capture_ticks=40, points=4,
                                      capture_sweeps=32, steps=3000, batch=32, dagger_rounds=2,
-                                     dagger_ticks=24, dagger_steps=1500, cycles=16, sweeps=32,
+                                     dagger_ticks=24, dagger_steps=1500, cycles=64, sweeps=32,
                                      log=log)
     test = workloads.fixed_channels(64, seed=1, eps=1e-7)
-    res = learned.compare(test, params, coarsen=4, cycles=16, sweeps=32, warm=10, ticks=ticks,
+    res = learned.compare(test, params, coarsen=8, cycles=64, sweeps=32, warm=10, ticks=ticks,
                           log=log)
     res["training"] = info
     res["workload"] = ("64 channels 512x256 with 1-4 random obstacles, Re 100-1000, inflow u=1, "
                        "red-black SOR omega=1.7 to relative residual 1e-7, tau=0.5, fp64; "
                        "trained on seed 1001, measured on seed 1 after 10 plain ticks")
     return res
+
+
+def net_line(a):
+    """The U-Net's forward and training step alone, on the frames the learned
+    solves use: TFLOP/s against the BF16 dense MFMA peak."""
+    from stroemung_amd.learned import PressureNet
+    from stroemung_amd.simulation import Context
+    ctx = Context(0)
+    rows = []
+    for (h, w, nb) in ((64, 64, 32), (128, 64, 64), (256, 256, 8)):
+        net = PressureNet(ctx, h, w, nb, seed=1)
+        fwd, trn = net.benchmark(nb, max(a.steps, 3), train=True)
+        flops = net.stats()["flops_forward_per_sample"] * nb
+        rows.append({"frame": [h, w], "batch": nb, "forward_ms": fwd, "train_step_ms": trn,
+                     "forward_tflops": flops / fwd / 1e9,
+                     "train_tflops": 3 * flops / trn / 1e9,
+                     "forward_frac": flops / fwd / 1e9 / MFMA_BF16_PEAK_TFLOPS})
+        net.close()
+    ctx.close()
+    best = max(rows, key=lambda r: r["forward_tflops"])
+    out = {
+        "metric": "U-Net forward TFLOP/s (bf16 MFMA, fp32 accumulate)",
+        "value": rows[0]["forward_tflops"], "unit": "TFLOP/s", "n_gpus": 1,
+        "steps": max(a.steps, 3), "warmup": 1, "ms_per_step": rows[0]["forward_ms"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16",
+        "data": "synthetic",
+        "config": {"workload": "4-level U-Net, 32 base channels, frames of the learned solves",
+                   "frames": rows},
+        "roofline": {"kernel": "U-Net forward", "bound": "mfma",
+                     "achieved": best["forward_tflops"], "peak": MFMA_BF16_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": best["forward_frac"], "traffic": None},
+    }
+    print(json.dumps(out), flush=True)
 
 
 def learned_line(a):
@@ -288,6 +321,9 @@ def main():
         return
     if a.workload.startswith("learned"):
         learned_line(a)
+        return
+    if a.workload == "net":
+        net_line(a)
         return
     n = a.n
     ctx, ch = make_context(a)

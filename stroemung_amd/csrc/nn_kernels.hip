@@ -112,6 +112,98 @@ k_conv3x3(const bf16* __restrict__ in, int cin, const bf16* __restrict__ w,
     }
 }
 
+// LDS-tiled variant for images at least 8 pixels wide.  A workgroup owns a
+// TH x TW pixel tile of one image (TH * TW = 128) and 32 * WN output channels;
+// for every 32-channel slice of the input it stages the (TH + 2) x (TW + 2)
+// patch once in LDS and serves all nine taps from it, so each input value is
+// read from memory once per slice instead of nine times.  Waves: (4 / WN) in
+// M x WN in N, each (128 * WN / 4) pixels x 32 channels.
+template <int TW, int WN>
+__global__ void __launch_bounds__(256)
+k_conv3x3_tile(const bf16* __restrict__ in, int cin, const bf16* __restrict__ w,
+               const float* __restrict__ bias, bf16* __restrict__ out, float* __restrict__ out32,
+               int cout, int nb, int H, int W, int flags)
+{
+    constexpr int TH = 128 / TW, PH = TH + 2, PW = TW + 2;
+    constexpr int CS = 40;                       // padded channel stride of a staged pixel
+    constexpr int WM = 4 / WN;                   // waves along M
+    constexpr int MT = 128 / WM / 16;            // M tiles per wave
+    __shared__ __attribute__((aligned(16))) bf16 patch[PH * PW * CS];
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave / WN, wn = wave % WN;
+    const int tiles_w = (W + TW - 1) / TW, tiles_h = (H + TH - 1) / TH;
+    int t = blockIdx.x;
+    const int tw0 = (t % tiles_w) * TW;
+    t /= tiles_w;
+    const int th0 = (t % tiles_h) * TH;
+    const int b = t / tiles_h;
+    const int n_base = blockIdx.y * (32 * WN) + wn * 32;
+    const int row = lane & 15, kq = (lane >> 4) * 8;
+
+    // the LDS offset of each M tile's pixel for this lane (tap (0, 0) at +1, +1)
+    int aoff[MT];
+#pragma unroll
+    for (int i = 0; i < MT; i++) {
+        const int m = (wm * MT + i) * 16 + row;
+        aoff[i] = ((m / TW) * PW + (m % TW)) * CS + kq;
+    }
+    f32x4 acc[MT][2];
+#pragma unroll
+    for (int i = 0; i < MT; i++)
+        for (int u = 0; u < 2; u++) acc[i][u] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    const bf16* img = in + (int64_t)b * H * W * cin;
+    for (int c0 = 0; c0 < cin; c0 += 32) {
+        __syncthreads();
+        for (int v = threadIdx.x; v < PH * PW * 4; v += 256) {
+            const int pix = v >> 2, part = v & 3;
+            const int ph = pix / PW, pw = pix - ph * PW;
+            const int gh = th0 + ph - 1, gw = tw0 + pw - 1;
+            bf16x8 x = zero8();
+            if (gh >= 0 && gh < H && gw >= 0 && gw < W)
+                x = load8(img + ((int64_t)gh * W + gw) * cin + c0 + part * 8);
+            *reinterpret_cast<bf16x8*>(&patch[pix * CS + part * 8]) = x;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int tap = 0; tap < 9; tap++) {
+            const int dy = tap / 3, dx = tap % 3;    // patch offsets 0..2
+            bf16x8 bv[2];
+#pragma unroll
+            for (int u = 0; u < 2; u++)
+                bv[u] = load8(w + ((int64_t)(n_base + u * 16 + row) * 9 + tap) * cin + c0 + kq);
+            const int shift = (dy * PW + dx) * CS;
+#pragma unroll
+            for (int i = 0; i < MT; i++) {
+                const bf16x8 a = *reinterpret_cast<const bf16x8*>(&patch[aoff[i] + shift]);
+#pragma unroll
+                for (int u = 0; u < 2; u++)
+                    acc[i][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bv[u], acc[i][u], 0, 0, 0);
+            }
+        }
+    }
+    // epilogue: lane holds column n = l % 16, rows 4 * (l / 16) + j of each M tile
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+        const int n = n_base + u * 16 + row;
+        const float bvs = (flags & 1) ? bias[n] : 0.f;
+#pragma unroll
+        for (int i = 0; i < MT; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int m = (wm * MT + i) * 16 + (lane >> 4) * 4 + j;
+                const int gh = th0 + m / TW, gw = tw0 + m % TW;
+                if (gh >= H || gw >= W) continue;
+                float v = acc[i][u][j] + bvs;
+                if (flags & 2) v = v > 0.f ? v : 0.f;
+                const int64_t o = (((int64_t)b * H + gh) * W + gw) * cout + n;
+                if (out) out[o] = (bf16)v;
+                if (out32) out32[o] = v;
+            }
+    }
+}
+
 // w'[ci][8 - tap][co] = w[co][tap][ci]
 __global__ void k_weight_flip(const bf16* __restrict__ w, bf16* __restrict__ wt, int cin, int cout)
 {
@@ -401,9 +493,29 @@ __global__ void k_to_bf16(int64_t n, const float* __restrict__ x, bf16* __restri
 
 static inline unsigned nblk(int64_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
 
+template <int TW, int WN>
+static void conv_tile(hipStream_t st, const bf16* in, int cin, const bf16* w, const float* bias,
+                      bf16* out, float* out32, int cout, int nb, int H, int W, int flags) {
+    constexpr int TH = 128 / TW;
+    const unsigned tiles = (unsigned)(((W + TW - 1) / TW) * ((H + TH - 1) / TH) * nb);
+    dim3 grid(tiles, cout / (32 * WN));
+    k_conv3x3_tile<TW, WN><<<grid, 256, 0, st>>>(in, cin, w, bias, out, out32, cout, nb, H, W,
+                                                 flags);
+}
+
 void conv3x3(hipStream_t st, const bf16* in, int cin, const bf16* w, const float* bias, bf16* out,
              float* out32, int cout, int nb, int H, int W, int flags) {
     const int64_t npix = (int64_t)nb * H * W;
+    // staged tiles where a full 128-pixel tile fits the image
+    const bool w16 = W % 16 == 0 && H >= 8, w8 = W % 8 == 0 && H >= 16;
+    if (w16 || w8) {
+        const bool n64 = cout % 64 == 0;
+        if (w16 && n64) conv_tile<16, 2>(st, in, cin, w, bias, out, out32, cout, nb, H, W, flags);
+        else if (w16) conv_tile<16, 1>(st, in, cin, w, bias, out, out32, cout, nb, H, W, flags);
+        else if (n64) conv_tile<8, 2>(st, in, cin, w, bias, out, out32, cout, nb, H, W, flags);
+        else conv_tile<8, 1>(st, in, cin, w, bias, out, out32, cout, nb, H, W, flags);
+        return;
+    }
     if (npix >= 64 * 256) {
         dim3 grid(nblk(npix, 128), (cout + 63) / 64);
         k_conv3x3<128><<<grid, 256, 0, st>>>(in, cin, w, bias, out, out32, cout, nb, H, W, flags);

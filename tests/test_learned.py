@@ -209,14 +209,15 @@ def _sparse_params(seed):
 
 
 @pytest.mark.gpu
-def test_net_exact_arithmetic_forward_is_bit_identical_and_gradients_tight():
+@pytest.mark.parametrize("H,W", [(32, 40), (32, 48), (64, 64)])
+def test_net_exact_arithmetic_forward_is_bit_identical_and_gradients_tight(H, W):
     """Integer activations: no rounding in the forward pass, so any index
     mistake (tap, channel, pooling window, upsampling, concatenation order,
     head) shows as a difference; the backward pass then rounds only its
     gradients to BF16, at the places the emulated reference does."""
     from stroemung_amd.learned import PressureNet
     ctx = _gpu_ctx()
-    H, W, nb = 32, 40, 2
+    nb = 2
     net = PressureNet(ctx, H, W, nb, seed=1)
     p = _sparse_params(13)
     net.set_params(p)

@@ -659,7 +659,7 @@ int learn_correct(stm_ctx* ctx, Net& net, int f) {
     LAUNCH("learn_pack", launch_learn_pack(ctx->stream, n, ctx->d_lc, ctx->lrc, ctx->lmc, net.H,
                                            net.W, net.input(), nullptr, 0, 1));
     HIPCHK(ctx, hipEventRecord(ctx->nn_ev[0], ctx->stream));
-    LAUNCH("nn_forward", net.forward(n));
+    LAUNCH("nn_forward", net.forward_infer(n));
     HIPCHK(ctx, hipEventRecord(ctx->nn_ev[1], ctx->stream));
     LAUNCH("learn_prolong", launch_learn_prolong(ctx->stream, ctx->ntiles, ctx->d_cd,
                                                  ctx->d_tiles, ctx->d_lc, ctx->flags, net.y,

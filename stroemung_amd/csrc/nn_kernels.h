@@ -8,11 +8,27 @@ namespace nn {
 
 typedef __bf16 bf16;
 
+// How a convolution reads its input: channels [0, ca) from `a` (directly,
+// as the 2x2 max of an image twice the size, or as the nearest upsampling of
+// an image half the size), channels [ca, ca + cb) from `b` directly.
+enum : int { IN_DIRECT = 0, IN_POOL = 1, IN_UP = 2 };
+struct ConvIn {
+    const bf16* a;
+    int ca, mode;
+    const bf16* b;
+    int cb;
+};
+
 // 3x3 convolution, padding 1, NHWC bf16, fp32 accumulation on the matrix
 // cores.  w is [cout][9][cin] bf16; flags 1 = add bias (fp32), 2 = ReLU.
 // Writes bf16 `out` and/or fp32 `out32` (either may be null).
 void conv3x3(hipStream_t, const bf16* in, int cin, const bf16* w, const float* bias, bf16* out,
              float* out32, int cout, int nb, int H, int W, int flags);
+// The staged-tile convolution takes every image whose sides are multiples
+// of 8; conv3x3_in is that convolution with a described input.
+bool conv_tiles_fit(int H, int W);
+void conv3x3_in(hipStream_t, const ConvIn& src, const bf16* w, const float* bias, bf16* out,
+                float* out32, int cout, int nb, int H, int W, int flags);
 // w'[ci][8 - tap][co] = w[co][tap][ci]: the backward-input weights
 void weight_flip(hipStream_t, const bf16* w, bf16* wt, int cin, int cout);
 // dw32[co][tap][ci] += sum_pix gz[pix][co] * in[pix + off(tap)][ci]

@@ -112,24 +112,32 @@ k_conv3x3(const bf16* __restrict__ in, int cin, const bf16* __restrict__ w,
     }
 }
 
-// LDS-tiled variant for images at least 8 pixels wide.  A workgroup owns a
-// TH x TW pixel tile of one image (TH * TW = 128) and 32 * WN output channels;
-// for every 32-channel slice of the input it stages the (TH + 2) x (TW + 2)
-// patch once in LDS and serves all nine taps from it, so each input value is
-// read from memory once per slice instead of nine times.  Waves: (4 / WN) in
-// M x WN in N, each (128 * WN / 4) pixels x 32 channels.
-template <int TW, int WN>
+// LDS-tiled variant.  A workgroup owns a TH x TW pixel tile of one image
+// (TH * TW = 64 or 128) and 32 * WN output channels; for every 32-channel
+// slice of the input it stages the (TH + 2) x (TW + 2) patch once in LDS and
+// serves all nine taps from it, so each input value is read from memory once
+// per slice instead of nine times.  Waves: (4 / WN) in M x WN in N.
+//
+// The input is described by ConvIn: channels [0, ca) come from `a` read
+// directly, as the 2x2 max of an image twice the size (the U-Net's pooling)
+// or as the nearest upsampling of an image half the size; channels
+// [ca, ca + cb) come from `b` read directly (the skip half of a
+// concatenation).  Pooling, upsampling and concatenation thus happen while
+// staging, without materialising their outputs.
+template <int TW, int TH, int WN>
 __global__ void __launch_bounds__(256)
-k_conv3x3_tile(const bf16* __restrict__ in, int cin, const bf16* __restrict__ w,
-               const float* __restrict__ bias, bf16* __restrict__ out, float* __restrict__ out32,
-               int cout, int nb, int H, int W, int flags)
+k_conv3x3_tile(ConvIn src, const bf16* __restrict__ w, const float* __restrict__ bias,
+               bf16* __restrict__ out, float* __restrict__ out32, int cout, int nb, int H, int W,
+               int flags)
 {
-    constexpr int TH = 128 / TW, PH = TH + 2, PW = TW + 2;
+    constexpr int PH = TH + 2, PW = TW + 2;
     constexpr int CS = 40;                       // padded channel stride of a staged pixel
     constexpr int WM = 4 / WN;                   // waves along M
-    constexpr int MT = 128 / WM / 16;            // M tiles per wave
+    constexpr int MT = TH * TW / WM / 16;        // M tiles per wave
+    static_assert(MT >= 1 && TH * TW % (WM * 16) == 0, "tile shape");
     __shared__ __attribute__((aligned(16))) bf16 patch[PH * PW * CS];
 
+    const int cin = src.ca + src.cb;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave / WN, wn = wave % WN;
     const int tiles_w = (W + TW - 1) / TW, tiles_h = (H + TH - 1) / TH;
@@ -153,16 +161,34 @@ k_conv3x3_tile(const bf16* __restrict__ in, int cin, const bf16* __restrict__ w,
     for (int i = 0; i < MT; i++)
         for (int u = 0; u < 2; u++) acc[i][u] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    const bf16* img = in + (int64_t)b * H * W * cin;
     for (int c0 = 0; c0 < cin; c0 += 32) {
         __syncthreads();
+        const bool first = c0 < src.ca;
+        const int mode = first ? src.mode : IN_DIRECT;
+        const bf16* base = first ? src.a : src.b;
+        const int cs = first ? src.ca : src.cb, cc = first ? c0 : c0 - src.ca;
         for (int v = threadIdx.x; v < PH * PW * 4; v += 256) {
             const int pix = v >> 2, part = v & 3;
             const int ph = pix / PW, pw = pix - ph * PW;
             const int gh = th0 + ph - 1, gw = tw0 + pw - 1;
             bf16x8 x = zero8();
-            if (gh >= 0 && gh < H && gw >= 0 && gw < W)
-                x = load8(img + ((int64_t)gh * W + gw) * cin + c0 + part * 8);
+            if (gh >= 0 && gh < H && gw >= 0 && gw < W) {
+                const int ch = cc + part * 8;
+                if (mode == IN_DIRECT) {
+                    x = load8(base + (((int64_t)b * H + gh) * W + gw) * cs + ch);
+                } else if (mode == IN_UP) {
+                    x = load8(base + (((int64_t)b * (H / 2) + gh / 2) * (W / 2) + gw / 2) * cs + ch);
+                } else {   // IN_POOL: 2x2 max from (2H) x (2W)
+                    const int64_t r0 = (((int64_t)b * (2 * H) + 2 * gh) * (2 * W) + 2 * gw) * cs + ch;
+                    const int64_t rs = (int64_t)(2 * W) * cs;
+                    const bf16x8 p0 = load8(base + r0), p1 = load8(base + r0 + cs);
+                    const bf16x8 p2 = load8(base + r0 + rs), p3 = load8(base + r0 + rs + cs);
+#pragma unroll
+                    for (int e = 0; e < 8; e++)
+                        x[e] = (bf16)fmaxf(fmaxf((float)p0[e], (float)p1[e]),
+                                           fmaxf((float)p2[e], (float)p3[e]));
+                }
+            }
             *reinterpret_cast<bf16x8*>(&patch[pix * CS + part * 8]) = x;
         }
         __syncthreads();
@@ -493,27 +519,39 @@ __global__ void k_to_bf16(int64_t n, const float* __restrict__ x, bf16* __restri
 
 static inline unsigned nblk(int64_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
 
-template <int TW, int WN>
-static void conv_tile(hipStream_t st, const bf16* in, int cin, const bf16* w, const float* bias,
+template <int TW, int TH, int WN>
+static void conv_tile(hipStream_t st, const ConvIn& src, const bf16* w, const float* bias,
                       bf16* out, float* out32, int cout, int nb, int H, int W, int flags) {
-    constexpr int TH = 128 / TW;
     const unsigned tiles = (unsigned)(((W + TW - 1) / TW) * ((H + TH - 1) / TH) * nb);
     dim3 grid(tiles, cout / (32 * WN));
-    k_conv3x3_tile<TW, WN><<<grid, 256, 0, st>>>(in, cin, w, bias, out, out32, cout, nb, H, W,
-                                                 flags);
+    k_conv3x3_tile<TW, TH, WN><<<grid, 256, 0, st>>>(src, w, bias, out, out32, cout, nb, H, W,
+                                                     flags);
+}
+
+bool conv_tiles_fit(int H, int W) { return W % 8 == 0 && H % 8 == 0; }
+
+void conv3x3_in(hipStream_t st, const ConvIn& src, const bf16* w, const float* bias, bf16* out,
+                float* out32, int cout, int nb, int H, int W, int flags) {
+    // two N blocks per workgroup only where that still leaves >= 512 workgroups
+    const int64_t pix = (int64_t)nb * H * W;
+    const bool n64 = cout % 64 == 0 && (pix / 128) * (cout / 64) >= 512;
+    if (W % 16 == 0) {
+        if (n64) conv_tile<16, 8, 2>(st, src, w, bias, out, out32, cout, nb, H, W, flags);
+        else conv_tile<16, 8, 1>(st, src, w, bias, out, out32, cout, nb, H, W, flags);
+    } else if (H % 16 == 0) {
+        if (n64) conv_tile<8, 16, 2>(st, src, w, bias, out, out32, cout, nb, H, W, flags);
+        else conv_tile<8, 16, 1>(st, src, w, bias, out, out32, cout, nb, H, W, flags);
+    } else {   // 8 x 8 tiles of 64 pixels
+        conv_tile<8, 8, 1>(st, src, w, bias, out, out32, cout, nb, H, W, flags);
+    }
 }
 
 void conv3x3(hipStream_t st, const bf16* in, int cin, const bf16* w, const float* bias, bf16* out,
              float* out32, int cout, int nb, int H, int W, int flags) {
     const int64_t npix = (int64_t)nb * H * W;
-    // staged tiles where a full 128-pixel tile fits the image
-    const bool w16 = W % 16 == 0 && H >= 8, w8 = W % 8 == 0 && H >= 16;
-    if (w16 || w8) {
-        const bool n64 = cout % 64 == 0;
-        if (w16 && n64) conv_tile<16, 2>(st, in, cin, w, bias, out, out32, cout, nb, H, W, flags);
-        else if (w16) conv_tile<16, 1>(st, in, cin, w, bias, out, out32, cout, nb, H, W, flags);
-        else if (n64) conv_tile<8, 2>(st, in, cin, w, bias, out, out32, cout, nb, H, W, flags);
-        else conv_tile<8, 1>(st, in, cin, w, bias, out, out32, cout, nb, H, W, flags);
+    if (conv_tiles_fit(H, W)) {
+        const ConvIn src{in, cin, IN_DIRECT, nullptr, 0};
+        conv3x3_in(st, src, w, bias, out, out32, cout, nb, H, W, flags);
         return;
     }
     if (npix >= 64 * 256) {

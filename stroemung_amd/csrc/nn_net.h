@@ -72,7 +72,8 @@ struct Net {
 
     int init(stm_ctx* c, int H, int W, int cap, uint64_t seed);
     void release();
-    void forward(int nb);
+    void forward(int nb);          // materialises every activation (training)
+    void forward_infer(int nb);    // pooling / upsampling / concatenation fused into staging
     void backward(int nb);
     int train_step(int nb, float lr);
 };

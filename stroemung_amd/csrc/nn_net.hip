@@ -181,6 +181,65 @@ void Net::forward(int nb) {
     nn::head(stream, acts[20].v, 32, p32 + head_w, p32 + head_b, y, npix(0));
 }
 
+// Inference: the same network with pooling, upsampling and concatenation
+// done while each convolution stages its input, on every level whose sides
+// are multiples of 8; other levels materialise those inputs as in forward().
+void Net::forward_infer(int nb) {
+    using nn::ConvIn;
+    using nn::conv_tiles_fit;
+    auto npix = [&](int level) { return (int64_t)nb * h(level) * w(level); };
+    auto run = [&](int k, const ConvIn& in) {
+        const Conv& c = convs[k];
+        nn::conv3x3_in(stream, in, pb + c.woff, p32 + c.boff, acts[c.y].v, nullptr, c.cout, nb,
+                   h(c.level), w(c.level), 3);
+    };
+    auto plain = [&](int k) {
+        const Conv& c = convs[k];
+        nn::conv3x3(stream, acts[c.x].v, c.cin, pb + c.woff, p32 + c.boff, acts[c.y].v, nullptr,
+                    c.cout, nb, h(c.level), w(c.level), 3);
+    };
+    // conv k whose input is the pooled activation `src` (level - 1)
+    auto pooled = [&](int k, int src, int pooled_act) {
+        const Conv& c = convs[k];
+        if (conv_tiles_fit(h(c.level), w(c.level))) {
+            run(k, ConvIn{acts[src].v, acts[src].c, nn::IN_POOL, nullptr, 0});
+        } else {
+            nn::maxpool(stream, acts[src].v, acts[pooled_act].v, nb, h(c.level - 1),
+                        w(c.level - 1), acts[src].c);
+            plain(k);
+        }
+    };
+    // conv k whose input is [up(low), skip] materialised in `cat`
+    auto joined = [&](int k, int low, int skip, int cat) {
+        const Conv& c = convs[k];
+        const int cl = acts[low].c, cs = acts[skip].c;
+        if (conv_tiles_fit(h(c.level), w(c.level))) {
+            run(k, ConvIn{acts[low].v, cl, nn::IN_UP, acts[skip].v, cs});
+        } else {
+            nn::upsample(stream, acts[low].v, acts[cat].v, nb, h(c.level), w(c.level), cl, cl + cs,
+                         0);
+            nn::channel_copy(stream, acts[skip].v, cs, 0, acts[cat].v, cl + cs, cl, cs,
+                             npix(c.level), 0);
+            plain(k);
+        }
+    };
+    plain(0);
+    plain(1);
+    pooled(2, 2, 3);
+    plain(3);
+    pooled(4, 5, 6);
+    plain(5);
+    pooled(6, 8, 9);
+    plain(7);
+    joined(8, 11, 8, 12);
+    plain(9);
+    joined(10, 14, 5, 15);
+    plain(11);
+    joined(12, 17, 2, 18);
+    plain(13);
+    nn::head(stream, acts[20].v, 32, p32 + head_w, p32 + head_b, y, npix(0));
+}
+
 void Net::backward(int nb) {
     auto npix = [&](int level) { return (int64_t)nb * h(level) * w(level); };
     // gradient through conv k (ReLU included); the gradient of its output is in acts[y].g
@@ -348,7 +407,7 @@ int stm_net_forward(stm_net* n, uint32_t nb, const float* x, float* y) {
     k_pack_host<<<(unsigned)((npix + 255) / 256), 256, 0, net.stream>>>(dxh, nullptr, npix,
                                                                        net.input(), nullptr,
                                                                        nullptr);
-    net.forward((int)nb);
+    net.forward_infer((int)nb);
     hipMemcpyAsync(y, net.y, sizeof(float) * npix, hipMemcpyDeviceToHost, net.stream);
     hipError_t e = hipStreamSynchronize(net.stream);
     hipFree(dxh);
@@ -483,9 +542,9 @@ int stm_net_benchmark(stm_net* n, uint32_t nb, uint32_t reps, double* fwd_ms, do
     NCHK(hipEventCreate(&e2));
     // the input is whatever the buffers hold: the matrix-core time does not
     // depend on the values
-    net.forward((int)nb);   // warm-up
+    net.forward_infer((int)nb);   // warm-up
     NCHK(hipEventRecord(e0, net.stream));
-    for (uint32_t r = 0; r < reps; r++) net.forward((int)nb);
+    for (uint32_t r = 0; r < reps; r++) net.forward_infer((int)nb);
     NCHK(hipEventRecord(e1, net.stream));
     if (train_ms) {
         NCHK(hipMemsetAsync(net.target, 0, sizeof(float) * (size_t)nb * net.H * net.W, net.stream));

@@ -243,17 +243,20 @@ __global__ void k_weight_flip(const bf16* __restrict__ w, bf16* __restrict__ wt,
 }
 
 // dw[co][tap][ci] += sum over a pixel chunk of gz[pix][co] * in[pix+off][ci].
-// Block: 256 threads, tile 64 co x 64 ci for one tap, pixel chunks of 32
-// staged transposed in LDS ([channel][pixel]) so each lane's 8 consecutive
-// pixels are one 16-byte read.  Partial sums go to dw32 with atomics.
+// Block: 256 threads, a tile of up to 64 co x 64 ci for one tap; chunks of
+// 128 pixels are staged transposed in LDS ([channel][pixel]) so each lane's 8
+// consecutive pixels (the MFMA's k) are one 16-byte read.  The four waves
+// cover the tile in 32 x 32 pieces and split the chunk's four 32-pixel k
+// steps among themselves when the tile is narrower than 64 x 64, so no wave
+// multiplies padding.  Partial sums go to dw32 with atomics.
 __global__ void __launch_bounds__(256)
 k_conv3x3_dw(const bf16* __restrict__ gz, int cout, const bf16* __restrict__ in, int cin,
              float* __restrict__ dw32, int nb, int H, int W, int pix_per_block)
 {
-    __shared__ __attribute__((aligned(16))) bf16 sa[64][40];   // [co][pix], padded rows
-    __shared__ __attribute__((aligned(16))) bf16 sb[64][40];   // [ci][pix]
+    constexpr int KP = 128, LS = KP + 8;
+    __shared__ __attribute__((aligned(16))) bf16 sa[64][LS];   // [co][pix]
+    __shared__ __attribute__((aligned(16))) bf16 sb[64][LS];   // [ci][pix]
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wm = wave >> 1, wn = wave & 1;
     const int tap = blockIdx.z % 9;
     const int dy = tap / 3 - 1, dx = tap % 3 - 1;
     const int co0 = blockIdx.y * 64;
@@ -261,44 +264,55 @@ k_conv3x3_dw(const bf16* __restrict__ gz, int cout, const bf16* __restrict__ in,
     const int npix = nb * H * W;
     const int p_begin = (blockIdx.z / 9) * pix_per_block;
     const int p_end = min(npix, p_begin + pix_per_block);
-    const int ci_n = cin - ci0 < 64 ? cin - ci0 : 64;    // 32 for 32-channel layers
+    const int ci_n = cin - ci0 < 64 ? cin - ci0 : 64;    // 32 or 64
     const int co_n = cout - co0 < 64 ? cout - co0 : 64;
+    const int wo = co_n / 32, wi = ci_n / 32, ks = 4 / (wo * wi);
+    const int mo = wave % wo, ni = (wave / wo) % wi, kg = wave / (wo * wi);
     f32x4 acc[2][2];
     for (int t = 0; t < 2; t++)
         for (int u = 0; u < 2; u++) acc[t][u] = f32x4{0.f, 0.f, 0.f, 0.f};
     const int row = lane & 15, kq = (lane >> 4) * 8;
-    // staging: thread -> (pixel j = tid / 8, channel group g = tid % 8 of 8 channels)
-    const int sj = threadIdx.x >> 3, sg = (threadIdx.x & 7) * 8;
-    for (int p0 = p_begin; p0 < p_end; p0 += 32) {
+    // staging: thread -> pixel j = tid / 2, channels [32 h, 32 h + 32), h = tid % 2
+    const int sj = threadIdx.x >> 1, sh = (threadIdx.x & 1) * 32;
+    for (int p0 = p_begin; p0 < p_end; p0 += KP) {
         const int pix = p0 + sj;
-        bf16x8 va = zero8(), vb = zero8();
+        bf16x8 va[4], vb[4];
+        for (int q = 0; q < 4; q++) {
+            va[q] = zero8();
+            vb[q] = zero8();
+        }
         if (pix < p_end) {
-            if (sg < co_n) va = load8(gz + (int64_t)pix * cout + co0 + sg);
             const int b = pix / (H * W), r = pix - b * H * W;
             const int hh = r / W + dy, ww = r % W + dx;
-            if (hh >= 0 && hh < H && ww >= 0 && ww < W && sg < ci_n)
-                vb = load8(in + ((int64_t)(b * H + hh) * W + ww) * cin + ci0 + sg);
+            const bool inside = hh >= 0 && hh < H && ww >= 0 && ww < W;
+            for (int q = 0; q < 4; q++) {
+                const int c = sh + q * 8;
+                if (c < co_n) va[q] = load8(gz + (int64_t)pix * cout + co0 + c);
+                if (inside && c < ci_n)
+                    vb[q] = load8(in + ((int64_t)(b * H + hh) * W + ww) * cin + ci0 + c);
+            }
         }
         __syncthreads();
-        for (int i = 0; i < 8; i++) {
-            sa[sg + i][sj] = va[i];
-            sb[sg + i][sj] = vb[i];
-        }
+        for (int q = 0; q < 4; q++)
+            for (int i = 0; i < 8; i++) {
+                sa[sh + q * 8 + i][sj] = va[q][i];
+                sb[sh + q * 8 + i][sj] = vb[q][i];
+            }
         __syncthreads();
-        bf16x8 a[2], b[2];
-        for (int t = 0; t < 2; t++) a[t] = load8(&sa[wm * 32 + t * 16 + row][kq]);
-        for (int u = 0; u < 2; u++) b[u] = load8(&sb[wn * 32 + u * 16 + row][kq]);
-        for (int t = 0; t < 2; t++)
-            for (int u = 0; u < 2; u++)
-                acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[t], b[u], acc[t][u], 0, 0, 0);
+        for (int k = kg; k < 4; k += ks) {
+            bf16x8 a[2], bb[2];
+            for (int t = 0; t < 2; t++) a[t] = load8(&sa[mo * 32 + t * 16 + row][k * 32 + kq]);
+            for (int u = 0; u < 2; u++) bb[u] = load8(&sb[ni * 32 + u * 16 + row][k * 32 + kq]);
+            for (int t = 0; t < 2; t++)
+                for (int u = 0; u < 2; u++)
+                    acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[t], bb[u], acc[t][u], 0, 0, 0);
+        }
     }
     for (int t = 0; t < 2; t++)
         for (int u = 0; u < 2; u++) {
-            const int ci = ci0 + wn * 32 + u * 16 + row;
-            if (ci >= ci0 + ci_n) continue;
+            const int ci = ci0 + ni * 32 + u * 16 + row;
             for (int i = 0; i < 4; i++) {
-                const int co = co0 + wm * 32 + t * 16 + (lane >> 4) * 4 + i;
-                if (co >= co0 + co_n) continue;
+                const int co = co0 + mo * 32 + t * 16 + (lane >> 4) * 4 + i;
                 atomicAdd(&dw32[((int64_t)co * 9 + tap) * cin + ci], acc[t][u][i]);
             }
         }
@@ -314,15 +328,24 @@ __global__ void k_relu_bwd(int64_t n, const bf16* __restrict__ y, bf16* __restri
 }
 
 // db[c] += sum over pixels of g[pix][c]
-__global__ void k_bias_grad(const bf16* __restrict__ g, int c, int npix, float* __restrict__ db)
+// c divides 256: thread t sums channel t % c over pixels t / c, t / c + 256 / c,
+// ... of the block's chunk (consecutive threads read consecutive channels),
+// then the 256 / c partials of each channel are added in LDS.
+__global__ void __launch_bounds__(256)
+k_bias_grad(const bf16* __restrict__ g, int c, int npix, int per, float* __restrict__ db)
 {
-    const int ch = blockIdx.y * blockDim.x + threadIdx.x;
-    if (ch >= c) return;
-    const int per = (npix + gridDim.x - 1) / gridDim.x;
+    __shared__ float red[256];
+    const int ch = threadIdx.x % c, lanes = 256 / c;
     const int p0 = blockIdx.x * per, p1 = min(npix, p0 + per);
     float s = 0.f;
-    for (int p = p0; p < p1; p++) s += (float)g[(int64_t)p * c + ch];
-    atomicAdd(&db[ch], s);
+    for (int p = p0 + threadIdx.x / c; p < p1; p += lanes) s += (float)g[(int64_t)p * c + ch];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x < c) {
+        float t = 0.f;
+        for (int k = threadIdx.x; k < 256; k += c) t += red[k];
+        atomicAdd(&db[threadIdx.x], t);
+    }
 }
 
 // 2x2 max pooling, H and W even
@@ -427,20 +450,23 @@ __global__ void k_head_bwd(const bf16* __restrict__ x, int c, const float* __res
                            const float* __restrict__ gy, bf16* __restrict__ gx,
                            float* __restrict__ dw, float* __restrict__ db, int64_t npix)
 {
-    __shared__ float red[256];
+    __shared__ float red[4][33];
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const float g = p < npix ? gy[p] : 0.f;
     if (p < npix)
         for (int i = 0; i < c; i++) gx[p * c + i] = (bf16)(g * w[i]);
+    // per wave: shuffle sums of g and of g * x[i], then the 4 waves in LDS
     for (int i = -1; i < c; i++) {
-        red[threadIdx.x] = i < 0 ? g : (p < npix ? g * (float)x[p * c + i] : 0.f);
-        __syncthreads();
-        for (int s = 128; s > 0; s >>= 1) {
-            if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) atomicAdd(i < 0 ? db : &dw[i], red[0]);
-        __syncthreads();
+        float v = i < 0 ? g : (p < npix ? g * (float)x[p * c + i] : 0.f);
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+        if (lane == 0) red[wave][i + 1] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x <= c) {
+        const float t = (red[0][threadIdx.x] + red[1][threadIdx.x]) +
+                        (red[2][threadIdx.x] + red[3][threadIdx.x]);
+        atomicAdd(threadIdx.x == 0 ? db : &dw[threadIdx.x - 1], t);
     }
 }
 
@@ -580,8 +606,9 @@ void relu_bwd(hipStream_t st, int64_t n, const bf16* y, bf16* g) {
     k_relu_bwd<<<nblk(n), 256, 0, st>>>(n, y, g);
 }
 void bias_grad(hipStream_t st, const bf16* g, int c, int64_t npix, float* db) {
-    dim3 grid(64, (c + 255) / 256);
-    k_bias_grad<<<grid, 256, 0, st>>>(g, c, (int)npix, db);
+    // bias channels are 32, 64, 128 or 256
+    const int per = 1024;
+    k_bias_grad<<<nblk(npix, per), 256, 0, st>>>(g, c, (int)npix, per, db);
 }
 void maxpool(hipStream_t st, const bf16* x, bf16* y, int nb, int H, int W, int c) {
     k_maxpool<<<nblk((int64_t)nb * H * W * c / 4), 256, 0, st>>>(x, y, nb, H, W, c);

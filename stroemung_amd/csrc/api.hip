@@ -92,8 +92,9 @@ struct stm_ctx {
     double* lsnap = nullptr;       // capture: LEARN_POINTS snapshots of p
     float *lrc = nullptr, *lmc = nullptr;   // coarse residual and fluid fraction, nch frames
     double2* d_cpart = nullptr;
+    int32_t *d_map = nullptr, *d_inv = nullptr, *d_live = nullptr;   // running channels
     int64_t lframe = 0;            // pixels per frame the coarse buffers hold
-    hipEvent_t nn_ev[2] = {nullptr, nullptr};
+    std::vector<hipEvent_t> nn_evs;  // per-cycle forward timing, read after the solve
     // profiling
     uint32_t sample = 0;
     std::unordered_map<std::string, KStat> stats;
@@ -622,16 +623,19 @@ int learn_restrict(stm_ctx* ctx, const Net& net, int f, int save) {
 
 // Blocked red-black launches on the channels still running, `budget` sweeps
 // each (0 = up to itermax); final = 1 ends the solve.  A phase with a budget
-// needs no polling: ceil(budget / K) launches sweep and one more decides, and
-// launches on a stopped channel return at once.
+// is ceil(budget / K) launches without the residual test and without a host
+// poll (the next cycle's residual is the test); launches on a channel that
+// is not running return at once.
 int learn_phase(stm_ctx* ctx, int budget, int final_phase) {
     LAUNCH("learn_arm", launch_learn_arm(ctx->stream, nch(ctx), ctx->d_cd, ctx->d_cs, ctx->d_lc,
                                          ctx->d_active, budget));
     if (budget > 0) {
         const int K = TB_VARIANTS[ctx->variant].k;
-        const int nl = (budget + K - 1) / K + 1;
+        const int nl = (budget + K - 1) / K;
         for (int L = 0; L < nl; L++)
-            LAUNCH("rb_tb", launch_blocked(ctx, ctx->variant, L, TB_CHECK));
+            LAUNCH("rb_tb", launch_blocked(ctx, ctx->variant, L, 0));
+        LAUNCH("learn_phase_end", launch_learn_phase_end(ctx->stream, nch(ctx), ctx->d_cs,
+                                                         ctx->d_lc, nl, budget));
         LAUNCH("pbuf_fix", launch_pbuf_fix(ctx->stream, ctx->ntiles, nch(ctx), ctx->d_cd,
                                            ctx->d_cs, ctx->d_tiles, ctx->p2, ctx->p));
     } else {
@@ -652,18 +656,23 @@ int any_running(stm_ctx* ctx, bool* any) {
     return check_launch(ctx);
 }
 
-// The network's correction of p: forward on the packed coarse residual,
-// prolongation, ghosts, step length, update.
-int learn_correct(stm_ctx* ctx, Net& net, int f) {
+// The network's correction of p: the running channels packed to the front
+// of the batch (the network skips the rest), forward on their coarse
+// residuals, prolongation, ghosts, step length, update.
+int learn_correct(stm_ctx* ctx, Net& net, int f, hipEvent_t ev0, hipEvent_t ev1) {
     const int n = nch(ctx);
+    LAUNCH("learn_compact", launch_learn_compact(ctx->stream, n, ctx->d_lc, ctx->d_map,
+                                                 ctx->d_inv, ctx->d_live));
     LAUNCH("learn_pack", launch_learn_pack(ctx->stream, n, ctx->d_lc, ctx->lrc, ctx->lmc, net.H,
-                                           net.W, net.input(), nullptr, 0, 1));
-    HIPCHK(ctx, hipEventRecord(ctx->nn_ev[0], ctx->stream));
-    LAUNCH("nn_forward", net.forward_infer(n));
-    HIPCHK(ctx, hipEventRecord(ctx->nn_ev[1], ctx->stream));
+                                           net.W, net.input(), nullptr, 0, 1, ctx->d_map,
+                                           ctx->d_live));
+    HIPCHK(ctx, hipEventRecord(ev0, ctx->stream));
+    LAUNCH("nn_forward", net.forward_infer(n, ctx->d_live));
+    HIPCHK(ctx, hipEventRecord(ev1, ctx->stream));
     LAUNCH("learn_prolong", launch_learn_prolong(ctx->stream, ctx->ntiles, ctx->d_cd,
                                                  ctx->d_tiles, ctx->d_lc, ctx->flags, net.y,
-                                                 ctx->lmc, net.H, net.W, f, ctx->lcf));
+                                                 ctx->lmc, net.H, net.W, f, ctx->d_inv,
+                                                 ctx->lcf));
     LAUNCH("learn_ghost", launch_ghost(ctx->stream, ctx->nbtot, ctx->d_be, ctx->d_cs, ctx->lcf,
                                        -1, 0));
     LAUNCH("learn_dots", launch_learn_dots(ctx->stream, ctx->ntiles, ctx->d_cd, ctx->d_tiles,
@@ -693,7 +702,7 @@ int phased_solve(stm_ctx* ctx) {
     const int phases = L ? ctx->lcycles : ctx->cpoints;
     const size_t bytes = sizeof(double) * (size_t)ctx->total;
     const int slot_base = C ? C->dwrite : 0;
-    int captured = 0, rc;
+    int captured = 0, corrected = 0, rc;
     for (int k = 0; k < phases; k++) {
         if ((rc = learn_prepare(ctx, k == 0))) return rc;
         const bool cap = C && captured < ctx->cpoints;
@@ -706,24 +715,32 @@ int phased_solve(stm_ctx* ctx) {
                                        hipMemcpyDeviceToDevice, ctx->stream));
             captured++;
         }
-        if (L && (rc = learn_correct(ctx, *L, f))) return rc;
+        if (L) {
+            while ((int)ctx->nn_evs.size() < 2 * (k + 1)) {
+                hipEvent_t e;
+                HIPCHK(ctx, hipEventCreate(&e));
+                ctx->nn_evs.push_back(e);
+            }
+            if ((rc = learn_correct(ctx, *L, f, ctx->nn_evs[2 * k], ctx->nn_evs[2 * k + 1])))
+                return rc;
+            corrected++;
+        }
         const int budget = L ? ctx->lsweeps : ctx->csweeps << k;
         if ((rc = learn_phase(ctx, budget, 0))) return rc;
-        if (L) {
-            float ms = 0.f;
-            HIPCHK(ctx, hipEventSynchronize(ctx->nn_ev[1]));
-            HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->nn_ev[0], ctx->nn_ev[1]));
-            L->fwd_ms += ms;
-            L->fwd_calls += 1;
-            L->fwd_samples += (uint64_t)n;
-        }
-        if (L && k % 4 == 3) {
+        if (L && k % 8 == 7) {
             bool any = false;
             if ((rc = any_running(ctx, &any))) return rc;
             if (!any) break;
         }
     }
     if ((rc = learn_phase(ctx, 0, 1))) return rc;
+    for (int j = 0; j < corrected; j++) {   // the final phase has synchronised
+        float ms = 0.f;
+        HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->nn_evs[2 * j], ctx->nn_evs[2 * j + 1]));
+        L->fwd_ms += ms;
+        L->fwd_calls += 1;
+        L->fwd_samples += (uint64_t)n;
+    }
     for (int j = 0; j < captured; j++)
         LAUNCH("learn_target", launch_learn_target(ctx->stream, ctx->ntiles, n, ctx->d_cd,
                                                    ctx->d_tiles, ctx->d_lc, ctx->flags, ctx->p,
@@ -748,8 +765,9 @@ int learn_buffers(stm_ctx* ctx, const Net& net, bool capture) {
         HIPCHK(ctx, hipMalloc(&ctx->lcf, bytes));
         HIPCHK(ctx, hipMemset(ctx->lres, 0, bytes));
         HIPCHK(ctx, hipMemset(ctx->lcf, 0, bytes));
-        HIPCHK(ctx, hipEventCreate(&ctx->nn_ev[0]));
-        HIPCHK(ctx, hipEventCreate(&ctx->nn_ev[1]));
+        HIPCHK(ctx, hipMalloc(&ctx->d_map, sizeof(int32_t) * nch(ctx)));
+        HIPCHK(ctx, hipMalloc(&ctx->d_inv, sizeof(int32_t) * nch(ctx)));
+        HIPCHK(ctx, hipMalloc(&ctx->d_live, sizeof(int32_t)));
     }
     const int64_t frame = (int64_t)net.H * net.W;
     if (frame > ctx->lframe) {
@@ -914,14 +932,14 @@ void stm_destroy(stm_ctx* ctx) {
     void* bufs[] = {ctx->u, ctx->v, ctx->p, ctx->f, ctx->g, ctx->rhs, ctx->scratch, ctx->p2,
                     ctx->flags, ctx->d_cd, ctx->d_cs, ctx->d_be, ctx->d_restore, ctx->d_tiles,
                     ctx->d_part, ctx->d_active, ctx->d_tbpart, ctx->d_tb, ctx->d_lc, ctx->lres,
-                    ctx->lcf, ctx->lsnap, ctx->lrc, ctx->lmc, ctx->d_cpart};
+                    ctx->lcf, ctx->lsnap, ctx->lrc, ctx->lmc, ctx->d_cpart, ctx->d_map,
+                    ctx->d_inv, ctx->d_live};
     for (void* b : bufs)
         if (b) hipFree(b);
     if (ctx->h_active) hipHostFree(ctx->h_active);
     if (ctx->h_cs) hipHostFree(ctx->h_cs);
     if (ctx->h_lc) hipHostFree(ctx->h_lc);
-    for (hipEvent_t e : ctx->nn_ev)
-        if (e) hipEventDestroy(e);
+    for (hipEvent_t e : ctx->nn_evs) hipEventDestroy(e);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
 }

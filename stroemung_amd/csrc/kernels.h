@@ -66,11 +66,15 @@ int learn_coarse_blocks(int H, int W);
 void launch_learn_restrict(hipStream_t, int nch, const ChanDesc*, LearnChan*,
                            const uint8_t* flags, const double* res, int H, int W, int f,
                            float* rc, float* mc, double2* cpart, int save);
+// running channels -> batch entries (map), channels -> entries (inv), count
+void launch_learn_compact(hipStream_t, int nch, const LearnChan*, int32_t* map, int32_t* inv,
+                          int32_t* live);
 void launch_learn_pack(hipStream_t, int nch, const LearnChan*, const float* rc, const float* mc,
-                       int H, int W, __bf16* a0, __bf16* dx, int slot0, int dcap);
+                       int H, int W, __bf16* a0, __bf16* dx, int slot0, int dcap,
+                       const int32_t* map = nullptr, const int32_t* live = nullptr);
 void launch_learn_prolong(hipStream_t, int ntiles, const ChanDesc*, const Tile*,
                           const LearnChan*, const uint8_t* flags, const float* y,
-                          const float* mc, int H, int W, int f, double* cf);
+                          const float* mc, int H, int W, int f, const int32_t* inv, double* cf);
 void launch_learn_dots(hipStream_t, int ntiles, const ChanDesc*, const Tile*,
                        const uint8_t* flags, const double* res, const double* cf, double* part);
 void launch_learn_alpha(hipStream_t, int nch, const ChanDesc*, LearnChan*, const double* part,
@@ -80,6 +84,8 @@ void launch_learn_apply(hipStream_t, int ntiles, const ChanDesc*, const Tile*, c
 void launch_learn_arm(hipStream_t, int nch, const ChanDesc*, ChanState*, const LearnChan*,
                       int32_t* active, int budget);
 void launch_learn_collect(hipStream_t, int nch, ChanState*, LearnChan*, int final_phase);
+void launch_learn_phase_end(hipStream_t, int nch, ChanState*, const LearnChan*, int launches,
+                            int budget);
 void launch_learn_target(hipStream_t, int ntiles, int nch, const ChanDesc*, const Tile*,
                          LearnChan*, const uint8_t* flags, const double* p, const double* p0,
                          double* part, int H, int W, int f, float* dt, int slot0, int dcap,

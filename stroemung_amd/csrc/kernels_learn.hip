@@ -218,19 +218,39 @@ k_learn_nscale(LearnChan* __restrict__ lc, const double2* __restrict__ cpart, in
     if (save >= 0) l.cscale[save] = l.scale;
 }
 
-// Network input (nch x H x W x 32 bf16: -rc * scale, fluid fraction, zeros)
-// and/or a compact 2-channel dataset sample in slot (slot0 + b) % dcap.
+// The running channels in order: map[s] = channel of batch entry s,
+// inv[c] = batch entry of channel c (-1 if not running), *live = count.
+__global__ void k_learn_compact(const LearnChan* __restrict__ lc, int nch, int32_t* __restrict__ map,
+                                int32_t* __restrict__ inv, int32_t* __restrict__ live)
+{
+    if (threadIdx.x != 0) return;
+    int s = 0;
+    for (int c = 0; c < nch; c++) {
+        inv[c] = lc[c].run ? s : -1;
+        if (lc[c].run) map[s++] = c;
+    }
+    *live = s;
+}
+
+// Network input (H x W x 32 bf16 per batch entry: -rc * scale, fluid
+// fraction, zeros) and/or a compact 2-channel dataset sample in slot
+// (slot0 + b) % dcap.  With a map, batch entry s (s < *live) holds channel
+// map[s] and the dataset is not written; without, entry b holds channel b.
 __global__ void __launch_bounds__(256)
 k_learn_pack(const LearnChan* __restrict__ lc, const float* __restrict__ rc,
              const float* __restrict__ mc, int nch, int H, int W, __bf16* __restrict__ a0,
-             __bf16* __restrict__ dx, int slot0, int dcap)
+             __bf16* __restrict__ dx, int slot0, int dcap, const int32_t* __restrict__ map,
+             const int32_t* __restrict__ live)
 {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t hw = (int64_t)H * W;
     if (k >= hw * nch) return;
     const int b = (int)(k / hw);
     const int64_t q = k - (int64_t)b * hw;
-    const float v0 = (float)(-(double)rc[k] * lc[b].scale), v1 = mc[k];
+    if (map && b >= *live) return;
+    const int c = map ? map[b] : b;
+    const int64_t src = (int64_t)c * hw + q;
+    const float v0 = (float)(-(double)rc[src] * lc[c].scale), v1 = mc[src];
     if (a0) {
         typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
         bf16x8 z;
@@ -244,7 +264,7 @@ k_learn_pack(const LearnChan* __restrict__ lc, const float* __restrict__ rc,
         o[2] = z;
         o[3] = z;
     }
-    if (dx) {
+    if (dx && !map) {
         const int64_t s = (slot0 + b) % dcap;
         dx[(s * hw + q) * 2] = (__bf16)v0;
         dx[(s * hw + q) * 2 + 1] = (__bf16)v1;
@@ -259,7 +279,7 @@ __global__ void __launch_bounds__(TILE_THREADS)
 k_learn_prolong(const ChanDesc* __restrict__ cd, const Tile* __restrict__ tiles,
                 const LearnChan* __restrict__ lc, const uint8_t* __restrict__ flags,
                 const float* __restrict__ y, const float* __restrict__ mc, int H, int W, int f,
-                double* __restrict__ cf)
+                const int32_t* __restrict__ inv, double* __restrict__ cf)
 {
     const Tile t = tiles[blockIdx.x];
     const ChanDesc& d = cd[t.chan];
@@ -274,14 +294,17 @@ k_learn_prolong(const ChanDesc* __restrict__ cd, const Tile* __restrict__ tiles,
         const int i0 = (int)xi, j0 = (int)eta;
         const int i1 = i0 + 1 < hc_n ? i0 + 1 : i0, j1 = j0 + 1 < wc_n ? j0 + 1 : j0;
         const double tx = xi - i0, ty = eta - j0;
-        const int64_t base = (int64_t)t.chan * H * W;
-        const int64_t k00 = base + (int64_t)i0 * W + j0, k01 = base + (int64_t)i0 * W + j1;
-        const int64_t k10 = base + (int64_t)i1 * W + j0, k11 = base + (int64_t)i1 * W + j1;
-        const double w00 = (1.0 - tx) * (1.0 - ty) * mc[k00], w01 = (1.0 - tx) * ty * mc[k01];
-        const double w10 = tx * (1.0 - ty) * mc[k10], w11 = tx * ty * mc[k11];
+        const int64_t base = (int64_t)t.chan * H * W;     // channel's frame in mc
+        const int64_t ybase = (int64_t)(inv ? inv[t.chan] : t.chan) * H * W;  // its batch entry
+        const int64_t k00 = (int64_t)i0 * W + j0, k01 = (int64_t)i0 * W + j1;
+        const int64_t k10 = (int64_t)i1 * W + j0, k11 = (int64_t)i1 * W + j1;
+        const double w00 = (1.0 - tx) * (1.0 - ty) * mc[base + k00];
+        const double w01 = (1.0 - tx) * ty * mc[base + k01];
+        const double w10 = tx * (1.0 - ty) * mc[base + k10], w11 = tx * ty * mc[base + k11];
         const double ws = (w00 + w01) + (w10 + w11);
         if (ws > 0.0)
-            v = ((w00 * y[k00] + w01 * y[k01]) + (w10 * y[k10] + w11 * y[k11])) / ws;
+            v = ((w00 * y[ybase + k00] + w01 * y[ybase + k01]) +
+                 (w10 * y[ybase + k10] + w11 * y[ybase + k11])) / ws;
     }
     cf[i] = v;
 }
@@ -391,6 +414,23 @@ __global__ void k_learn_collect(ChanState* __restrict__ cs, LearnChan* __restric
     }
 }
 
+// End of a budgeted phase run as `launches` unchecked blocked launches: the
+// running channels swept `budget` times and their latest p is in the second
+// buffer when `launches` is odd.  The next cycle's residual decides whether
+// they have converged.
+__global__ void k_learn_phase_end(ChanState* __restrict__ cs, const LearnChan* __restrict__ lc,
+                                  int nch, int launches, int budget)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nch || !lc[c].run) return;
+    ChanState& s = cs[c];
+    s.pbuf = launches & 1;
+    s.sor_iters = (uint32_t)budget;
+    s.exhausted = 1;
+    s.active = 0;
+    s.stop_launch = launches - 1;
+}
+
 // Capture: slot 5 = sum of p - p0 over interior fluid cells.
 __global__ void __launch_bounds__(TILE_THREADS)
 k_learn_delta_sum(const ChanDesc* __restrict__ cd, const Tile* __restrict__ tiles,
@@ -465,16 +505,20 @@ void launch_learn_restrict(hipStream_t st, int nch, const ChanDesc* cd, LearnCha
     k_learn_restrict<<<dim3(nblk, nch), 256, 0, st>>>(cd, lc, flags, res, H, W, f, rc, mc, cpart);
     k_learn_nscale<<<nch, 256, 0, st>>>(lc, cpart, nblk, save);
 }
+void launch_learn_compact(hipStream_t st, int nch, const LearnChan* lc, int32_t* map, int32_t* inv,
+                          int32_t* live) {
+    k_learn_compact<<<1, 64, 0, st>>>(lc, nch, map, inv, live);
+}
 void launch_learn_pack(hipStream_t st, int nch, const LearnChan* lc, const float* rc,
                        const float* mc, int H, int W, __bf16* a0, __bf16* dx, int slot0,
-                       int dcap) {
+                       int dcap, const int32_t* map, const int32_t* live) {
     k_learn_pack<<<nb256((int64_t)H * W * nch), 256, 0, st>>>(lc, rc, mc, nch, H, W, a0, dx,
-                                                              slot0, dcap);
+                                                              slot0, dcap, map, live);
 }
 void launch_learn_prolong(hipStream_t st, int ntiles, const ChanDesc* cd, const Tile* tiles,
                           const LearnChan* lc, const uint8_t* flags, const float* y,
-                          const float* mc, int H, int W, int f, double* cf) {
-    k_learn_prolong<<<ntiles, TB, 0, st>>>(cd, tiles, lc, flags, y, mc, H, W, f, cf);
+                          const float* mc, int H, int W, int f, const int32_t* inv, double* cf) {
+    k_learn_prolong<<<ntiles, TB, 0, st>>>(cd, tiles, lc, flags, y, mc, H, W, f, inv, cf);
 }
 void launch_learn_dots(hipStream_t st, int ntiles, const ChanDesc* cd, const Tile* tiles,
                        const uint8_t* flags, const double* res, const double* cf, double* part) {
@@ -491,6 +535,10 @@ void launch_learn_apply(hipStream_t st, int ntiles, const ChanDesc* cd, const Ti
 void launch_learn_arm(hipStream_t st, int nch, const ChanDesc* cd, ChanState* cs,
                       const LearnChan* lc, int32_t* active, int budget) {
     k_learn_arm<<<nb256(nch), 256, 0, st>>>(cd, cs, lc, active, nch, budget);
+}
+void launch_learn_phase_end(hipStream_t st, int nch, ChanState* cs, const LearnChan* lc,
+                            int launches, int budget) {
+    k_learn_phase_end<<<nb256(nch), 256, 0, st>>>(cs, lc, nch, launches, budget);
 }
 void launch_learn_collect(hipStream_t st, int nch, ChanState* cs, LearnChan* lc, int final_phase) {
     k_learn_collect<<<nb256(nch), 256, 0, st>>>(cs, lc, nch, final_phase);

@@ -17,13 +17,15 @@ struct ConvIn {
     int ca, mode;
     const bf16* b;
     int cb;
+    const int* live = nullptr;   // device count: images at or beyond it are skipped
 };
 
 // 3x3 convolution, padding 1, NHWC bf16, fp32 accumulation on the matrix
 // cores.  w is [cout][9][cin] bf16; flags 1 = add bias (fp32), 2 = ReLU.
 // Writes bf16 `out` and/or fp32 `out32` (either may be null).
+// live (device int, may be null): only images [0, *live) are computed
 void conv3x3(hipStream_t, const bf16* in, int cin, const bf16* w, const float* bias, bf16* out,
-             float* out32, int cout, int nb, int H, int W, int flags);
+             float* out32, int cout, int nb, int H, int W, int flags, const int* live = nullptr);
 // The staged-tile convolution takes every image whose sides are multiples
 // of 8; conv3x3_in is that convolution with a described input.
 bool conv_tiles_fit(int H, int W);
@@ -36,18 +38,20 @@ void conv3x3_dw(hipStream_t, const bf16* gz, int cout, const bf16* in, int cin, 
                 int H, int W);
 void relu_bwd(hipStream_t, int64_t n, const bf16* y, bf16* g);
 void bias_grad(hipStream_t, const bf16* g, int c, int64_t npix, float* db);
-void maxpool(hipStream_t, const bf16* x, bf16* y, int nb, int H, int W, int c);
+void maxpool(hipStream_t, const bf16* x, bf16* y, int nb, int H, int W, int c,
+             const int* live = nullptr);
 // accumulates into gx
 void maxpool_bwd(hipStream_t, const bf16* x, const bf16* y, const bf16* gy, bf16* gx, int nb,
                  int H, int W, int c);
 // x (nb, H/2, W/2, c) -> channels [c0, c0 + c) of y (nb, H, W, cy)
-void upsample(hipStream_t, const bf16* x, bf16* y, int nb, int H, int W, int c, int cy, int c0);
+void upsample(hipStream_t, const bf16* x, bf16* y, int nb, int H, int W, int c, int cy, int c0,
+              const int* live = nullptr);
 void upsample_bwd(hipStream_t, const bf16* gy, bf16* gx, int nb, int H, int W, int c, int cy,
                   int c0);
 void channel_copy(hipStream_t, const bf16* src, int cs, int s0, bf16* dst, int cd, int d0, int c,
-                  int64_t npix, int add);
+                  int64_t npix, int add, const int* live = nullptr, int64_t hw = 1);
 void head(hipStream_t, const bf16* x, int c, const float* w, const float* b, float* y,
-          int64_t npix);
+          int64_t npix, const int* live = nullptr, int64_t hw = 1);
 void head_bwd(hipStream_t, const bf16* x, int c, const float* w, const float* gy, bf16* gx,
               float* dw, float* db, int64_t npix);
 void loss(hipStream_t, const float* y, const float* t, const float* m, float* denom, float* gy,

@@ -38,12 +38,13 @@ template <int BM>
 __global__ void __launch_bounds__(256)
 k_conv3x3(const bf16* __restrict__ in, int cin, const bf16* __restrict__ w,
           const float* __restrict__ bias, bf16* __restrict__ out, float* __restrict__ out32,
-          int cout, int nb, int H, int W, int flags)
+          int cout, int nb, int H, int W, int flags, const int* __restrict__ live)
 {
     constexpr int MT = BM / 32;          // M tiles per wave
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave >> 1, wn = wave & 1;
-    const int npix = nb * H * W;
+    const int npix = (live ? min(nb, *live) : nb) * H * W;
+    if ((int)(blockIdx.x * BM) >= npix) return;   // the whole block is beyond the live images
     const int m_base = blockIdx.x * BM + wm * (BM / 2);
     const int n_base = blockIdx.y * 64 + wn * 32;
     if (n_base >= cout) return;          // 32-channel layers use half the tile (no barriers here)
@@ -146,6 +147,7 @@ k_conv3x3_tile(ConvIn src, const bf16* __restrict__ w, const float* __restrict__
     t /= tiles_w;
     const int th0 = (t % tiles_h) * TH;
     const int b = t / tiles_h;
+    if (src.live && b >= *src.live) return;      // uniform over the block, before any barrier
     const int n_base = blockIdx.y * (32 * WN) + wn * 32;
     const int row = lane & 15, kq = (lane >> 4) * 8;
 
@@ -349,11 +351,12 @@ k_bias_grad(const bf16* __restrict__ g, int c, int npix, int per, float* __restr
 }
 
 // 2x2 max pooling, H and W even
-__global__ void k_maxpool(const bf16* __restrict__ x, bf16* __restrict__ y, int nb, int H, int W, int c)
+__global__ void k_maxpool(const bf16* __restrict__ x, bf16* __restrict__ y, int nb, int H, int W, int c,
+                          const int* __restrict__ live)
 {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int Ho = H / 2, Wo = W / 2;
-    const int64_t n = (int64_t)nb * Ho * Wo * c;
+    const int64_t n = (int64_t)(live ? min(nb, *live) : nb) * Ho * Wo * c;
     if (k >= n) return;
     const int ch = (int)(k % c);
     const int64_t q = k / c;
@@ -392,10 +395,10 @@ __global__ void k_maxpool_bwd(const bf16* __restrict__ x, const bf16* __restrict
 // nearest 2x upsampling of x (nb, H/2, W/2, c) into channels [c0, c0 + c) of
 // y (nb, H, W, cy) -- the concatenation with the skip happens in place
 __global__ void k_upsample(const bf16* __restrict__ x, bf16* __restrict__ y, int nb, int H, int W,
-                           int c, int cy, int c0)
+                           int c, int cy, int c0, const int* __restrict__ live)
 {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t n = (int64_t)nb * H * W * c;
+    const int64_t n = (int64_t)(live ? min(nb, *live) : nb) * H * W * c;
     if (k >= n) return;
     const int ch = (int)(k % c);
     const int64_t q = k / c;
@@ -423,10 +426,12 @@ __global__ void k_upsample_bwd(const bf16* __restrict__ gy, bf16* __restrict__ g
 // copy channels: y[:, c0:c0+c] = x (forward of the skip half of a concat) or
 // x = y[:, c0:c0+c] (its backward), add = accumulate into x
 __global__ void k_channel_copy(const bf16* __restrict__ src, int cs, int s0, bf16* __restrict__ dst,
-                               int cd, int d0, int c, int64_t npix, int add)
+                               int cd, int d0, int c, int64_t npix, int add,
+                               const int* __restrict__ live, int64_t hw)
 {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= npix * c) return;
+    if (live && k / c >= (int64_t)*live * hw) return;
     const int ch = (int)(k % c);
     const int64_t p = k / c;
     const bf16 v = src[p * cs + s0 + ch];
@@ -436,10 +441,12 @@ __global__ void k_channel_copy(const bf16* __restrict__ src, int cs, int s0, bf1
 
 // 1x1 head: y[pix] = b + sum_c x[pix, c] * w[c]   (fp32 out)
 __global__ void k_head(const bf16* __restrict__ x, int c, const float* __restrict__ w,
-                       const float* __restrict__ b, float* __restrict__ y, int64_t npix)
+                       const float* __restrict__ b, float* __restrict__ y, int64_t npix,
+                       const int* __restrict__ live, int64_t hw)
 {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= npix) return;
+    if (live && p >= (int64_t)*live * hw) return;
     float s = b[0];
     for (int i = 0; i < c; i++) s += (float)x[p * c + i] * w[i];
     y[p] = s;
@@ -573,19 +580,21 @@ void conv3x3_in(hipStream_t st, const ConvIn& src, const bf16* w, const float* b
 }
 
 void conv3x3(hipStream_t st, const bf16* in, int cin, const bf16* w, const float* bias, bf16* out,
-             float* out32, int cout, int nb, int H, int W, int flags) {
+             float* out32, int cout, int nb, int H, int W, int flags, const int* live) {
     const int64_t npix = (int64_t)nb * H * W;
     if (conv_tiles_fit(H, W)) {
-        const ConvIn src{in, cin, IN_DIRECT, nullptr, 0};
+        const ConvIn src{in, cin, IN_DIRECT, nullptr, 0, live};
         conv3x3_in(st, src, w, bias, out, out32, cout, nb, H, W, flags);
         return;
     }
     if (npix >= 64 * 256) {
         dim3 grid(nblk(npix, 128), (cout + 63) / 64);
-        k_conv3x3<128><<<grid, 256, 0, st>>>(in, cin, w, bias, out, out32, cout, nb, H, W, flags);
+        k_conv3x3<128><<<grid, 256, 0, st>>>(in, cin, w, bias, out, out32, cout, nb, H, W, flags,
+                                             live);
     } else {
         dim3 grid(nblk(npix, 64), (cout + 63) / 64);
-        k_conv3x3<64><<<grid, 256, 0, st>>>(in, cin, w, bias, out, out32, cout, nb, H, W, flags);
+        k_conv3x3<64><<<grid, 256, 0, st>>>(in, cin, w, bias, out, out32, cout, nb, H, W, flags,
+                                            live);
     }
 }
 void weight_flip(hipStream_t st, const bf16* w, bf16* wt, int cin, int cout) {
@@ -610,27 +619,29 @@ void bias_grad(hipStream_t st, const bf16* g, int c, int64_t npix, float* db) {
     const int per = 1024;
     k_bias_grad<<<nblk(npix, per), 256, 0, st>>>(g, c, (int)npix, per, db);
 }
-void maxpool(hipStream_t st, const bf16* x, bf16* y, int nb, int H, int W, int c) {
-    k_maxpool<<<nblk((int64_t)nb * H * W * c / 4), 256, 0, st>>>(x, y, nb, H, W, c);
+void maxpool(hipStream_t st, const bf16* x, bf16* y, int nb, int H, int W, int c, const int* live) {
+    k_maxpool<<<nblk((int64_t)nb * H * W * c / 4), 256, 0, st>>>(x, y, nb, H, W, c, live);
 }
 void maxpool_bwd(hipStream_t st, const bf16* x, const bf16* y, const bf16* gy, bf16* gx, int nb,
                  int H, int W, int c) {
     k_maxpool_bwd<<<nblk((int64_t)nb * H * W * c / 4), 256, 0, st>>>(x, y, gy, gx, nb, H, W, c);
 }
-void upsample(hipStream_t st, const bf16* x, bf16* y, int nb, int H, int W, int c, int cy, int c0) {
-    k_upsample<<<nblk((int64_t)nb * H * W * c), 256, 0, st>>>(x, y, nb, H, W, c, cy, c0);
+void upsample(hipStream_t st, const bf16* x, bf16* y, int nb, int H, int W, int c, int cy, int c0,
+              const int* live) {
+    k_upsample<<<nblk((int64_t)nb * H * W * c), 256, 0, st>>>(x, y, nb, H, W, c, cy, c0, live);
 }
 void upsample_bwd(hipStream_t st, const bf16* gy, bf16* gx, int nb, int H, int W, int c, int cy,
                   int c0) {
     k_upsample_bwd<<<nblk((int64_t)nb * H * W * c / 4), 256, 0, st>>>(gy, gx, nb, H, W, c, cy, c0);
 }
 void channel_copy(hipStream_t st, const bf16* src, int cs, int s0, bf16* dst, int cd, int d0, int c,
-                  int64_t npix, int add) {
-    k_channel_copy<<<nblk(npix * c), 256, 0, st>>>(src, cs, s0, dst, cd, d0, c, npix, add);
+                  int64_t npix, int add, const int* live, int64_t hw) {
+    k_channel_copy<<<nblk(npix * c), 256, 0, st>>>(src, cs, s0, dst, cd, d0, c, npix, add, live,
+                                                   hw);
 }
 void head(hipStream_t st, const bf16* x, int c, const float* w, const float* b, float* y,
-          int64_t npix) {
-    k_head<<<nblk(npix), 256, 0, st>>>(x, c, w, b, y, npix);
+          int64_t npix, const int* live, int64_t hw) {
+    k_head<<<nblk(npix), 256, 0, st>>>(x, c, w, b, y, npix, live, hw);
 }
 void head_bwd(hipStream_t st, const bf16* x, int c, const float* w, const float* gy, bf16* gx,
               float* dw, float* db, int64_t npix) {

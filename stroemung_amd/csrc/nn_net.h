@@ -73,7 +73,9 @@ struct Net {
     int init(stm_ctx* c, int H, int W, int cap, uint64_t seed);
     void release();
     void forward(int nb);          // materialises every activation (training)
-    void forward_infer(int nb);    // pooling / upsampling / concatenation fused into staging
+    // pooling / upsampling / concatenation fused into staging; live (device
+    // int, may be null) limits the work to batch entries [0, *live)
+    void forward_infer(int nb, const int* live = nullptr);
     void backward(int nb);
     int train_step(int nb, float lr);
 };

@@ -184,19 +184,21 @@ void Net::forward(int nb) {
 // Inference: the same network with pooling, upsampling and concatenation
 // done while each convolution stages its input, on every level whose sides
 // are multiples of 8; other levels materialise those inputs as in forward().
-void Net::forward_infer(int nb) {
+void Net::forward_infer(int nb, const int* live) {
     using nn::ConvIn;
     using nn::conv_tiles_fit;
     auto npix = [&](int level) { return (int64_t)nb * h(level) * w(level); };
     auto run = [&](int k, const ConvIn& in) {
         const Conv& c = convs[k];
-        nn::conv3x3_in(stream, in, pb + c.woff, p32 + c.boff, acts[c.y].v, nullptr, c.cout, nb,
+        ConvIn src = in;
+        src.live = live;
+        nn::conv3x3_in(stream, src, pb + c.woff, p32 + c.boff, acts[c.y].v, nullptr, c.cout, nb,
                    h(c.level), w(c.level), 3);
     };
     auto plain = [&](int k) {
         const Conv& c = convs[k];
         nn::conv3x3(stream, acts[c.x].v, c.cin, pb + c.woff, p32 + c.boff, acts[c.y].v, nullptr,
-                    c.cout, nb, h(c.level), w(c.level), 3);
+                    c.cout, nb, h(c.level), w(c.level), 3, live);
     };
     // conv k whose input is the pooled activation `src` (level - 1)
     auto pooled = [&](int k, int src, int pooled_act) {
@@ -205,7 +207,7 @@ void Net::forward_infer(int nb) {
             run(k, ConvIn{acts[src].v, acts[src].c, nn::IN_POOL, nullptr, 0});
         } else {
             nn::maxpool(stream, acts[src].v, acts[pooled_act].v, nb, h(c.level - 1),
-                        w(c.level - 1), acts[src].c);
+                        w(c.level - 1), acts[src].c, live);
             plain(k);
         }
     };
@@ -217,9 +219,9 @@ void Net::forward_infer(int nb) {
             run(k, ConvIn{acts[low].v, cl, nn::IN_UP, acts[skip].v, cs});
         } else {
             nn::upsample(stream, acts[low].v, acts[cat].v, nb, h(c.level), w(c.level), cl, cl + cs,
-                         0);
+                         0, live);
             nn::channel_copy(stream, acts[skip].v, cs, 0, acts[cat].v, cl + cs, cl, cs,
-                             npix(c.level), 0);
+                             npix(c.level), 0, live, (int64_t)h(c.level) * w(c.level));
             plain(k);
         }
     };
@@ -237,7 +239,8 @@ void Net::forward_infer(int nb) {
     plain(11);
     joined(12, 17, 2, 18);
     plain(13);
-    nn::head(stream, acts[20].v, 32, p32 + head_w, p32 + head_b, y, npix(0));
+    nn::head(stream, acts[20].v, 32, p32 + head_w, p32 + head_b, y, npix(0), live,
+             (int64_t)h(0) * w(0));
 }
 
 void Net::backward(int nb) {

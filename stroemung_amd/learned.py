@@ -262,10 +262,13 @@ def compare(grids, params, coarsen=4, cycles=16, sweeps=32, warm=20, ticks=10, l
     ctx = make_context(grids)
     for _ in range(warm):
         ctx.tick()
+    ctx.autotune(8)
     el, sw, ok = _run_ticks(ctx, ticks, cells)
+    variant = ctx.solver_variant()
     ctx.close()
     out["plain"] = {"cell_updates_per_s": sum(cells) * ticks / el, "ms_per_tick": el * 1e3 / ticks,
-                    "sweeps_per_solve": sw / ticks / len(grids), "residual_bound_met": ok}
+                    "sweeps_per_solve": sw / ticks / len(grids), "residual_bound_met": ok,
+                    "variant": {"id": variant[0], "sweeps_per_launch": variant[1]}}
     log("plain: %.2f ms/tick, %.1f sweeps/solve" % (el * 1e3 / ticks, sw / ticks / len(grids)))
     ctx = make_context(grids)
     net = PressureNet(ctx, H, W, len(grids), seed=1)
@@ -273,6 +276,7 @@ def compare(grids, params, coarsen=4, cycles=16, sweeps=32, warm=20, ticks=10, l
         net.set_params(params)
         for _ in range(warm):
             ctx.tick()
+        ctx.autotune(8)
         ctx.set_learned(net, cycles=cycles, sweeps=sweeps, step="energy", coarsen=coarsen)
         el, sw, ok = _run_ticks(ctx, ticks, cells)
         st = net.stats()

@@ -615,8 +615,8 @@ void relu_bwd(hipStream_t st, int64_t n, const bf16* y, bf16* g) {
     k_relu_bwd<<<nblk(n), 256, 0, st>>>(n, y, g);
 }
 void bias_grad(hipStream_t st, const bf16* g, int c, int64_t npix, float* db) {
-    // bias channels are 32, 64, 128 or 256
-    const int per = 1024;
+    // bias channels are 32, 64, 128 or 256; short chunks keep many blocks in flight
+    const int per = 128;
     k_bias_grad<<<nblk(npix, per), 256, 0, st>>>(g, c, (int)npix, per, db);
 }
 void maxpool(hipStream_t st, const bf16* x, bf16* y, int nb, int H, int W, int c, const int* live) {

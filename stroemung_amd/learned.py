@@ -182,6 +182,8 @@ def train_net(grids, coarsen=4, capture_ticks=96, points=4, capture_sweeps=32, s
     t0 = time.perf_counter()
     try:
         net.reserve(points * n * (capture_ticks + dagger_rounds * dagger_ticks))
+        ctx.tick()
+        ctx.autotune(8)
         ctx.set_capture(net, points=points, sweeps=capture_sweeps, coarsen=coarsen)
         plain_sweeps = []
         for i in range(capture_ticks):

@@ -270,6 +270,12 @@ int stm_net_benchmark(stm_net *net, uint32_t nb, uint32_t reps, double *fwd_ms,
 #define STM_STEP_NETWORK 1
 int stm_set_learned(stm_ctx *ctx, stm_net *net, uint32_t cycles, uint32_t sweeps, int32_t step,
                     uint32_t coarsen);
+/* on != 0: learned cycles after the first are captured once into a hipGraph
+ * (per net, coarsening, sweeps, step rule and solver variant) and replayed;
+ * results are bit-identical to launching them one by one.  Not used while
+ * capturing training pairs or profiling (stm_profile > 0). */
+int stm_set_graphs(stm_ctx *ctx, int32_t on);
+
 /* With a net set, every red-black solve also stores `points` (1..8)
  * training pairs per channel in the net's training set (ring order): the
  * residual, restricted as for stm_set_learned, at `points` moments of the

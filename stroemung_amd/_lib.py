@@ -123,6 +123,7 @@ SIGNATURES = {
     "stm_net_benchmark": (C.c_int, [_NET, C.c_uint32, C.c_uint32, _D, _D]),
     "stm_set_learned": (C.c_int, [_CTX, _NET, C.c_uint32, C.c_uint32, C.c_int32, C.c_uint32]),
     "stm_set_capture": (C.c_int, [_CTX, _NET, C.c_uint32, C.c_uint32, C.c_uint32]),
+    "stm_set_graphs": (C.c_int, [_CTX, C.c_int32]),
 }
 
 _lib = None

@@ -95,6 +95,10 @@ struct stm_ctx {
     int32_t *d_map = nullptr, *d_inv = nullptr, *d_live = nullptr;   // running channels
     int64_t lframe = 0;            // pixels per frame the coarse buffers hold
     std::vector<hipEvent_t> nn_evs;  // per-cycle forward timing, read after the solve
+    // learned cycles replayed from a captured hipGraph
+    bool use_graphs = false;
+    hipGraphExec_t cycle_exec = nullptr;
+    std::vector<int64_t> cycle_key;  // what the captured cycle depends on
     // profiling
     uint32_t sample = 0;
     std::unordered_map<std::string, KStat> stats;
@@ -666,9 +670,9 @@ int learn_correct(stm_ctx* ctx, Net& net, int f, hipEvent_t ev0, hipEvent_t ev1)
     LAUNCH("learn_pack", launch_learn_pack(ctx->stream, n, ctx->d_lc, ctx->lrc, ctx->lmc, net.H,
                                            net.W, net.input(), nullptr, 0, 1, ctx->d_map,
                                            ctx->d_live));
-    HIPCHK(ctx, hipEventRecord(ev0, ctx->stream));
+    if (ev0) HIPCHK(ctx, hipEventRecord(ev0, ctx->stream));
     LAUNCH("nn_forward", net.forward_infer(n, ctx->d_live));
-    HIPCHK(ctx, hipEventRecord(ev1, ctx->stream));
+    if (ev1) HIPCHK(ctx, hipEventRecord(ev1, ctx->stream));
     LAUNCH("learn_prolong", launch_learn_prolong(ctx->stream, ctx->ntiles, ctx->d_cd,
                                                  ctx->d_tiles, ctx->d_lc, ctx->flags, net.y,
                                                  ctx->lmc, net.H, net.W, f, ctx->d_inv,
@@ -681,6 +685,45 @@ int learn_correct(stm_ctx* ctx, Net& net, int f, hipEvent_t ev0, hipEvent_t ev1)
                                              ctx->lstep, f));
     LAUNCH("learn_apply", launch_learn_apply(ctx->stream, ctx->ntiles, ctx->d_cd, ctx->d_tiles,
                                              ctx->d_lc, ctx->flags, ctx->lcf, ctx->p));
+    return STM_OK;
+}
+
+// One learned cycle after the first: residual, convergence test, network
+// correction, budgeted sweeps.  Every launch depends only on device pointers
+// and on the settings in `key`, so with graphs on the sequence is captured
+// once into a hipGraph and replayed.
+int learned_cycle(stm_ctx* ctx, Net& net, int f) {
+    int rc;
+    if ((rc = learn_prepare(ctx, 0))) return rc;
+    if ((rc = learn_restrict(ctx, net, f, -1))) return rc;
+    if ((rc = learn_correct(ctx, net, f, nullptr, nullptr))) return rc;
+    return learn_phase(ctx, ctx->lsweeps, 0);
+}
+
+int graph_cycle(stm_ctx* ctx, Net& net, int f) {
+    const std::vector<int64_t> key = {(int64_t)(intptr_t)&net, f, ctx->lsweeps, ctx->lstep,
+                                      ctx->variant, net.H, net.W, nch(ctx)};
+    if (!ctx->cycle_exec || key != ctx->cycle_key) {
+        if (ctx->cycle_exec) HIPCHK(ctx, hipGraphExecDestroy(ctx->cycle_exec));
+        ctx->cycle_exec = nullptr;
+        hipGraph_t g = nullptr;
+        HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = learned_cycle(ctx, net, f);
+        const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+        if (rc) {
+            if (g) hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(ctx, STM_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
+        const hipError_t ei = hipGraphInstantiateWithFlags(&ctx->cycle_exec, g, 0);
+        hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+            ctx->cycle_exec = nullptr;
+            return fail(ctx, STM_ERR_HIP, "graph instantiate: %s", hipGetErrorString(ei));
+        }
+        ctx->cycle_key = key;
+    }
+    HIPCHK(ctx, hipGraphLaunch(ctx->cycle_exec, ctx->stream));
     return STM_OK;
 }
 
@@ -703,7 +746,18 @@ int phased_solve(stm_ctx* ctx) {
     const size_t bytes = sizeof(double) * (size_t)ctx->total;
     const int slot_base = C ? C->dwrite : 0;
     int captured = 0, corrected = 0, rc;
+    // graphs replay cycles 1.. of a learned solve that captures nothing and is not profiled
+    const bool graphs = L && !C && ctx->use_graphs && ctx->sample == 0;
     for (int k = 0; k < phases; k++) {
+        if (graphs && k > 0) {
+            if ((rc = graph_cycle(ctx, *L, f))) return rc;
+            if (k % 8 == 7) {
+                bool any = false;
+                if ((rc = any_running(ctx, &any))) return rc;
+                if (!any) break;
+            }
+            continue;
+        }
         if ((rc = learn_prepare(ctx, k == 0))) return rc;
         const bool cap = C && captured < ctx->cpoints;
         if ((rc = learn_restrict(ctx, L ? *L : *C, f, cap ? captured : -1))) return rc;
@@ -940,6 +994,7 @@ void stm_destroy(stm_ctx* ctx) {
     if (ctx->h_cs) hipHostFree(ctx->h_cs);
     if (ctx->h_lc) hipHostFree(ctx->h_lc);
     for (hipEvent_t e : ctx->nn_evs) hipEventDestroy(e);
+    if (ctx->cycle_exec) hipGraphExecDestroy(ctx->cycle_exec);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -1384,6 +1439,13 @@ int stm_set_learned(stm_ctx* ctx, stm_net* net, uint32_t cycles, uint32_t sweeps
     ctx->lnet = net;
     ctx->lcycles = (int)cycles;
     ctx->lsweeps = (int)sweeps;
+    return STM_OK;
+}
+
+int stm_set_graphs(stm_ctx* ctx, int32_t on) {
+    int rc = ready(ctx);
+    if (rc) return rc;
+    ctx->use_graphs = on != 0;
     return STM_OK;
 }
 

@@ -252,7 +252,8 @@ def _run_ticks(ctx, ticks, cells):
     return el, sweeps, bound
 
 
-def compare(grids, params, coarsen=4, cycles=16, sweeps=32, warm=20, ticks=10, log=_quiet):
+def compare(grids, params, coarsen=4, cycles=16, sweeps=32, warm=20, ticks=10, graphs=True,
+            log=_quiet):
     """Plain red-black solve vs the learned solve on `grids`, each from the
     same state after `warm` plain ticks (two contexts, same deterministic
     warm-up).  Every channel of both runs ends each solve at its residual
@@ -280,6 +281,7 @@ def compare(grids, params, coarsen=4, cycles=16, sweeps=32, warm=20, ticks=10, l
             ctx.tick()
         ctx.autotune(8)
         ctx.set_learned(net, cycles=cycles, sweeps=sweeps, step="energy", coarsen=coarsen)
+        ctx.set_graphs(graphs)
         el, sw, ok = _run_ticks(ctx, ticks, cells)
         st = net.stats()
         ctx.set_learned(None)
@@ -291,7 +293,9 @@ def compare(grids, params, coarsen=4, cycles=16, sweeps=32, warm=20, ticks=10, l
         "cell_updates_per_s": sum(cells) * ticks / el, "ms_per_tick": el * 1e3 / ticks,
         "sweeps_per_solve": sw / ticks / len(grids), "residual_bound_met": ok,
         "cycles": cycles, "sweeps_per_cycle": sweeps, "coarsen": coarsen, "frame": [H, W],
-        "forward_ms_per_tick": st["forward_ms"] / ticks,
+        "graphs": graphs,
+        # forward passes timed with events (with graphs: the first cycle of each solve)
+        "forward_ms_per_call": st["forward_ms"] / st["forward_calls"] if st["forward_calls"] else None,
         "forward_tflops": fwd_flops / (st["forward_ms"] * 1e-3) / 1e12 if st["forward_ms"] else None,
     }
     out["speedup"] = out["learned"]["cell_updates_per_s"] / out["plain"]["cell_updates_per_s"]

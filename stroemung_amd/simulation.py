@@ -190,6 +190,10 @@ class Context:
         self._check(lib().stm_set_learned(self._ctx, net._net if net else None, cycles, sweeps,
                                           rule, coarsen))
 
+    def set_graphs(self, on=True):
+        """Replay learned cycles from a captured hipGraph."""
+        self._check(lib().stm_set_graphs(self._ctx, 1 if on else 0))
+
     def set_capture(self, net, points=2, sweeps=32, coarsen=1):
         """Store `points` training pairs per channel from every solve in net's
         training set (None: off)."""

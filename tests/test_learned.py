@@ -351,3 +351,33 @@ def test_capture_train_and_learned_solve_reach_the_same_residual(f, step):
     ctx.set_learned(None)
     net.close()
     ctx.close()
+
+
+@pytest.mark.gpu
+def test_graph_replayed_cycles_are_bit_identical_to_launches():
+    """Learned cycles replayed from a captured hipGraph give the same fields,
+    bit for bit, as the same cycles launched one by one -- over several ticks,
+    each replaying the graph several times between host synchronisations."""
+    from stroemung_amd.learned import PressureNet
+    n, count, f = 64, 4, 4
+    runs = []
+    for graphs in (False, True):
+        ctx, _ = _cavity_context(count, n, seed=31)
+        net = PressureNet(ctx, n // f, n // f, count, seed=32)
+        for _ in range(2):
+            ctx.tick()
+        ctx.set_learned(net, cycles=6, sweeps=8, step="energy", coarsen=f)
+        ctx.set_graphs(graphs)
+        its = [ctx.tick()[0] for _ in range(4)]
+        fields = [ctx.download(c, ("u", "v", "p")) for c in range(count)]
+        for c in range(count):
+            s = ctx.state(c)
+            assert s["residual_squared"] <= s["tolerance_squared"] and not s["exhausted"]
+        runs.append((its, fields))
+        ctx.set_learned(None)
+        net.close()
+        ctx.close()
+    assert runs[0][0] == runs[1][0]
+    for a, b in zip(runs[0][1], runs[1][1]):
+        for k in ("u", "v", "p"):
+            np.testing.assert_array_equal(a[k], b[k])

@@ -381,3 +381,22 @@ def test_graph_replayed_cycles_are_bit_identical_to_launches():
     for a, b in zip(runs[0][1], runs[1][1]):
         for k in ("u", "v", "p"):
             np.testing.assert_array_equal(a[k], b[k])
+
+
+@pytest.mark.gpu
+def test_trained_network_saves_sweeps_on_unseen_layouts():
+    """What the learned part gains: a network trained (on the GPU, with the
+    library's kernels) on pairs from 64x64 cavities of one seed needs far
+    fewer red-black sweeps than the plain solver on cavities of another
+    seed, from the same state and to the same residual."""
+    from stroemung_amd import learned, workloads
+    train = workloads.random_cavities(16, n=64, seed=41, eps=1e-6)
+    params, info = learned.train_net(train, coarsen=4, capture_ticks=24, points=4,
+                                     capture_sweeps=16, steps=600, batch=32, dagger_rounds=1,
+                                     dagger_ticks=8, dagger_steps=300, cycles=12, sweeps=16)
+    test = workloads.random_cavities(8, n=64, seed=42, eps=1e-6)
+    res = learned.compare(test, params, coarsen=4, cycles=12, sweeps=16, warm=6, ticks=4)
+    print("plain %.1f sweeps/solve, learned %.1f" % (res["plain"]["sweeps_per_solve"],
+                                                    res["learned"]["sweeps_per_solve"]))
+    assert res["plain"]["residual_bound_met"] and res["learned"]["residual_bound_met"]
+    assert res["learned"]["sweeps_per_solve"] < 0.5 * res["plain"]["sweeps_per_solve"]

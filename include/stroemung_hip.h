@@ -270,6 +270,17 @@ int stm_net_benchmark(stm_net *net, uint32_t nb, uint32_t reps, double *fwd_ms,
 #define STM_STEP_NETWORK 1
 int stm_set_learned(stm_ctx *ctx, stm_net *net, uint32_t cycles, uint32_t sweeps, int32_t step,
                     uint32_t coarsen);
+/* ---- the GUI image (src/visualization.rs:29-105, render_simulation)
+ * Colours every cell of a channel on the device from its pressure or speed
+ * and the channel's pressure_range / speed_range, exactly as the reference's
+ * render_simulation fills its macroquad Image (fluid: hue 240 at the low end
+ * of the range to 0 at the high end, saturation 1, lightness 0.5; boundary
+ * cells fixed grey or dark red), and copies back nx*ny RGBA pixels, pixel
+ * (x, y) at (y * nx + x) * 4 -- 4 bytes per cell instead of the fields. */
+#define STM_RENDER_PRESSURE 0
+#define STM_RENDER_SPEED 1
+int stm_render(stm_ctx *ctx, uint32_t channel, int32_t mode, uint8_t *rgba);
+
 /* on != 0: learned cycles after the first are captured once into a hipGraph
  * (per net, coarsening, sweeps, step rule and solver variant) and replayed;
  * results are bit-identical to launching them one by one.  Not used while

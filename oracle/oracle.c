@@ -331,6 +331,66 @@ void orc_calculate_speed_range(orc_sim *s)
     s->speed_range[1] = sqrt(mx);
 }
 
+/* --------------------------------------------------------- visualization.rs */
+
+/* src/visualization.rs:7-27, in f32 */
+static void hsl_to_rgb(float hue, float sat, float light, float *r, float *g, float *b)
+{
+    const float c = (1.0f - fabsf(2.0f * light - 1.0f)) * sat;
+    const float x = c * (1.0f - fabsf(fmodf(hue / 60.0f, 2.0f) - 1.0f));
+    const float m = light - c / 2.0f;
+    float rr, gg, bb;
+    if (hue < 60.0f) { rr = c; gg = x; bb = 0.0f; }
+    else if (hue < 120.0f) { rr = x; gg = c; bb = 0.0f; }
+    else if (hue < 180.0f) { rr = 0.0f; gg = c; bb = x; }
+    else if (hue < 240.0f) { rr = 0.0f; gg = x; bb = c; }
+    else if (hue < 300.0f) { rr = x; gg = 0.0f; bb = c; }
+    else { rr = c; gg = 0.0f; bb = x; }
+    *r = rr + m;
+    *g = gg + m;
+    *b = bb + m;
+}
+
+/* Rust `(c * 255.) as u8` on f32: truncation toward zero, saturating, NaN -> 0 */
+static uint8_t to_u8(float c)
+{
+    const float v = c * 255.0f;
+    if (!(v > 0.0f)) return 0;
+    if (v >= 255.0f) return 255;
+    return (uint8_t)v;
+}
+
+/* src/visualization.rs:29-105 */
+void orc_render(const orc_sim *s, int mode, uint8_t *rgba)
+{
+    for (int x = 0; x < s->nx; x++)
+        for (int y = 0; y < s->ny; y++) {
+            const int i = IDX(s, x, y);
+            float r, g, b;
+            if (s->cell_type[i] != ORC_FLUID) {
+                r = 0.5f;
+                g = mode ? 0.5f : 0.0f;
+                b = mode ? 0.5f : 0.0f;
+            } else {
+                double t;
+                if (mode) {
+                    const double speed = sqrt(sq(s->u[i]) + sq(s->v[i]));
+                    t = 240.0 - (speed - s->speed_range[0]) * 240.0 /
+                                    (s->speed_range[1] - s->speed_range[0]);
+                } else {
+                    t = 240.0 - (s->p[i] - s->pressure_range[0]) * 240.0 /
+                                    (s->pressure_range[1] - s->pressure_range[0]);
+                }
+                hsl_to_rgb((float)t, 1.0f, 0.5f, &r, &g, &b);
+            }
+            uint8_t *o = rgba + ((size_t)y * s->nx + x) * 4;
+            o[0] = to_u8(r);
+            o[1] = to_u8(g);
+            o[2] = to_u8(b);
+            o[3] = 255;
+        }
+}
+
 /* --------------------------------------------------------- simulation.rs */
 
 static void gather3(const orc_sim *s, const double *a, int x, int y, double *o)

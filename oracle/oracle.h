@@ -111,6 +111,11 @@ int orc_solve_sor(orc_sim *s, uint32_t *iters, double *norm2);
 void orc_set_u_and_v(orc_sim *s);
 void orc_calculate_pressure_range(orc_sim *s);
 void orc_calculate_speed_range(orc_sim *s);
+/* src/visualization.rs:7-105: the GUI's image, RGBA u8, pixel (x, y) at
+ * (y * nx + x) * 4; mode 0 pressure, 1 speed.  The f32 colour to u8 step is
+ * macroquad 0.4.13's Color -> [u8; 4] (component * 255, truncated,
+ * saturating; NaN -> 0), restated: macroquad is not in the reference. */
+void orc_render(const orc_sim *s, int mode, uint8_t *rgba);
 int orc_construct(orc_sim *s);            /* Simulation::try_from */
 int orc_run_simulation_tick(orc_sim *s, uint32_t *iters, double *norm2);
 

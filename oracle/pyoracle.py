@@ -121,6 +121,8 @@ def _declare(L):
     L.orc_masked_residual_sum.restype = C.c_double
     L.orc_rhs_mean.argtypes = [P]
     L.orc_rhs_mean.restype = C.c_double
+    L.orc_render.argtypes = [P, C.c_int, C.POINTER(C.c_uint8)]
+    L.orc_render.restype = None
 
 
 def _dp(a):
@@ -227,6 +229,12 @@ class Sim:
         a, b = C.c_double(0), C.c_double(0)
         lib().orc_max_uv(C.byref(self.s), C.byref(a), C.byref(b))
         return a.value, b.value
+
+    def render(self, mode=0):
+        """src/visualization.rs: (ny, nx, 4) RGBA u8; mode 0 pressure, 1 speed."""
+        out = np.zeros((self.s.ny, self.s.nx, 4), np.uint8)
+        lib().orc_render(C.byref(self.s), mode, out.ctypes.data_as(C.POINTER(C.c_uint8)))
+        return out
 
     def restore_list(self):
         return [(r.idx, r.has_u, r.u, r.has_v, r.v)

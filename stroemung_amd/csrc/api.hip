@@ -93,6 +93,8 @@ struct stm_ctx {
     float *lrc = nullptr, *lmc = nullptr;   // coarse residual and fluid fraction, nch frames
     double2* d_cpart = nullptr;
     int32_t *d_map = nullptr, *d_inv = nullptr, *d_live = nullptr;   // running channels
+    uint32_t* d_rgba = nullptr;    // rendered image
+    int64_t rgba_capacity = 0;
     int64_t lframe = 0;            // pixels per frame the coarse buffers hold
     std::vector<hipEvent_t> nn_evs;  // per-cycle forward timing, read after the solve
     // learned cycles replayed from a captured hipGraph
@@ -987,7 +989,7 @@ void stm_destroy(stm_ctx* ctx) {
                     ctx->flags, ctx->d_cd, ctx->d_cs, ctx->d_be, ctx->d_restore, ctx->d_tiles,
                     ctx->d_part, ctx->d_active, ctx->d_tbpart, ctx->d_tb, ctx->d_lc, ctx->lres,
                     ctx->lcf, ctx->lsnap, ctx->lrc, ctx->lmc, ctx->d_cpart, ctx->d_map,
-                    ctx->d_inv, ctx->d_live};
+                    ctx->d_inv, ctx->d_live, ctx->d_rgba};
     for (void* b : bufs)
         if (b) hipFree(b);
     if (ctx->h_active) hipHostFree(ctx->h_active);
@@ -1440,6 +1442,30 @@ int stm_set_learned(stm_ctx* ctx, stm_net* net, uint32_t cycles, uint32_t sweeps
     ctx->lcycles = (int)cycles;
     ctx->lsweeps = (int)sweeps;
     return STM_OK;
+}
+
+int stm_render(stm_ctx* ctx, uint32_t ch, int32_t mode, uint8_t* rgba) {
+    int rc = ready(ctx);
+    if (rc) return rc;
+    if ((rc = check_chan(ctx, ch))) return rc;
+    if (!rgba) return fail(ctx, STM_ERR_INVALID_ARG, "null image");
+    if (mode != STM_RENDER_PRESSURE && mode != STM_RENDER_SPEED)
+        return fail(ctx, STM_ERR_INVALID_ARG, "unknown render mode %d", mode);
+    const HostChan& c = ctx->chans[ch];
+    const int64_t npix = (int64_t)c.nx * c.ny;
+    if (npix > ctx->rgba_capacity) {
+        if (ctx->d_rgba) HIPCHK(ctx, hipFree(ctx->d_rgba));
+        ctx->d_rgba = nullptr;
+        ctx->rgba_capacity = 0;
+        HIPCHK(ctx, hipMalloc(&ctx->d_rgba, sizeof(uint32_t) * npix));
+        ctx->rgba_capacity = npix;
+    }
+    LAUNCH("render", launch_render(ctx->stream, ctx->d_cd, ctx->d_cs, (int)ch, npix, ctx->flags,
+                                   ctx->u, ctx->v, ctx->p, mode, ctx->d_rgba));
+    HIPCHK(ctx, hipMemcpyAsync(rgba, ctx->d_rgba, sizeof(uint32_t) * npix, hipMemcpyDeviceToHost,
+                               ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return check_launch(ctx);
 }
 
 int stm_set_graphs(stm_ctx* ctx, int32_t on) {

@@ -55,6 +55,11 @@ void launch_rb_tb(hipStream_t, int variant, int ntb, const ChanDesc*, ChanState*
 void launch_pbuf_fix(hipStream_t, int ntiles, int nchan, const ChanDesc*, ChanState*,
                      const Tile*, const double* pb, double* pa);
 
+// kernels_view.hip: the GUI image of one channel (npix = nx * ny pixels)
+void launch_render(hipStream_t, const ChanDesc*, const ChanState*, int ch, int64_t npix,
+                   const uint8_t* flags, const double* u, const double* v, const double* p,
+                   int mode, uint32_t* rgba);
+
 // kernels_learn.hip: the FP64 side of the learned correction and of capture
 void launch_learn_residual(hipStream_t, int ntiles, const ChanDesc*, const ChanState*,
                            const Tile*, const uint8_t* flags, const double* rhs, const double* p,

@@ -151,3 +151,60 @@ def load_reference_json(fh_or_text):
     floats as the reference's serde_json does."""
     text = fh_or_text if isinstance(fh_or_text, str) else fh_or_text.read()
     return json.loads(text, parse_float=serde_float)
+
+
+# ------------------------------------------------------------ NaSt2D output
+
+NAST2D_FLUID = 16   # NaSt2D's C_F flag; boundary cells carry only neighbour bits
+
+
+def read_nast2d_out(data, int_bytes=4, int_type="i"):
+    """A NaSt2D .out file (python/generate_test_data.py:97-154): two C ints
+    imax, jmax, then U, V, P, T as (imax+2) x (jmax+2) C doubles each, x
+    outer, then the flags as C ints.  Returns a dict of imax, jmax, the four
+    float64 arrays [x][y] and the flags reduced to FLUID (16) or BOUNDARY (0),
+    as the reference's Flag(IntFlag, boundary=CONFORM) keeps only the fluid
+    bit.  `data` is the file's bytes or a binary file object."""
+    raw = data if isinstance(data, (bytes, bytearray)) else data.read()
+    # struct's int codes ("i" signed, "I" unsigned) at the given width, little-endian
+    ity = np.dtype("<%s%d" % ("u" if int_type.isupper() else "i", int_bytes))
+    imax, jmax = (int(v) for v in np.frombuffer(raw, ity, 2, 0))
+    shape = (imax + 2, jmax + 2)
+    n = shape[0] * shape[1]
+    off = 2 * int_bytes
+    fields = []
+    for _ in range(4):
+        fields.append(np.frombuffer(raw, "<f8", n, off).reshape(shape).copy())
+        off += 8 * n
+    flags = np.frombuffer(raw, ity, n, off).reshape(shape)
+    flags = np.where(flags & NAST2D_FLUID, NAST2D_FLUID, 0).astype(np.int64)
+    U, V, P, T = fields
+    return {"imax": imax, "jmax": jmax, "U": U, "V": V, "P": P, "T": T, "flags": flags}
+
+
+def nast2d_grid(out):
+    """The reference's SimulationGrid JSON for a NaSt2D output
+    (python/generate_test_data.py:42-76): NaSt2D does not record boundary
+    kinds, so its presets are assumed -- west wall Inflow with the cell's own
+    (U, V), east wall Outflow, every other boundary cell NoSlip (walls'
+    corners included)."""
+    imax, jmax = out["imax"], out["jmax"]
+    size = [imax + 2, jmax + 2]
+    U, V, flags = out["U"], out["V"], out["flags"]
+    cells = []
+    for x in range(size[0]):
+        for y in range(size[1]):
+            if flags[x, y] == NAST2D_FLUID:
+                cells.append("Fluid")
+            elif 0 < y <= jmax and x == 0:
+                cells.append({"Boundary": {"Inflow": {"velocity": [float(U[x, y]),
+                                                                   float(V[x, y])]}}})
+            elif 0 < y <= jmax and x == imax + 1:
+                cells.append({"Boundary": "Outflow"})
+            else:
+                cells.append({"Boundary": "NoSlip"})
+
+    def flat(a):
+        return {"v": 1, "dim": size, "data": [float(t) for t in a.reshape(-1)]}
+    return {"size": size, "u": flat(U), "v": flat(V), "pressure": flat(out["P"]),
+            "cell_type": {"v": 1, "dim": size, "data": cells}}

@@ -199,6 +199,20 @@ MATH_VECTORS = {
 }
 
 
+def nast2d_fixture():
+    """python/test_data: a NaSt2D .out file (raw bytes, as a list of ints)
+    with the two outputs the reference's test_generate_test_data.py expects
+    from it."""
+    d = "python/test_data/small_data.out"
+    with open(os.path.join(REF, d), "rb") as fh:
+        raw = list(fh.read())
+    with open(os.path.join(REF, d + "_expected.json")) as fh:
+        parsed = json.load(fh)
+    with open(os.path.join(REF, d + "_rust_expected.json")) as fh:
+        grid = json.load(fh)
+    return {"out_bytes": raw, "parsed": parsed, "rust_grid": grid}
+
+
 def main():
     if not os.path.isdir(REF):
         sys.exit("reference checkout not found at " + REF)
@@ -207,6 +221,7 @@ def main():
         "tick_4x3.json": tick_fixture(),
         "simulation_io.json": deserialize_fixture(),
         "grid_io.json": grid_fixture(),
+        "nast2d_small.json": nast2d_fixture(),
     }
     for name, data in fixtures.items():
         with open(os.path.join(HERE, name), "w") as fh:

@@ -18,6 +18,7 @@ struct ConvIn {
     const bf16* b;
     int cb;
     const int* live = nullptr;   // device count: images at or beyond it are skipped
+    float* acc32 = nullptr;      // zeroed FP32 scratch of nb*H*W*cout: allows split K
 };
 
 // 3x3 convolution, padding 1, NHWC bf16, fp32 accumulation on the matrix

@@ -163,7 +163,10 @@ k_conv3x3_tile(ConvIn src, const bf16* __restrict__ w, const float* __restrict__
     for (int i = 0; i < MT; i++)
         for (int u = 0; u < 2; u++) acc[i][u] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    for (int c0 = 0; c0 < cin; c0 += 32) {
+    // split K: block z of gridDim.z takes its share of the 32-channel slices
+    const int nsl = cin / 32, z = blockIdx.z, nz = gridDim.z;
+    const int sl0 = z * nsl / nz, sl1 = (z + 1) * nsl / nz;
+    for (int c0 = sl0 * 32; c0 < sl1 * 32; c0 += 32) {
         __syncthreads();
         const bool first = c0 < src.ca;
         const int mode = first ? src.mode : IN_DIRECT;
@@ -223,13 +226,34 @@ k_conv3x3_tile(ConvIn src, const bf16* __restrict__ w, const float* __restrict__
                 const int m = (wm * MT + i) * 16 + (lane >> 4) * 4 + j;
                 const int gh = th0 + m / TW, gw = tw0 + m % TW;
                 if (gh >= H || gw >= W) continue;
+                const int64_t o = (((int64_t)b * H + gh) * W + gw) * cout + n;
+                if (nz > 1) {                   // partial sum; k_conv_finish completes it
+                    atomicAdd(&src.acc32[o], acc[i][u][j]);
+                    continue;
+                }
                 float v = acc[i][u][j] + bvs;
                 if (flags & 2) v = v > 0.f ? v : 0.f;
-                const int64_t o = (((int64_t)b * H + gh) * W + gw) * cout + n;
                 if (out) out[o] = (bf16)v;
                 if (out32) out32[o] = v;
             }
     }
+}
+
+// Completes a split-K convolution: bias, ReLU and BF16 from the FP32 partial
+// sums, which are zeroed again for the next use.
+__global__ void __launch_bounds__(256)
+k_conv_finish(float* __restrict__ acc32, const float* __restrict__ bias, bf16* __restrict__ out,
+              float* __restrict__ out32, int64_t n, int cout, int flags,
+              const int* __restrict__ live, int64_t per_image)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    if (live && k >= (int64_t)*live * per_image) return;   // never written: still zero
+    float v = acc32[k] + ((flags & 1) ? bias[k % cout] : 0.f);
+    acc32[k] = 0.f;
+    if (flags & 2) v = v > 0.f ? v : 0.f;
+    if (out) out[k] = (bf16)v;
+    if (out32) out32[k] = v;
 }
 
 // w'[ci][8 - tap][co] = w[co][tap][ci]
@@ -556,9 +580,22 @@ template <int TW, int TH, int WN>
 static void conv_tile(hipStream_t st, const ConvIn& src, const bf16* w, const float* bias,
                       bf16* out, float* out32, int cout, int nb, int H, int W, int flags) {
     const unsigned tiles = (unsigned)(((W + TW - 1) / TW) * ((H + TH - 1) / TH) * nb);
-    dim3 grid(tiles, cout / (32 * WN));
+    const unsigned nblocks = tiles * (unsigned)(cout / (32 * WN));
+    // small layers (< 512 workgroups, e.g. the deep levels, or a batch the
+    // learned solve has compacted): split the input channels over up to 4
+    // blocks per tile, each with at least two 32-channel slices
+    const int nsl = (src.ca + src.cb) / 32;
+    int ks = 1;
+    if (src.acc32 && nblocks < 512)
+        while (ks < 4 && nblocks * ks < 1024 && nsl / (2 * ks) >= 2) ks *= 2;
+    dim3 grid(tiles, cout / (32 * WN), ks);
     k_conv3x3_tile<TW, TH, WN><<<grid, 256, 0, st>>>(src, w, bias, out, out32, cout, nb, H, W,
                                                      flags);
+    if (ks > 1) {
+        const int64_t n = (int64_t)nb * H * W * cout;
+        k_conv_finish<<<nblk(n), 256, 0, st>>>(src.acc32, bias, out, out32, n, cout, flags,
+                                               src.live, (int64_t)H * W * cout);
+    }
 }
 
 bool conv_tiles_fit(int H, int W) { return W % 8 == 0 && H % 8 == 0; }

@@ -48,6 +48,7 @@ struct Net {
     float *p32 = nullptr, *g32 = nullptr, *m32 = nullptr, *v32 = nullptr;
     nn::bf16* pb = nullptr;      // bf16 copy of the parameters
     nn::bf16* wflip = nullptr;   // scratch for the backward-input weights
+    float* splitk = nullptr;     // zeroed FP32 partial sums of split-K convolutions
     float* y = nullptr;          // head output, cap x H x W (fp32)
     float* gy = nullptr;
     float* target = nullptr;     // training batch: targets and fluid mask

@@ -102,6 +102,10 @@ int Net::init(stm_ctx* c, int H_, int W_, int cap_, uint64_t seed) {
     int64_t maxw = 0;
     for (auto& cv : convs) maxw = std::max(maxw, (int64_t)cv.cout * 9 * cv.cin);
     NCHK(hipMalloc(&wflip, sizeof(bf16) * maxw));
+    // the largest conv output is level 0's: cap x H x W x 32 (levels below have
+    // a quarter of the pixels and twice the channels)
+    NCHK(hipMalloc(&splitk, sizeof(float) * (size_t)cap * H * W * 32));
+    NCHK(hipMemset(splitk, 0, sizeof(float) * (size_t)cap * H * W * 32));
     for (auto& a : acts) {
         const size_t n = (size_t)cap * h(a.level) * w(a.level) * a.c;
         NCHK(hipMalloc(&a.v, sizeof(bf16) * n));
@@ -138,7 +142,8 @@ int Net::init(stm_ctx* c, int H_, int W_, int cap_, uint64_t seed) {
 }
 
 void Net::release() {
-    void* bufs[] = {p32, g32, m32, v32, pb, wflip, y, gy, target, mask, denom, loss, d_idx, dx, dt};
+    void* bufs[] = {p32, g32, m32, v32, pb, wflip, splitk, y, gy, target, mask, denom, loss,
+                    d_idx, dx, dt};
     for (void* b : bufs)
         if (b) hipFree(b);
     for (auto& a : acts) {
@@ -192,13 +197,17 @@ void Net::forward_infer(int nb, const int* live) {
         const Conv& c = convs[k];
         ConvIn src = in;
         src.live = live;
+        src.acc32 = splitk;
         nn::conv3x3_in(stream, src, pb + c.woff, p32 + c.boff, acts[c.y].v, nullptr, c.cout, nb,
                    h(c.level), w(c.level), 3);
     };
     auto plain = [&](int k) {
         const Conv& c = convs[k];
-        nn::conv3x3(stream, acts[c.x].v, c.cin, pb + c.woff, p32 + c.boff, acts[c.y].v, nullptr,
-                    c.cout, nb, h(c.level), w(c.level), 3, live);
+        if (conv_tiles_fit(h(c.level), w(c.level)))
+            run(k, ConvIn{acts[c.x].v, c.cin, nn::IN_DIRECT, nullptr, 0});
+        else
+            nn::conv3x3(stream, acts[c.x].v, c.cin, pb + c.woff, p32 + c.boff, acts[c.y].v,
+                        nullptr, c.cout, nb, h(c.level), w(c.level), 3, live);
     };
     // conv k whose input is the pooled activation `src` (level - 1)
     auto pooled = [&](int k, int src, int pooled_act) {

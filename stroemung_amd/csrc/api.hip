@@ -642,8 +642,11 @@ int learn_phase(stm_ctx* ctx, int budget, int final_phase) {
             LAUNCH("rb_tb", launch_blocked(ctx, ctx->variant, L, 0));
         LAUNCH("learn_phase_end", launch_learn_phase_end(ctx->stream, nch(ctx), ctx->d_cs,
                                                          ctx->d_lc, nl, budget));
-        LAUNCH("pbuf_fix", launch_pbuf_fix(ctx->stream, ctx->ntiles, nch(ctx), ctx->d_cd,
-                                           ctx->d_cs, ctx->d_tiles, ctx->p2, ctx->p));
+        // an even number of launches leaves p in its own buffer
+        if (nl & 1)
+            LAUNCH("pbuf_fix", launch_pbuf_fix(ctx->stream, ctx->ntiles, nch(ctx), ctx->d_cd,
+                                               ctx->d_cs, ctx->d_tiles, ctx->p2, ctx->p));
+        if (!final_phase) return STM_OK;   // the sweeps are counted by learn_phase_end
     } else {
         int rc = blocked_solve(ctx, 1);
         if (rc) return rc;

@@ -89,7 +89,7 @@ void launch_learn_apply(hipStream_t, int ntiles, const ChanDesc*, const Tile*, c
 void launch_learn_arm(hipStream_t, int nch, const ChanDesc*, ChanState*, const LearnChan*,
                       int32_t* active, int budget);
 void launch_learn_collect(hipStream_t, int nch, ChanState*, LearnChan*, int final_phase);
-void launch_learn_phase_end(hipStream_t, int nch, ChanState*, const LearnChan*, int launches,
+void launch_learn_phase_end(hipStream_t, int nch, ChanState*, LearnChan*, int launches,
                             int budget);
 void launch_learn_target(hipStream_t, int ntiles, int nch, const ChanDesc*, const Tile*,
                          LearnChan*, const uint8_t* flags, const double* p, const double* p0,

@@ -415,15 +415,16 @@ __global__ void k_learn_collect(ChanState* __restrict__ cs, LearnChan* __restric
 }
 
 // End of a budgeted phase run as `launches` unchecked blocked launches: the
-// running channels swept `budget` times and their latest p is in the second
-// buffer when `launches` is odd.  The next cycle's residual decides whether
-// they have converged.
-__global__ void k_learn_phase_end(ChanState* __restrict__ cs, const LearnChan* __restrict__ lc,
+// running channels swept `budget` more times (counted here) and their latest
+// p is in the second buffer when `launches` is odd.  They stay running: the
+// next cycle's residual decides whether they have converged.
+__global__ void k_learn_phase_end(ChanState* __restrict__ cs, LearnChan* __restrict__ lc,
                                   int nch, int launches, int budget)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nch || !lc[c].run) return;
     ChanState& s = cs[c];
+    lc[c].sweeps += budget;
     s.pbuf = launches & 1;
     s.sor_iters = (uint32_t)budget;
     s.exhausted = 1;
@@ -536,7 +537,7 @@ void launch_learn_arm(hipStream_t st, int nch, const ChanDesc* cd, ChanState* cs
                       const LearnChan* lc, int32_t* active, int budget) {
     k_learn_arm<<<nb256(nch), 256, 0, st>>>(cd, cs, lc, active, nch, budget);
 }
-void launch_learn_phase_end(hipStream_t st, int nch, ChanState* cs, const LearnChan* lc,
+void launch_learn_phase_end(hipStream_t st, int nch, ChanState* cs, LearnChan* lc,
                             int launches, int budget) {
     k_learn_phase_end<<<nb256(nch), 256, 0, st>>>(cs, lc, nch, launches, budget);
 }

@@ -67,6 +67,11 @@ template <int K, int R, int HY, int NS, int RP0>
 struct Blk {
     static constexpr int NT = NS * HY;
     static constexpr int EX = NS * R;
+    // each colour's LDS plane has HY guard slots before and after the EX x HY
+    // region, so the neighbour reads of cells on the region's border stay in
+    // bounds (their results are discarded); slot (lx, m) is at OFF + lx*HY + m
+    static constexpr int OFF = HY;
+    static constexpr int PS = EX * HY + 2 * HY;
 
     __device__ static __forceinline__ int col(int r, int q) { return RP0 ^ (r & 1) ^ q; }
     // own slot of colour c in row r
@@ -74,21 +79,21 @@ struct Blk {
 
     // the four neighbours (colour col(r,q) ^ 1) of own cell (r, q)
     template <int r, int q>
-    __device__ static __forceinline__ void nbrs(const double (*P)[EX * HY],
+    __device__ static __forceinline__ void nbrs(const double (*P)[PS],
                                                 const double (&pv)[R][2], int lx0, int m,
                                                 double& pn, double& ps, double& pe, double& pw) {
         constexpr int o = RP0 ^ (r & 1) ^ q ^ 1;
         const int lx = lx0 + r;
-        const double x = P[o][lx * HY + m - 1 + 2 * q];
+        const double x = P[o][OFF + lx * HY + m - 1 + 2 * q];
         pn = q ? pv[r][0] : x;
         ps = q ? x : pv[r][1];
-        if constexpr (r + 1 < R) pe = pv[r + 1][q]; else pe = P[o][(lx + 1) * HY + m];
-        if constexpr (r >= 1) pw = pv[r - 1][q]; else pw = P[o][(lx - 1) * HY + m];
+        if constexpr (r + 1 < R) pe = pv[r + 1][q]; else pe = P[o][OFF + (lx + 1) * HY + m];
+        if constexpr (r >= 1) pw = pv[r - 1][q]; else pw = P[o][OFF + (lx - 1) * HY + m];
     }
 
     // ghost pressure (src/grid/mod.rs:343-412) of own ghosts of colour g
     template <int r, int g>
-    __device__ static __forceinline__ void refresh_row(double (*P)[EX * HY], double (&pv)[R][2],
+    __device__ static __forceinline__ void refresh_row(double (*P)[PS], double (&pv)[R][2],
                                                        const uint8_t (&cc)[R][2], int lx0, int m) {
         constexpr int q = g ^ RP0 ^ (r & 1);
         const int e = cc[r][q] >> C_EDGE_SHIFT;
@@ -107,11 +112,11 @@ struct Blk {
         default: v = (pn + pw) * 0.5; break;   // E_NW
         }
         pv[r][q] = v;
-        P[g][(lx0 + r) * HY + m] = v;
+        P[g][OFF + (lx0 + r) * HY + m] = v;
     }
 
     template <int g, int r = 0>
-    __device__ static __forceinline__ void refresh(double (*P)[EX * HY], double (&pv)[R][2],
+    __device__ static __forceinline__ void refresh(double (*P)[PS], double (&pv)[R][2],
                                                    const uint8_t (&cc)[R][2], int lx0, int m) {
         if constexpr (r < R) {
             refresh_row<r, g>(P, pv, cc, lx0, m);
@@ -119,30 +124,30 @@ struct Blk {
         }
     }
 
-    // red-black update of own fluid cells of colour c
+    // red-black update of own fluid cells of colour c.  Branch-free: every
+    // own cell of the colour is computed and the cells that are not updated
+    // (ghosts, solids, the region's border) keep their value; bit 2r+q of
+    // `upd` says which are updated.
     template <int c, int r = 0>
-    __device__ static __forceinline__ void update(double (*P)[EX * HY], double (&pv)[R][2],
-                                                  const double (&rr)[R][2],
-                                                  const uint8_t (&cc)[R][2], int lx0, int m,
-                                                  double omw, double mid, double rdx2,
-                                                  double rdy2) {
+    __device__ static __forceinline__ void update(double (*P)[PS], double (&pv)[R][2],
+                                                  const double (&rr)[R][2], uint32_t upd,
+                                                  int lx0, int m, double omw, double mid,
+                                                  double rdx2, double rdy2) {
         if constexpr (r < R) {
             constexpr int q = c ^ RP0 ^ (r & 1);
-            if (cc[r][q] & C_UPDATE) {
-                double pn, ps, pe, pw;
-                nbrs<r, q>(P, pv, lx0, m, pn, ps, pe, pw);
-                const double tt = fma(pe + pw, rdx2, (ps + pn) * rdy2);
-                const double nv = fma(omw, pv[r][q], mid * (tt - rr[r][q]));
-                pv[r][q] = nv;
-                P[c][(lx0 + r) * HY + m] = nv;
-            }
-            update<c, r + 1>(P, pv, rr, cc, lx0, m, omw, mid, rdx2, rdy2);
+            double pn, ps, pe, pw;
+            nbrs<r, q>(P, pv, lx0, m, pn, ps, pe, pw);
+            const double tt = fma(pe + pw, rdx2, (ps + pn) * rdy2);
+            const double nv = fma(omw, pv[r][q], mid * (tt - rr[r][q]));
+            pv[r][q] = ((upd >> (2 * r + q)) & 1u) ? nv : pv[r][q];
+            P[c][OFF + (lx0 + r) * HY + m] = pv[r][q];
+            update<c, r + 1>(P, pv, rr, upd, lx0, m, omw, mid, rdx2, rdy2);
         }
     }
 
     // sum of r^2 over own fluid output cells (ghosts fresh)
     template <int r = 0, int q = 0>
-    __device__ static __forceinline__ double resid(double (*P)[EX * HY], const double (&pv)[R][2],
+    __device__ static __forceinline__ double resid(double (*P)[PS], const double (&pv)[R][2],
                                                    const double (&rr)[R][2],
                                                    const uint8_t (&cc)[R][2], int lx0, int m,
                                                    double rdx2, double rdy2, double acc) {
@@ -163,29 +168,35 @@ struct Blk {
         }
     }
 
-    __device__ static __forceinline__ double run(double (*P)[EX * HY], double (&pv)[R][2],
+    __device__ static __forceinline__ double run(double (*P)[PS], double (&pv)[R][2],
                                                  const double (&rr)[R][2],
                                                  const uint8_t (&cc)[R][2], int lx0, int m,
                                                  int nsw, bool check, const ChanDesc& d) {
         const double omw = d.omw, mid = d.mid, rdx2 = d.rdx2, rdy2 = d.rdy2;
+        uint32_t upd = 0, ghost = 0;
 #pragma unroll
         for (int r = 0; r < R; r++)
 #pragma unroll
-            for (int q = 0; q < 2; q++) P[col(r, q)][(lx0 + r) * HY + m] = pv[r][q];
+            for (int q = 0; q < 2; q++) {
+                P[col(r, q)][OFF + (lx0 + r) * HY + m] = pv[r][q];
+                if (cc[r][q] & C_UPDATE) upd |= 1u << (2 * r + q);
+                if (cc[r][q] >> C_EDGE_SHIFT) ghost = 1;
+            }
         __syncthreads();
-        refresh<1>(P, pv, cc, lx0, m);              // black ghosts, for the first red half
+        // ghost refreshes only in waves that own a ghost cell
+        if (ghost) refresh<1>(P, pv, cc, lx0, m);   // black ghosts, for the first red half
         __syncthreads();
         for (int sw = 0; sw < nsw; sw++) {
-            update<0>(P, pv, rr, cc, lx0, m, omw, mid, rdx2, rdy2);
-            refresh<0>(P, pv, cc, lx0, m);          // red ghosts, for the black half
+            update<0>(P, pv, rr, upd, lx0, m, omw, mid, rdx2, rdy2);
+            if (ghost) refresh<0>(P, pv, cc, lx0, m);   // red ghosts, for the black half
             __syncthreads();
-            update<1>(P, pv, rr, cc, lx0, m, omw, mid, rdx2, rdy2);
-            if (check || sw + 1 < nsw) refresh<1>(P, pv, cc, lx0, m);   // for the next red half
+            update<1>(P, pv, rr, upd, lx0, m, omw, mid, rdx2, rdy2);
+            if (ghost && (check || sw + 1 < nsw)) refresh<1>(P, pv, cc, lx0, m);
             __syncthreads();
         }
         double acc = 0.0;
         if (check) {
-            refresh<0>(P, pv, cc, lx0, m);          // red ghosts from the final black
+            if (ghost) refresh<0>(P, pv, cc, lx0, m);   // red ghosts from the final black
             __syncthreads();
             acc = resid(P, pv, rr, cc, lx0, m, rdx2, rdy2, 0.0);
         }
@@ -211,7 +222,7 @@ k_rb_strip(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
     static_assert(R % 2 == 0, "rows per thread must be even");
     static_assert(EX > 2 * H && EY > 2 * H, "tile smaller than its halo");
 
-    __shared__ double P[2][EX * HY];
+    __shared__ double P[2][EX * HY + 2 * HY];   // plus guard slots, see Blk::OFF
     __shared__ double wsum[NT / 64];
 
     const TbTile t = tiles[blockIdx.x];

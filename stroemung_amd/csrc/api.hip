@@ -97,6 +97,9 @@ struct stm_ctx {
     int64_t rgba_capacity = 0;
     int64_t lframe = 0;            // pixels per frame the coarse buffers hold
     std::vector<hipEvent_t> nn_evs;  // per-cycle forward timing, read after the solve
+    // running-channel count of recent learned cycles, read back without draining the queue
+    int32_t* h_live_ring = nullptr;  // pinned, POLL_RING entries
+    hipEvent_t poll_evs[4] = {};
     // learned cycles replayed from a captured hipGraph
     bool use_graphs = false;
     hipGraphExec_t cycle_exec = nullptr;
@@ -612,8 +615,8 @@ int learn_prepare(stm_ctx* ctx, int first) {
     LAUNCH("learn_ghost", launch_ghost(ctx->stream, ctx->nbtot, ctx->d_be, ctx->d_cs, ctx->p, -1,
                                        0));
     LAUNCH("learn_residual", launch_learn_residual(ctx->stream, ctx->ntiles, ctx->d_cd, ctx->d_cs,
-                                                   ctx->d_tiles, ctx->flags, ctx->rhs, ctx->p,
-                                                   ctx->lres, ctx->d_part));
+                                                   ctx->d_lc, first, ctx->d_tiles, ctx->flags,
+                                                   ctx->rhs, ctx->p, ctx->lres, ctx->d_part));
     LAUNCH("learn_check", launch_learn_check(ctx->stream, nch(ctx), ctx->d_cd, ctx->d_cs,
                                              ctx->d_lc, ctx->d_part, first));
     return STM_OK;
@@ -656,15 +659,6 @@ int learn_phase(stm_ctx* ctx, int budget, int final_phase) {
     return STM_OK;
 }
 
-int any_running(stm_ctx* ctx, bool* any) {
-    HIPCHK(ctx, hipMemcpyAsync(ctx->h_lc, ctx->d_lc, sizeof(LearnChan) * nch(ctx),
-                               hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-    *any = false;
-    for (int k = 0; k < nch(ctx); k++) *any = *any || ctx->h_lc[k].run;
-    return check_launch(ctx);
-}
-
 // The network's correction of p: the running channels packed to the front
 // of the batch (the network skips the rest), forward on their coarse
 // residuals, prolongation, ghosts, step length, update.
@@ -685,7 +679,8 @@ int learn_correct(stm_ctx* ctx, Net& net, int f, hipEvent_t ev0, hipEvent_t ev1)
     LAUNCH("learn_ghost", launch_ghost(ctx->stream, ctx->nbtot, ctx->d_be, ctx->d_cs, ctx->lcf,
                                        -1, 0));
     LAUNCH("learn_dots", launch_learn_dots(ctx->stream, ctx->ntiles, ctx->d_cd, ctx->d_tiles,
-                                           ctx->flags, ctx->lres, ctx->lcf, ctx->d_part));
+                                           ctx->d_lc, ctx->flags, ctx->lres, ctx->lcf,
+                                           ctx->d_part));
     LAUNCH("learn_alpha", launch_learn_alpha(ctx->stream, n, ctx->d_cd, ctx->d_lc, ctx->d_part,
                                              ctx->lstep, f));
     LAUNCH("learn_apply", launch_learn_apply(ctx->stream, ctx->ntiles, ctx->d_cd, ctx->d_tiles,
@@ -732,6 +727,23 @@ int graph_cycle(stm_ctx* ctx, Net& net, int f) {
     return STM_OK;
 }
 
+// The learned solve reads back how many channels cycle k found running
+// (k_learn_compact's count) into a small ring of pinned slots, each with an
+// event.  Before queueing cycle k the host waits for cycle k - 1 - POLL_LAG
+// only, so the queue keeps POLL_LAG + 1 cycles of work while the host
+// decides; a slot is rewritten POLL_RING cycles later, after its event was
+// waited for.
+constexpr int POLL_LAG = 1, POLL_RING = 4;
+static_assert(POLL_RING > POLL_LAG + 1, "a slot is read before it is rewritten");
+
+int post_live(stm_ctx* ctx, int k) {
+    const int j = k % POLL_RING;
+    HIPCHK(ctx, hipMemcpyAsync(ctx->h_live_ring + j, ctx->d_live, sizeof(int32_t),
+                               hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipEventRecord(ctx->poll_evs[j], ctx->stream));
+    return STM_OK;
+}
+
 // Red-black solve in phases.  Phase k starts with the residual and the
 // convergence test; when capturing, its first `cpoints` phases each store
 // a training input and a snapshot of p; with the learned solve on, each
@@ -754,13 +766,15 @@ int phased_solve(stm_ctx* ctx) {
     // graphs replay cycles 1.. of a learned solve that captures nothing and is not profiled
     const bool graphs = L && !C && ctx->use_graphs && ctx->sample == 0;
     for (int k = 0; k < phases; k++) {
+        if (L && k > POLL_LAG) {
+            // cycle k - 1 - POLL_LAG found no channel running: the cycles after it are no-ops
+            const int j = (k - 1 - POLL_LAG) % POLL_RING;
+            HIPCHK(ctx, hipEventSynchronize(ctx->poll_evs[j]));
+            if (ctx->h_live_ring[j] == 0) break;
+        }
         if (graphs && k > 0) {
             if ((rc = graph_cycle(ctx, *L, f))) return rc;
-            if (k % 8 == 7) {
-                bool any = false;
-                if ((rc = any_running(ctx, &any))) return rc;
-                if (!any) break;
-            }
+            if ((rc = post_live(ctx, k))) return rc;
             continue;
         }
         if ((rc = learn_prepare(ctx, k == 0))) return rc;
@@ -786,11 +800,7 @@ int phased_solve(stm_ctx* ctx) {
         }
         const int budget = L ? ctx->lsweeps : ctx->csweeps << k;
         if ((rc = learn_phase(ctx, budget, 0))) return rc;
-        if (L && k % 8 == 7) {
-            bool any = false;
-            if ((rc = any_running(ctx, &any))) return rc;
-            if (!any) break;
-        }
+        if (L && (rc = post_live(ctx, k))) return rc;
     }
     if ((rc = learn_phase(ctx, 0, 1))) return rc;
     for (int j = 0; j < corrected; j++) {   // the final phase has synchronised
@@ -827,6 +837,10 @@ int learn_buffers(stm_ctx* ctx, const Net& net, bool capture) {
         HIPCHK(ctx, hipMalloc(&ctx->d_map, sizeof(int32_t) * nch(ctx)));
         HIPCHK(ctx, hipMalloc(&ctx->d_inv, sizeof(int32_t) * nch(ctx)));
         HIPCHK(ctx, hipMalloc(&ctx->d_live, sizeof(int32_t)));
+        HIPCHK(ctx, hipHostMalloc(&ctx->h_live_ring, sizeof(int32_t) * POLL_RING,
+                                  hipHostMallocDefault));
+        for (int j = 0; j < POLL_RING; j++)
+            HIPCHK(ctx, hipEventCreateWithFlags(&ctx->poll_evs[j], hipEventDisableTiming));
     }
     const int64_t frame = (int64_t)net.H * net.W;
     if (frame > ctx->lframe) {
@@ -998,6 +1012,9 @@ void stm_destroy(stm_ctx* ctx) {
     if (ctx->h_active) hipHostFree(ctx->h_active);
     if (ctx->h_cs) hipHostFree(ctx->h_cs);
     if (ctx->h_lc) hipHostFree(ctx->h_lc);
+    if (ctx->h_live_ring) hipHostFree(ctx->h_live_ring);
+    for (hipEvent_t e : ctx->poll_evs)
+        if (e) hipEventDestroy(e);
     for (hipEvent_t e : ctx->nn_evs) hipEventDestroy(e);
     if (ctx->cycle_exec) hipGraphExecDestroy(ctx->cycle_exec);
     if (ctx->stream) hipStreamDestroy(ctx->stream);

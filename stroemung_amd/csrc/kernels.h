@@ -62,8 +62,8 @@ void launch_render(hipStream_t, const ChanDesc*, const ChanState*, int ch, int64
 
 // kernels_learn.hip: the FP64 side of the learned correction and of capture
 void launch_learn_residual(hipStream_t, int ntiles, const ChanDesc*, const ChanState*,
-                           const Tile*, const uint8_t* flags, const double* rhs, const double* p,
-                           double* res, double* part);
+                           const LearnChan*, int first, const Tile*, const uint8_t* flags,
+                           const double* rhs, const double* p, double* res, double* part);
 void launch_learn_check(hipStream_t, int nch, const ChanDesc*, ChanState*, LearnChan*,
                         const double* part, int first);
 // pixel blocks of an H x W frame (the size of cpart per channel)
@@ -80,7 +80,7 @@ void launch_learn_pack(hipStream_t, int nch, const LearnChan*, const float* rc, 
 void launch_learn_prolong(hipStream_t, int ntiles, const ChanDesc*, const Tile*,
                           const LearnChan*, const uint8_t* flags, const float* y,
                           const float* mc, int H, int W, int f, const int32_t* inv, double* cf);
-void launch_learn_dots(hipStream_t, int ntiles, const ChanDesc*, const Tile*,
+void launch_learn_dots(hipStream_t, int ntiles, const ChanDesc*, const Tile*, const LearnChan*,
                        const uint8_t* flags, const double* res, const double* cf, double* part);
 void launch_learn_alpha(hipStream_t, int nch, const ChanDesc*, LearnChan*, const double* part,
                         int step, int f);

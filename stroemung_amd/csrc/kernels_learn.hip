@@ -117,15 +117,19 @@ __device__ __forceinline__ double block_fine_sum(const ChanDesc& d, const uint8_
 }  // namespace
 
 // r = Lap(p) - (rhs - mean) at interior fluid cells, 0 elsewhere; slot 2 = sum r^2.
-// p's ghosts must be current.
+// p's ghosts must be current.  After the first phase of a solve, tiles of
+// channels that are no longer running are skipped (their r and slot 2 are
+// read only while a channel runs).
 __global__ void __launch_bounds__(TILE_THREADS)
 k_learn_residual(const ChanDesc* __restrict__ cd, const ChanState* __restrict__ cs,
+                 const LearnChan* __restrict__ lc, int first,
                  const Tile* __restrict__ tiles, const uint8_t* __restrict__ flags,
                  const double* __restrict__ rhs, const double* __restrict__ p,
                  double* __restrict__ res, double* __restrict__ part)
 {
     __shared__ double sh[TILE_THREADS];
     const Tile t = tiles[blockIdx.x];
+    if (!first && !lc[t.chan].run) return;       // uniform over the block, before any barrier
     const ChanDesc& d = cd[t.chan];
     const int x = t.x0 + threadIdx.y, y = t.y0 + threadIdx.x;
     double r2 = 0.0;
@@ -282,12 +286,13 @@ k_learn_prolong(const ChanDesc* __restrict__ cd, const Tile* __restrict__ tiles,
                 const int32_t* __restrict__ inv, double* __restrict__ cf)
 {
     const Tile t = tiles[blockIdx.x];
+    if (!lc[t.chan].run) return;                 // z of a stopped channel is never used
     const ChanDesc& d = cd[t.chan];
     const int x = t.x0 + threadIdx.y, yy = t.y0 + threadIdx.x;
     if (x >= d.nx || yy >= d.ny) return;
     const int64_t i = d.off + x * (int64_t)d.pitch + yy;
     double v = 0.0;
-    if (lc[t.chan].run && interior(d, x, yy) && flag_fluid(flags[i])) {
+    if (interior(d, x, yy) && flag_fluid(flags[i])) {
         const int hc_n = (d.nx - 2 + f - 1) / f, wc_n = (d.ny - 2 + f - 1) / f;
         const double xi = fmin(fmax(((x - 1) + 0.5) / f - 0.5, 0.0), (double)(hc_n - 1));
         const double eta = fmin(fmax(((yy - 1) + 0.5) / f - 0.5, 0.0), (double)(wc_n - 1));
@@ -309,14 +314,17 @@ k_learn_prolong(const ChanDesc* __restrict__ cd, const Tile* __restrict__ tiles,
     cf[i] = v;
 }
 
-// q = Lap(z) at interior fluid cells: slot 3 = sum r z, slot 4 = sum z q.
+// q = Lap(z) at interior fluid cells: slot 3 = sum r z, slot 4 = sum z q
+// (running channels only: the step length of the others is 0).
 __global__ void __launch_bounds__(TILE_THREADS)
 k_learn_dots(const ChanDesc* __restrict__ cd, const Tile* __restrict__ tiles,
-             const uint8_t* __restrict__ flags, const double* __restrict__ res,
-             const double* __restrict__ cf, double* __restrict__ part)
+             const LearnChan* __restrict__ lc, const uint8_t* __restrict__ flags,
+             const double* __restrict__ res, const double* __restrict__ cf,
+             double* __restrict__ part)
 {
     __shared__ double sh[TILE_THREADS];
     const Tile t = tiles[blockIdx.x];
+    if (!lc[t.chan].run) return;                 // uniform over the block, before any barrier
     const ChanDesc& d = cd[t.chan];
     const int x = t.x0 + threadIdx.y, y = t.y0 + threadIdx.x;
     double rz = 0.0, zq = 0.0;
@@ -491,9 +499,10 @@ static const dim3 TB(TILE_Y, TILE_X);
 int learn_coarse_blocks(int H, int W) { return (int)nb256((int64_t)H * W); }
 
 void launch_learn_residual(hipStream_t st, int ntiles, const ChanDesc* cd, const ChanState* cs,
-                           const Tile* tiles, const uint8_t* flags, const double* rhs,
-                           const double* p, double* res, double* part) {
-    k_learn_residual<<<ntiles, TB, 0, st>>>(cd, cs, tiles, flags, rhs, p, res, part);
+                           const LearnChan* lc, int first, const Tile* tiles,
+                           const uint8_t* flags, const double* rhs, const double* p, double* res,
+                           double* part) {
+    k_learn_residual<<<ntiles, TB, 0, st>>>(cd, cs, lc, first, tiles, flags, rhs, p, res, part);
 }
 void launch_learn_check(hipStream_t st, int nch, const ChanDesc* cd, ChanState* cs, LearnChan* lc,
                         const double* part, int first) {
@@ -522,8 +531,9 @@ void launch_learn_prolong(hipStream_t st, int ntiles, const ChanDesc* cd, const 
     k_learn_prolong<<<ntiles, TB, 0, st>>>(cd, tiles, lc, flags, y, mc, H, W, f, inv, cf);
 }
 void launch_learn_dots(hipStream_t st, int ntiles, const ChanDesc* cd, const Tile* tiles,
-                       const uint8_t* flags, const double* res, const double* cf, double* part) {
-    k_learn_dots<<<ntiles, TB, 0, st>>>(cd, tiles, flags, res, cf, part);
+                       const LearnChan* lc, const uint8_t* flags, const double* res,
+                       const double* cf, double* part) {
+    k_learn_dots<<<ntiles, TB, 0, st>>>(cd, tiles, lc, flags, res, cf, part);
 }
 void launch_learn_alpha(hipStream_t st, int nch, const ChanDesc* cd, LearnChan* lc,
                         const double* part, int step, int f) {

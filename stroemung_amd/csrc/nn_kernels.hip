@@ -125,6 +125,12 @@ k_conv3x3(const bf16* __restrict__ in, int cin, const bf16* __restrict__ w,
 // [ca, ca + cb) come from `b` read directly (the skip half of a
 // concatenation).  Pooling, upsampling and concatenation thus happen while
 // staging, without materialising their outputs.
+//
+// The patch is double-buffered: the global loads of slice s + 1 are issued
+// into registers before the MFMAs of slice s, and stored to the other buffer
+// after them, so one barrier per slice suffices and the load latency hides
+// behind the matrix work.  Every thread issues all its loads of a slice
+// before its first LDS store.
 template <int TW, int TH, int WN>
 __global__ void __launch_bounds__(256)
 k_conv3x3_tile(ConvIn src, const bf16* __restrict__ w, const float* __restrict__ bias,
@@ -135,8 +141,10 @@ k_conv3x3_tile(ConvIn src, const bf16* __restrict__ w, const float* __restrict__
     constexpr int CS = 40;                       // padded channel stride of a staged pixel
     constexpr int WM = 4 / WN;                   // waves along M
     constexpr int MT = TH * TW / WM / 16;        // M tiles per wave
+    constexpr int NPV = PH * PW * 4;             // 16-byte vectors of one staged slice
+    constexpr int NV = (NPV + 255) / 256;        // of them per thread
     static_assert(MT >= 1 && TH * TW % (WM * 16) == 0, "tile shape");
-    __shared__ __attribute__((aligned(16))) bf16 patch[PH * PW * CS];
+    __shared__ __attribute__((aligned(16))) bf16 patch[2][PH * PW * CS];
 
     const int cin = src.ca + src.cb;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -166,18 +174,22 @@ k_conv3x3_tile(ConvIn src, const bf16* __restrict__ w, const float* __restrict__
     // split K: block z of gridDim.z takes its share of the 32-channel slices
     const int nsl = cin / 32, z = blockIdx.z, nz = gridDim.z;
     const int sl0 = z * nsl / nz, sl1 = (z + 1) * nsl / nz;
-    for (int c0 = sl0 * 32; c0 < sl1 * 32; c0 += 32) {
-        __syncthreads();
+
+    // the NV staged vectors of this thread for the slice at channel c0
+    bf16x8 xs[NV];
+    auto fetch = [&](int c0) {
         const bool first = c0 < src.ca;
         const int mode = first ? src.mode : IN_DIRECT;
         const bf16* base = first ? src.a : src.b;
         const int cs = first ? src.ca : src.cb, cc = first ? c0 : c0 - src.ca;
-        for (int v = threadIdx.x; v < PH * PW * 4; v += 256) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) {
+            const int v = threadIdx.x + k * 256;
             const int pix = v >> 2, part = v & 3;
             const int ph = pix / PW, pw = pix - ph * PW;
             const int gh = th0 + ph - 1, gw = tw0 + pw - 1;
             bf16x8 x = zero8();
-            if (gh >= 0 && gh < H && gw >= 0 && gw < W) {
+            if (v < NPV && gh >= 0 && gh < H && gw >= 0 && gw < W) {
                 const int ch = cc + part * 8;
                 if (mode == IN_DIRECT) {
                     x = load8(base + (((int64_t)b * H + gh) * W + gw) * cs + ch);
@@ -194,9 +206,25 @@ k_conv3x3_tile(ConvIn src, const bf16* __restrict__ w, const float* __restrict__
                                            fmaxf((float)p2[e], (float)p3[e]));
                 }
             }
-            *reinterpret_cast<bf16x8*>(&patch[pix * CS + part * 8]) = x;
+            xs[k] = x;
         }
-        __syncthreads();
+    };
+    auto put = [&](int buf) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) {
+            const int v = threadIdx.x + k * 256;
+            if (v < NPV) *reinterpret_cast<bf16x8*>(&patch[buf][(v >> 2) * CS + (v & 3) * 8]) = xs[k];
+        }
+    };
+
+    if (sl0 < sl1) {
+        fetch(sl0 * 32);
+        put(0);
+    }
+    __syncthreads();
+    for (int s = sl0; s < sl1; s++) {
+        const int buf = (s - sl0) & 1, c0 = s * 32;
+        if (s + 1 < sl1) fetch(c0 + 32);         // in flight during the MFMAs below
 #pragma unroll
         for (int tap = 0; tap < 9; tap++) {
             const int dy = tap / 3, dx = tap % 3;    // patch offsets 0..2
@@ -207,12 +235,15 @@ k_conv3x3_tile(ConvIn src, const bf16* __restrict__ w, const float* __restrict__
             const int shift = (dy * PW + dx) * CS;
 #pragma unroll
             for (int i = 0; i < MT; i++) {
-                const bf16x8 a = *reinterpret_cast<const bf16x8*>(&patch[aoff[i] + shift]);
+                const bf16x8 a = *reinterpret_cast<const bf16x8*>(&patch[buf][aoff[i] + shift]);
 #pragma unroll
                 for (int u = 0; u < 2; u++)
                     acc[i][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bv[u], acc[i][u], 0, 0, 0);
             }
         }
+        // the other buffer was last read before the previous barrier
+        if (s + 1 < sl1) put(buf ^ 1);
+        __syncthreads();
     }
     // epilogue: lane holds column n = l % 16, rows 4 * (l / 16) + j of each M tile
 #pragma unroll

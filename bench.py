@@ -214,11 +214,11 @@ def learned_cavities(ticks=10, log=None):
     train = workloads.random_cavities(64, n=256, seed=0, eps=1e-6)
     params, info = learned.train_net(train, coarsen=4, capture_ticks=48, points=4,
                                      capture_sweeps=32, steps=3000, batch=32, dagger_rounds=2,
-                                     dagger_ticks=16, dagger_steps=1500, cycles=16, sweeps=32,
-                                     log=log)
+                                     dagger_ticks=16, dagger_steps=1500, cycles=32, sweeps=16,
+                                     step="momentum", log=log)
     test = workloads.random_cavities(32, n=256, seed=100, eps=1e-6)
-    res = learned.compare(test, params, coarsen=4, cycles=16, sweeps=32, warm=20, ticks=ticks,
-                          log=log)
+    res = learned.compare(test, params, coarsen=4, cycles=32, sweeps=16, warm=20, ticks=ticks,
+                          step="momentum", log=log)
     res["training"] = info
     res["workload"] = ("32 lid cavities 256x256 with 1-4 random obstacles, Re 50-1000, "
                        "red-black SOR omega=1.8 to relative residual 1e-6, tau=0.5, fp64; "
@@ -236,10 +236,10 @@ def learned_channels(ticks, log=None):
     params, info = learned.train_net(train, coarsen=8, capture_ticks=40, points=4,
                                      capture_sweeps=32, steps=3000, batch=32, dagger_rounds=2,
                                      dagger_ticks=24, dagger_steps=1500, cycles=64, sweeps=32,
-                                     log=log)
+                                     step="momentum", log=log)
     test = workloads.fixed_channels(64, seed=1, eps=1e-7)
     res = learned.compare(test, params, coarsen=8, cycles=64, sweeps=32, warm=10, ticks=ticks,
-                          log=log)
+                          step="momentum", log=log)
     res["training"] = info
     res["workload"] = ("64 channels 512x256 with 1-4 random obstacles, Re 100-1000, inflow u=1, "
                        "red-black SOR omega=1.7 to relative residual 1e-7, tau=0.5, fp64; "

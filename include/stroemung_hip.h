@@ -260,7 +260,11 @@ int stm_net_benchmark(stm_net *net, uint32_t nb, uint32_t reps, double *fwd_ms,
  * to output the restricted error times rms(R r) / h_c^2 (h_c^2 = coarsen^2
  * dx dy); step STM_STEP_NETWORK takes that magnitude (a = h_c^2 / rms(R r)),
  * STM_STEP_ENERGY the a that minimises the energy norm of the new error
- * (a = <r, z> / <z, -Lap z>).  The stopping test and the FP64 arithmetic of
+ * (a = <r, z> / <z, -Lap z>), STM_STEP_MOMENTUM sets p += a z + b s with
+ * (a, b) minimising that norm over span{z, s}, s being the change of p over
+ * the previous cycle (a flexible-CG-like acceleration across cycles; the
+ * first cycle of a solve takes the energy step).  The stopping test and the
+ * FP64 arithmetic of
  * the sweeps are those of the plain solver, so the result meets the same
  * residual.  net = NULL or cycles = 0 turns it off.  The net must come from
  * this context, hold one batch entry per channel and an H x W frame with
@@ -268,6 +272,7 @@ int stm_net_benchmark(stm_net *net, uint32_t nb, uint32_t reps, double *fwd_ms,
  * falls in pixel ((x - 1) / coarsen, (y - 1) / coarsen)). */
 #define STM_STEP_ENERGY 0
 #define STM_STEP_NETWORK 1
+#define STM_STEP_MOMENTUM 2
 int stm_set_learned(stm_ctx *ctx, stm_net *net, uint32_t cycles, uint32_t sweeps, int32_t step,
                     uint32_t coarsen);
 /* ---- the GUI image (src/visualization.rs:29-105, render_simulation)

@@ -89,6 +89,7 @@ struct stm_ctx {
     LearnChan* d_lc = nullptr;
     LearnChan* h_lc = nullptr;     // pinned
     double *lres = nullptr, *lcf = nullptr;
+    double* lprev = nullptr;       // two-direction step: p at the previous cycle
     double* lsnap = nullptr;       // capture: LEARN_POINTS snapshots of p
     float *lrc = nullptr, *lmc = nullptr;   // coarse residual and fluid fraction, nch frames
     double2* d_cpart = nullptr;
@@ -662,8 +663,9 @@ int learn_phase(stm_ctx* ctx, int budget, int final_phase) {
 // The network's correction of p: the running channels packed to the front
 // of the batch (the network skips the rest), forward on their coarse
 // residuals, prolongation, ghosts, step length, update.
-int learn_correct(stm_ctx* ctx, Net& net, int f, hipEvent_t ev0, hipEvent_t ev1) {
+int learn_correct(stm_ctx* ctx, Net& net, int f, int first, hipEvent_t ev0, hipEvent_t ev1) {
     const int n = nch(ctx);
+    double* prev = ctx->lstep == STM_STEP_MOMENTUM ? ctx->lprev : nullptr;
     LAUNCH("learn_compact", launch_learn_compact(ctx->stream, n, ctx->d_lc, ctx->d_map,
                                                  ctx->d_inv, ctx->d_live));
     LAUNCH("learn_pack", launch_learn_pack(ctx->stream, n, ctx->d_lc, ctx->lrc, ctx->lmc, net.H,
@@ -679,12 +681,12 @@ int learn_correct(stm_ctx* ctx, Net& net, int f, hipEvent_t ev0, hipEvent_t ev1)
     LAUNCH("learn_ghost", launch_ghost(ctx->stream, ctx->nbtot, ctx->d_be, ctx->d_cs, ctx->lcf,
                                        -1, 0));
     LAUNCH("learn_dots", launch_learn_dots(ctx->stream, ctx->ntiles, ctx->d_cd, ctx->d_tiles,
-                                           ctx->d_lc, ctx->flags, ctx->lres, ctx->lcf,
-                                           ctx->d_part));
+                                           ctx->d_lc, ctx->flags, ctx->lres, ctx->lcf, ctx->p,
+                                           prev, ctx->d_part));
     LAUNCH("learn_alpha", launch_learn_alpha(ctx->stream, n, ctx->d_cd, ctx->d_lc, ctx->d_part,
-                                             ctx->lstep, f));
+                                             ctx->lstep, f, first));
     LAUNCH("learn_apply", launch_learn_apply(ctx->stream, ctx->ntiles, ctx->d_cd, ctx->d_tiles,
-                                             ctx->d_lc, ctx->flags, ctx->lcf, ctx->p));
+                                             ctx->d_lc, ctx->flags, ctx->lcf, ctx->p, prev));
     return STM_OK;
 }
 
@@ -696,7 +698,7 @@ int learned_cycle(stm_ctx* ctx, Net& net, int f) {
     int rc;
     if ((rc = learn_prepare(ctx, 0))) return rc;
     if ((rc = learn_restrict(ctx, net, f, -1))) return rc;
-    if ((rc = learn_correct(ctx, net, f, nullptr, nullptr))) return rc;
+    if ((rc = learn_correct(ctx, net, f, 0, nullptr, nullptr))) return rc;
     return learn_phase(ctx, ctx->lsweeps, 0);
 }
 
@@ -794,7 +796,8 @@ int phased_solve(stm_ctx* ctx) {
                 HIPCHK(ctx, hipEventCreate(&e));
                 ctx->nn_evs.push_back(e);
             }
-            if ((rc = learn_correct(ctx, *L, f, ctx->nn_evs[2 * k], ctx->nn_evs[2 * k + 1])))
+            if ((rc = learn_correct(ctx, *L, f, k == 0, ctx->nn_evs[2 * k],
+                                    ctx->nn_evs[2 * k + 1])))
                 return rc;
             corrected++;
         }
@@ -834,6 +837,8 @@ int learn_buffers(stm_ctx* ctx, const Net& net, bool capture) {
         HIPCHK(ctx, hipMalloc(&ctx->lcf, bytes));
         HIPCHK(ctx, hipMemset(ctx->lres, 0, bytes));
         HIPCHK(ctx, hipMemset(ctx->lcf, 0, bytes));
+        HIPCHK(ctx, hipMalloc(&ctx->lprev, bytes));
+        HIPCHK(ctx, hipMemset(ctx->lprev, 0, bytes));
         HIPCHK(ctx, hipMalloc(&ctx->d_map, sizeof(int32_t) * nch(ctx)));
         HIPCHK(ctx, hipMalloc(&ctx->d_inv, sizeof(int32_t) * nch(ctx)));
         HIPCHK(ctx, hipMalloc(&ctx->d_live, sizeof(int32_t)));
@@ -1005,7 +1010,7 @@ void stm_destroy(stm_ctx* ctx) {
     void* bufs[] = {ctx->u, ctx->v, ctx->p, ctx->f, ctx->g, ctx->rhs, ctx->scratch, ctx->p2,
                     ctx->flags, ctx->d_cd, ctx->d_cs, ctx->d_be, ctx->d_restore, ctx->d_tiles,
                     ctx->d_part, ctx->d_active, ctx->d_tbpart, ctx->d_tb, ctx->d_lc, ctx->lres,
-                    ctx->lcf, ctx->lsnap, ctx->lrc, ctx->lmc, ctx->d_cpart, ctx->d_map,
+                    ctx->lcf, ctx->lprev, ctx->lsnap, ctx->lrc, ctx->lmc, ctx->d_cpart, ctx->d_map,
                     ctx->d_inv, ctx->d_live, ctx->d_rgba};
     for (void* b : bufs)
         if (b) hipFree(b);
@@ -1452,7 +1457,7 @@ int stm_set_learned(stm_ctx* ctx, stm_net* net, uint32_t cycles, uint32_t sweeps
         return STM_OK;
     }
     if (sweeps == 0) return fail(ctx, STM_ERR_INVALID_ARG, "sweeps per cycle must be positive");
-    if (step != STM_STEP_ENERGY && step != STM_STEP_NETWORK)
+    if (step != STM_STEP_ENERGY && step != STM_STEP_NETWORK && step != STM_STEP_MOMENTUM)
         return fail(ctx, STM_ERR_INVALID_ARG, "unknown step rule %d", step);
     if ((rc = check_net(ctx, net, coarsen))) return rc;
     if ((rc = learn_buffers(ctx, net->net, false))) return rc;

@@ -51,6 +51,16 @@ __device__ __forceinline__ double lap(const double* __restrict__ a, int64_t i, i
     return fma((a[i + P] - c) - (c - a[i - P]), rdx2, ((a[i + 1] - c) - (c - a[i - 1])) * rdy2);
 }
 
+// Lap(a - b) at i, differences taken cell by cell.
+__device__ __forceinline__ double lap_diff(const double* __restrict__ a,
+                                           const double* __restrict__ b, int64_t i, int64_t P,
+                                           double rdx2, double rdy2) {
+    const double c = a[i] - b[i];
+    const double e = a[i + P] - b[i + P], w = a[i - P] - b[i - P];
+    const double n = a[i - 1] - b[i - 1], so = a[i + 1] - b[i + 1];
+    return fma((e - c) - (c - w), rdx2, ((so - c) - (c - n)) * rdy2);
+}
+
 // Sum of one tile block's values into part[blockIdx.x * NPART + slot].
 __device__ void tile_sum(double v, double* sh, double* __restrict__ part, int slot) {
     const int tid = threadIdx.y * blockDim.x + threadIdx.x;
@@ -88,6 +98,8 @@ __device__ double chan_total(const ChanDesc& d, const double* __restrict__ part,
 }
 
 constexpr int S_RES2 = 2, S_RZ = 3, S_ZQ = 4, S_DSUM = 5;
+// two-direction step (slots 0 and 1 are free once the solve has begun)
+constexpr int S_RS = 0, S_SQ = 1, S_ZS = 6;
 
 // Sum over the interior fluid cells of coarse pixel (hc, wc) of a[g], or of
 // (a[g] - a0[g]) - shift when a0 is given; *nf = the number of those cells.
@@ -168,6 +180,7 @@ k_learn_check(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
         s.exhausted = 0;
     }
     l.alpha = 0.0;
+    l.beta = 0.0;
 }
 
 // Coarse residual rc and fluid fraction mc of every pixel of an H x W frame
@@ -315,11 +328,15 @@ k_learn_prolong(const ChanDesc* __restrict__ cd, const Tile* __restrict__ tiles,
 }
 
 // q = Lap(z) at interior fluid cells: slot 3 = sum r z, slot 4 = sum z q
-// (running channels only: the step length of the others is 0).
+// (running channels only: the step length of the others is 0).  With
+// prev != nullptr also s = p - prev, the change of p over the previous cycle
+// (p and prev both hold ghosts consistent with their interiors, so s does):
+// slot 0 = sum r s, slot 1 = sum s Lap(s), slot 6 = sum z Lap(s).
 __global__ void __launch_bounds__(TILE_THREADS)
 k_learn_dots(const ChanDesc* __restrict__ cd, const Tile* __restrict__ tiles,
              const LearnChan* __restrict__ lc, const uint8_t* __restrict__ flags,
              const double* __restrict__ res, const double* __restrict__ cf,
+             const double* __restrict__ p, const double* __restrict__ prev,
              double* __restrict__ part)
 {
     __shared__ double sh[TILE_THREADS];
@@ -327,50 +344,100 @@ k_learn_dots(const ChanDesc* __restrict__ cd, const Tile* __restrict__ tiles,
     if (!lc[t.chan].run) return;                 // uniform over the block, before any barrier
     const ChanDesc& d = cd[t.chan];
     const int x = t.x0 + threadIdx.y, y = t.y0 + threadIdx.x;
-    double rz = 0.0, zq = 0.0;
+    double rz = 0.0, zq = 0.0, rs = 0.0, sq = 0.0, zs = 0.0;
     if (x < d.nx && y < d.ny && interior(d, x, y)) {
         const int64_t i = d.off + x * (int64_t)d.pitch + y;
         if (flag_fluid(flags[i])) {
             const double q = lap(cf, i, d.pitch, d.rdx2, d.rdy2);
             rz = res[i] * cf[i];
             zq = cf[i] * q;
+            if (prev) {
+                const double sv = p[i] - prev[i];
+                const double ls = lap_diff(p, prev, i, d.pitch, d.rdx2, d.rdy2);
+                rs = res[i] * sv;
+                sq = sv * ls;
+                zs = cf[i] * ls;
+            }
         }
     }
     tile_sum(rz, sh, part, S_RZ);
     tile_sum(zq, sh, part, S_ZQ);
+    if (prev) {
+        tile_sum(rs, sh, part, S_RS);
+        tile_sum(sq, sh, part, S_SQ);
+        tile_sum(zs, sh, part, S_ZS);
+    }
 }
 
-// step 0: energy-norm line search; step 1: the network's own magnitude.
+// step 0: energy-norm line search; step 1: the network's own magnitude;
+// step 2: p + a z + b s with (a, b) minimising the energy norm of the error
+// over span{z, s} (s = change of p over the previous cycle; first = 1 on the
+// first cycle of a solve, which has no s).  With A = -Lap, symmetric positive
+// definite on the mean-free fluid functions, and r = b - A p the residual:
+//   [<z,Az> <z,As>] [a]   [<r,z>]
+//   [<z,As> <s,As>] [b] = [<r,s>]
+// falling back to the line search along z when z and s are nearly parallel.
 __global__ void __launch_bounds__(256)
 k_learn_alpha(const ChanDesc* __restrict__ cd, LearnChan* __restrict__ lc,
-              const double* __restrict__ part, int step, int f)
+              const double* __restrict__ part, int step, int f, int first)
 {
     __shared__ double sh[256];
     const int c = blockIdx.x;
     const ChanDesc& d = cd[c];
     const double rz = chan_total(d, part, S_RZ, sh);
     const double zq = chan_total(d, part, S_ZQ, sh);
+    double rs = 0.0, sq = 0.0, zs = 0.0;
+    if (step == 2 && !first) {
+        rs = chan_total(d, part, S_RS, sh);
+        sq = chan_total(d, part, S_SQ, sh);
+        zs = chan_total(d, part, S_ZS, sh);
+    }
     if (threadIdx.x) return;
     LearnChan& l = lc[c];
-    double a = 0.0;
+    double a = 0.0, b = 0.0;
     if (l.run && l.scale > 0.0) {
-        if (step == 1) a = (double)(f * f) * d.delx * d.dely / l.scale;
-        else if (zq < 0.0) a = -rz / zq;    // <z, Lap z> < 0 for any z not constant
+        if (step == 1) {
+            a = (double)(f * f) * d.delx * d.dely / l.scale;
+        } else if (zq < 0.0) {               // <z, Lap z> < 0 for any z not constant
+            a = -rz / zq;
+            if (step == 2 && !first && sq < 0.0) {
+                const double zaz = -zq, sas = -sq, zas = -zs;
+                const double det = zaz * sas - zas * zas;
+                if (det > 1e-10 * zaz * sas) {
+                    a = (rz * sas - zas * rs) / det;
+                    b = (zaz * rs - zas * rz) / det;
+                }
+            }
+        }
     }
     l.alpha = a;
+    l.beta = b;
 }
 
-// p += alpha * z at interior fluid cells.
+// p += alpha * z at interior fluid cells; with prev != nullptr (the
+// two-direction step) p += alpha * z + beta * (p - prev), and prev takes the
+// old p.
 __global__ void __launch_bounds__(TILE_THREADS)
 k_learn_apply(const ChanDesc* __restrict__ cd, const Tile* __restrict__ tiles,
               const LearnChan* __restrict__ lc, const uint8_t* __restrict__ flags,
-              const double* __restrict__ cf, double* __restrict__ p)
+              const double* __restrict__ cf, double* __restrict__ p, double* __restrict__ prev)
 {
     const Tile t = tiles[blockIdx.x];
     const ChanDesc& d = cd[t.chan];
     const double a = lc[t.chan].alpha;
-    if (a == 0.0) return;
     const int x = t.x0 + threadIdx.y, y = t.y0 + threadIdx.x;
+    if (prev) {
+        // two-direction step: every cell of a running channel also keeps
+        // the p it had (ghosts included) for the next cycle's s
+        if (!lc[t.chan].run || x >= d.nx || y >= d.ny) return;
+        const double b = lc[t.chan].beta;
+        const int64_t i = d.off + x * (int64_t)d.pitch + y;
+        const double po = p[i];
+        if (interior(d, x, y) && flag_fluid(flags[i])) p[i] = fma(b, po - prev[i], fma(a, cf[i], po));
+        prev[i] = po;
+        return;
+    }
+    if (a == 0.0) return;
     if (x >= d.nx || y >= d.ny || !interior(d, x, y)) return;
     const int64_t i = d.off + x * (int64_t)d.pitch + y;
     if (flag_fluid(flags[i])) p[i] = fma(a, cf[i], p[i]);
@@ -532,16 +599,17 @@ void launch_learn_prolong(hipStream_t st, int ntiles, const ChanDesc* cd, const 
 }
 void launch_learn_dots(hipStream_t st, int ntiles, const ChanDesc* cd, const Tile* tiles,
                        const LearnChan* lc, const uint8_t* flags, const double* res,
-                       const double* cf, double* part) {
-    k_learn_dots<<<ntiles, TB, 0, st>>>(cd, tiles, lc, flags, res, cf, part);
+                       const double* cf, const double* p, const double* prev, double* part) {
+    k_learn_dots<<<ntiles, TB, 0, st>>>(cd, tiles, lc, flags, res, cf, p, prev, part);
 }
 void launch_learn_alpha(hipStream_t st, int nch, const ChanDesc* cd, LearnChan* lc,
-                        const double* part, int step, int f) {
-    k_learn_alpha<<<nch, 256, 0, st>>>(cd, lc, part, step, f);
+                        const double* part, int step, int f, int first) {
+    k_learn_alpha<<<nch, 256, 0, st>>>(cd, lc, part, step, f, first);
 }
 void launch_learn_apply(hipStream_t st, int ntiles, const ChanDesc* cd, const Tile* tiles,
-                        const LearnChan* lc, const uint8_t* flags, const double* cf, double* p) {
-    k_learn_apply<<<ntiles, TB, 0, st>>>(cd, tiles, lc, flags, cf, p);
+                        const LearnChan* lc, const uint8_t* flags, const double* cf, double* p,
+                        double* prev) {
+    k_learn_apply<<<ntiles, TB, 0, st>>>(cd, tiles, lc, flags, cf, p, prev);
 }
 void launch_learn_arm(hipStream_t st, int nch, const ChanDesc* cd, ChanState* cs,
                       const LearnChan* lc, int32_t* active, int budget) {

@@ -67,6 +67,7 @@ struct ChanState {
 struct LearnChan {
     double scale;                // network input scale: 1 / rms of the coarse residual
     double alpha;                // step length of the network's correction
+    double beta;                 // two-direction step: length along the previous change
     double dmean;                // capture: mean of the pressure change
     double cscale[8];            // capture: `scale` of each stored residual (LEARN_POINTS)
     int32_t run;                 // the channel has not converged yet

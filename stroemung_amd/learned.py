@@ -169,7 +169,7 @@ def _quiet(*_):
 
 def train_net(grids, coarsen=4, capture_ticks=96, points=4, capture_sweeps=32, steps=3000,
               batch=32, lr=1e-3, dagger_rounds=2, dagger_ticks=32, dagger_steps=1500,
-              cycles=16, sweeps=32, seed=1, log=_quiet):
+              cycles=16, sweeps=32, step="energy", seed=1, log=_quiet):
     """Train a pressure network on pairs the solver produces on `grids`:
     `capture_ticks` plain ticks storing `points` pairs per channel, `steps`
     Adam steps, then `dagger_rounds` rounds of learned ticks that store pairs
@@ -205,7 +205,7 @@ def train_net(grids, coarsen=4, capture_ticks=96, points=4, capture_sweeps=32, s
         log("trained %d steps, loss %.4f" % (steps, loss))
         rounds = []
         for r in range(dagger_rounds):
-            ctx.set_learned(net, cycles=cycles, sweeps=sweeps, step="energy", coarsen=coarsen)
+            ctx.set_learned(net, cycles=cycles, sweeps=sweeps, step=step, coarsen=coarsen)
             ctx.set_capture(net, points=points, sweeps=capture_sweeps, coarsen=coarsen)
             sw = []
             for _ in range(dagger_ticks):
@@ -253,7 +253,7 @@ def _run_ticks(ctx, ticks, cells):
 
 
 def compare(grids, params, coarsen=4, cycles=16, sweeps=32, warm=20, ticks=10, graphs=True,
-            log=_quiet):
+            step="energy", log=_quiet):
     """Plain red-black solve vs the learned solve on `grids`, each from the
     same state after `warm` plain ticks (two contexts, same deterministic
     warm-up).  Every channel of both runs ends each solve at its residual
@@ -280,7 +280,7 @@ def compare(grids, params, coarsen=4, cycles=16, sweeps=32, warm=20, ticks=10, g
         for _ in range(warm):
             ctx.tick()
         ctx.autotune(8)
-        ctx.set_learned(net, cycles=cycles, sweeps=sweeps, step="energy", coarsen=coarsen)
+        ctx.set_learned(net, cycles=cycles, sweeps=sweeps, step=step, coarsen=coarsen)
         ctx.set_graphs(graphs)
         el, sw, ok = _run_ticks(ctx, ticks, cells)
         st = net.stats()
@@ -292,7 +292,8 @@ def compare(grids, params, coarsen=4, cycles=16, sweeps=32, warm=20, ticks=10, g
     out["learned"] = {
         "cell_updates_per_s": sum(cells) * ticks / el, "ms_per_tick": el * 1e3 / ticks,
         "sweeps_per_solve": sw / ticks / len(grids), "residual_bound_met": ok,
-        "cycles": cycles, "sweeps_per_cycle": sweeps, "coarsen": coarsen, "frame": [H, W],
+        "cycles": cycles, "sweeps_per_cycle": sweeps, "step": step, "coarsen": coarsen,
+        "frame": [H, W],
         "graphs": graphs,
         # forward passes timed with events (with graphs: the first cycle of each solve)
         "forward_ms_per_call": st["forward_ms"] / st["forward_calls"] if st["forward_calls"] else None,

@@ -185,8 +185,9 @@ class Context:
     def set_learned(self, net, cycles=1, sweeps=32, step="network", coarsen=1):
         """Two-grid cycles with the network before every red-black solve (net
         None: off).  step: "network" takes the network's magnitude, "energy"
-        a line search; coarsen: 1, 2, 4 or 8."""
-        rule = {"energy": 0, "network": 1}[step]
+        a line search, "momentum" the energy minimum over the correction and
+        the previous cycle's change of p; coarsen: 1, 2, 4 or 8."""
+        rule = {"energy": 0, "network": 1, "momentum": 2}[step]
         self._check(lib().stm_set_learned(self._ctx, net._net if net else None, cycles, sweeps,
                                           rule, coarsen))
 

@@ -310,7 +310,8 @@ def test_capture_pairs_follow_the_restriction(f):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("f,step", [(1, "network"), (4, "network"), (4, "energy")])
+@pytest.mark.parametrize("f,step", [(1, "network"), (4, "network"), (4, "energy"),
+                                    (4, "momentum")])
 def test_capture_train_and_learned_solve_reach_the_same_residual(f, step):
     from stroemung_amd.learned import PressureNet
     n, count = 64, 4
@@ -354,7 +355,8 @@ def test_capture_train_and_learned_solve_reach_the_same_residual(f, step):
 
 
 @pytest.mark.gpu
-def test_graph_replayed_cycles_are_bit_identical_to_launches():
+@pytest.mark.parametrize("step", ["energy", "momentum"])
+def test_graph_replayed_cycles_are_bit_identical_to_launches(step):
     """Learned cycles replayed from a captured hipGraph give the same fields,
     bit for bit, as the same cycles launched one by one -- over several ticks,
     each replaying the graph several times between host synchronisations."""
@@ -366,7 +368,7 @@ def test_graph_replayed_cycles_are_bit_identical_to_launches():
         net = PressureNet(ctx, n // f, n // f, count, seed=32)
         for _ in range(2):
             ctx.tick()
-        ctx.set_learned(net, cycles=6, sweeps=8, step="energy", coarsen=f)
+        ctx.set_learned(net, cycles=6, sweeps=8, step=step, coarsen=f)
         ctx.set_graphs(graphs)
         its = [ctx.tick()[0] for _ in range(4)]
         fields = [ctx.download(c, ("u", "v", "p")) for c in range(count)]

@@ -177,6 +177,11 @@ int stm_solver_variant(stm_ctx *ctx, int32_t variant, int32_t *current, int32_t 
 /* Times `launches` launches of each permitted variant on the current fields
  * (restored afterwards) and keeps the fastest per sweep. */
 int stm_autotune(stm_ctx *ctx, uint32_t launches, int32_t *chosen, double *us_per_sweep);
+/* Runs one real tick per permitted variant (the simulation advances by that
+ * many ticks) and keeps the variant with the most interior cell-sweeps per
+ * second of the whole tick -- for batches of uneven grids, whose solves spend
+ * most of their time with few channels still running. */
+int stm_autotune_ticks(stm_ctx *ctx, int32_t *chosen, double *cell_sweeps_per_s);
 
 /* ---- measurement (no reference counterpart) */
 int stm_synchronize(stm_ctx *ctx);

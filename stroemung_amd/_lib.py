@@ -100,6 +100,7 @@ SIGNATURES = {
     "stm_rb_sweeps": (C.c_int, [_CTX, C.c_uint32]),
     "stm_solver_variant": (C.c_int, [_CTX, C.c_int32] + [C.POINTER(C.c_int32)] * 4),
     "stm_autotune": (C.c_int, [_CTX, C.c_uint32, C.POINTER(C.c_int32), _D]),
+    "stm_autotune_ticks": (C.c_int, [_CTX, C.POINTER(C.c_int32), _D]),
     "stm_synchronize": (C.c_int, [_CTX]),
     "stm_stream_copy": (C.c_int, [_CTX, C.c_uint64, C.c_uint32, _D]),
     "stm_profile": (C.c_int, [_CTX, C.c_uint32]),

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -1386,6 +1387,44 @@ int stm_autotune(stm_ctx* ctx, uint32_t launches, int32_t* chosen, double* us_pe
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     if (chosen) *chosen = best;
     if (us_per_sweep) *us_per_sweep = best_us;
+    return STM_OK;
+}
+
+// One real tick per permitted variant, scored by interior cell-sweeps per
+// second of wall time over the whole tick; keeps the best.  Unlike
+// stm_autotune this sees every phase of a solve, including the ones where
+// few channels of an uneven batch are still running.
+int stm_autotune_ticks(stm_ctx* ctx, int32_t* chosen, double* cell_sweeps_per_s) {
+    int rc = ready(ctx);
+    if (rc) return rc;
+    if (!ctx->any_rb) return fail(ctx, STM_ERR_STATE, "no red-black channel to tune");
+    std::vector<uint32_t> iters(nch(ctx));
+    int best = -1;
+    double best_rate = -1.0;
+    for (int v = 0; v < N_TB_VARIANTS; v++) {
+        if (!variant_allowed(ctx, v)) continue;
+        if ((rc = plan_tb(ctx, v))) return rc;
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        const auto t0 = std::chrono::steady_clock::now();
+        if ((rc = stm_run_simulation_tick(ctx, iters.data(), nullptr))) return rc;
+        const double sec =
+            std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        double work = 0.0;
+        for (int k = 0; k < nch(ctx); k++) {
+            const HostChan& c = ctx->chans[k];
+            work += (double)(c.nx - 2) * (double)(c.ny - 2) * iters[k];
+        }
+        const double rate = sec > 0.0 ? work / sec : 0.0;
+        if (rate > best_rate) {
+            best_rate = rate;
+            best = v;
+        }
+    }
+    if (best < 0) return fail(ctx, STM_ERR_STATE, "no permitted variant");
+    if ((rc = plan_tb(ctx, best))) return rc;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (chosen) *chosen = best;
+    if (cell_sweeps_per_s) *cell_sweeps_per_s = best_rate;
     return STM_OK;
 }
 

@@ -163,10 +163,22 @@ class Context:
         return g.value
 
     def autotune(self, launches=8):
+        """Times `launches` blocked launches of each variant on the current
+        fields (restored afterwards); keeps the fastest per sweep.  Returns
+        (variant, us per sweep)."""
         chosen = C.c_int32()
         us = C.c_double()
         self._check(lib().stm_autotune(self._ctx, launches, C.byref(chosen), C.byref(us)))
         return chosen.value, us.value
+
+    def autotune_ticks(self):
+        """One real tick per variant (the simulation advances), scored by
+        interior cell-sweeps per second of the whole tick; keeps the best.
+        Returns (variant, cell-sweeps/s)."""
+        chosen = C.c_int32()
+        rate = C.c_double()
+        self._check(lib().stm_autotune_ticks(self._ctx, C.byref(chosen), C.byref(rate)))
+        return chosen.value, rate.value
 
     def profile(self, sample):
         self._check(lib().stm_profile(self._ctx, sample))

@@ -264,6 +264,30 @@ def test_batch_equals_single_channels():
             assert_bits_equal(a[n], b[n], "channel %d %s" % (k, n))
 
 
+def test_whole_tick_autotune_picks_a_permitted_variant_and_keeps_the_bound():
+    say("test: whole-tick autotune on an uneven batch")
+    from stroemung_amd import workloads
+    ws = workloads.random_channels(count=6, seed=3, imax=(24, 64), jmax=(16, 32), eps=1e-5)
+    ctx = Context(0)
+    for ct, vel, prm in ws:
+        ctx.add_grid(prm, ct, vel)
+    ctx.construct()
+    ctx.tick()
+    t0 = [ctx.state(c)["time"] for c in range(len(ws))]
+    v, rate = ctx.autotune_ticks()
+    assert 0 <= v < 8 and rate > 0
+    assert ctx.solver_variant()[0] == v
+    for c in range(len(ws)):   # one real tick per variant
+        assert ctx.state(c)["time"] > t0[c]
+    for _ in range(2):
+        its, _ = ctx.tick()
+        for c in range(len(ws)):
+            s = ctx.state(c)
+            assert s["residual_squared"] <= s["tolerance_squared"] and not s["exhausted"]
+            assert s["sor_iterations"] == its[c]
+    ctx.close()
+
+
 # ---------------------------------------------- trajectories at equal residual
 
 def _rel(a, b, mask):

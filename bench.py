@@ -11,6 +11,7 @@ RHS, pressure solve to the residual, velocity update, next step size).
     python bench.py --workload channels           # 256 uneven channels, plain
     python bench.py --workload learned_cavities   # 32 cavities 256x256: plain vs learned
     python bench.py --workload learned_channels   # 64 channels 512x256: plain vs learned
+    python bench.py --workload learned_channels_500   # the same, 500 learned steps from rest
 
 Prints one JSON line.  `value` = interior cells * K / wall seconds of the K
 timed ticks, inputs resident in HBM.  The roofline entry is for the dominant
@@ -57,7 +58,8 @@ def parse():
     ap.add_argument("--profile", action="store_true",
                     help="short fixed run for hardware counters, no JSON line")
     ap.add_argument("--workload", choices=("cavity", "channels", "learned_cavities",
-                                           "learned_channels", "net"), default="cavity",
+                                           "learned_channels", "learned_channels_500", "net"),
+                    default="cavity",
                     help="cavity: the 1024x1024 lid cavity (default); channels: 256 "
                          "independent channels of uneven size, seed 0; learned_*: plain vs "
                          "learned solve")
@@ -249,6 +251,48 @@ def learned_channels(ticks, log=None):
     return res
 
 
+def learned_channels_500(log=None):
+    """Config-5 trajectory: the 64 channels 512 x 256 of seed 1 advanced 500
+    learned ticks from rest, with plain ticks timed on the same state at
+    checkpoints; network trained as for learned_channels."""
+    from stroemung_amd import learned, workloads
+    log = log or (lambda *x: None)
+    train = workloads.fixed_channels(64, seed=1001, eps=1e-7)
+    params, info = learned.train_net(train, coarsen=8, capture_ticks=40, points=4,
+                                     capture_sweeps=32, steps=3000, batch=32, dagger_rounds=2,
+                                     dagger_ticks=24, dagger_steps=1500, cycles=64, sweeps=32,
+                                     step="momentum", log=log)
+    test = workloads.fixed_channels(64, seed=1, eps=1e-7)
+    res = learned.trajectory(test, params, coarsen=8, cycles=64, sweeps=32, step="momentum",
+                             ticks=500, log=log)
+    res["training"] = info
+    return res
+
+
+def learned_500_line(a):
+    log = lambda *x: print(*x, file=sys.stderr, flush=True)  # noqa: E731
+    res = learned_channels_500(log)
+    lr = res["learned"]
+    out = {
+        "metric": METRIC,
+        "value": lr["cell_updates_per_s"],
+        "unit": "cell-updates/s",
+        "n_gpus": 1, "steps": res["ticks"], "warmup": 1,
+        "ms_per_step": lr["ms_per_tick"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 (solve), bf16 (network)", "data": "synthetic",
+        "config": {"workload": "64 channels 512x256 with 1-4 random obstacles, Re 100-1000, "
+                               "inflow u=1, red-black SOR omega=1.7 to relative residual 1e-7, "
+                               "tau=0.5, fp64, seed 1: 500 learned steps from rest, plain "
+                               "steps timed on the same state at checkpoints; trained on "
+                               "seed 1001",
+                   "learned": lr, "checkpoints": res["checkpoints"],
+                   "training": res["training"]},
+        "roofline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def net_line(a):
     """The U-Net's forward and training step alone, on the frames the learned
     solves use: TFLOP/s against the BF16 dense MFMA peak."""
@@ -321,6 +365,9 @@ def main():
         return
     if a.workload == "channels":
         channels_run(a)
+        return
+    if a.workload == "learned_channels_500":
+        learned_500_line(a)
         return
     if a.workload.startswith("learned"):
         learned_line(a)

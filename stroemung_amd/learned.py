@@ -305,3 +305,64 @@ def compare(grids, params, coarsen=4, cycles=16, sweeps=32, warm=20, ticks=10, g
                                                              sw / ticks / len(grids),
                                                              out["speedup"]))
     return out
+
+
+def trajectory(grids, params, coarsen=8, cycles=64, sweeps=32, step="momentum", ticks=500,
+               checkpoints=(10, 50, 100, 200, 300, 400, 500), log=_quiet):
+    """A long learned run: `ticks` learned ticks from rest (after one plain
+    tick and the autotuner), each timed; after the learned tick numbered in
+    `checkpoints`, one plain tick on the same state is timed as well (it
+    advances the simulation like any tick).  Every learned tick's channels
+    end at their residual bound.  Returns the learned totals and, per
+    checkpoint, learned (mean of the last 10 ticks) vs plain."""
+    cells = [(ct.shape[0] - 2) * (ct.shape[1] - 2) for ct, _, _ in grids]
+    H, W = frame_for(grids, coarsen)
+    ctx = make_context(grids)
+    net = PressureNet(ctx, H, W, len(grids), seed=1)
+    marks = []
+    try:
+        net.set_params(params)
+        ctx.tick()
+        ctx.autotune(8)
+        ctx.set_graphs(True)
+        ctx.set_learned(net, cycles=cycles, sweeps=sweeps, step=step, coarsen=coarsen)
+        ctx.synchronize()
+        ms, sw, bound = [], [], True
+        for k in range(1, ticks + 1):
+            t = time.perf_counter()
+            its, _ = ctx.tick()
+            ms.append((time.perf_counter() - t) * 1e3)
+            sw.append(float(np.mean(its)))
+            for c in range(len(grids)):
+                st = ctx.state(c)
+                bound = bound and st["residual_squared"] <= st["tolerance_squared"] \
+                    and not st["exhausted"]
+            if k in checkpoints:
+                ctx.set_learned(None)
+                t = time.perf_counter()
+                pits, _ = ctx.tick()
+                pm = (time.perf_counter() - t) * 1e3
+                ctx.set_learned(net, cycles=cycles, sweeps=sweeps, step=step, coarsen=coarsen)
+                lm = float(np.mean(ms[-10:]))
+                marks.append({"tick": k, "time": ctx.state(0)["time"],
+                              "learned_ms_per_tick": lm,
+                              "learned_sweeps_per_solve": float(np.mean(sw[-10:])),
+                              "plain_ms_per_tick": pm,
+                              "plain_sweeps_per_solve": float(np.mean(pits)),
+                              "plain_over_learned": pm / lm})
+                log("tick %d: learned %.1f ms (%.0f sweeps), plain %.0f ms (%.0f sweeps), %.1fx"
+                    % (k, lm, marks[-1]["learned_sweeps_per_solve"], pm,
+                       marks[-1]["plain_sweeps_per_solve"], pm / lm))
+        ctx.set_learned(None)
+    finally:
+        net.close()
+        ctx.close()
+    wall = sum(ms) * 1e-3
+    return {"ticks": ticks, "channels": len(grids), "interior_cells": int(sum(cells)),
+            "learned": {"wall_s": wall, "ms_per_tick": wall * 1e3 / ticks,
+                        "cell_updates_per_s": sum(cells) * ticks / wall,
+                        "sweeps_per_solve": float(np.mean(sw)), "residual_bound_met": bound,
+                        "cycles": cycles, "sweeps_per_cycle": sweeps, "step": step,
+                        "coarsen": coarsen, "frame": [H, W], "graphs": True},
+            "checkpoints": marks}
+

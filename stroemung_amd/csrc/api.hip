@@ -79,6 +79,14 @@ struct stm_ctx {
     double* d_tbpart = nullptr;    // blocked solver: residual partials, two launches
     TbTile* d_tb = nullptr;
     int ntb = 0, tb_capacity = 0, part_capacity = 0;
+    std::vector<TbTile> h_tb;      // host copy of the full list
+    // the list launches use: d_tb, or during a polled solve the tiles of the
+    // channels still running (d_tbc, staged through pinned h_tbc)
+    const TbTile* tb_cur = nullptr;
+    int ntb_cur = 0;
+    TbTile* d_tbc = nullptr;
+    TbTile* h_tbc = nullptr;
+    int tbc_capacity = 0;
     int variant = -1;              // blocked-solver variant in use
     ChanState* h_cs = nullptr;     // pinned
     int be_capacity = 0;
@@ -368,7 +376,8 @@ int plan_tb(stm_ctx* ctx, int variant) {
         c.tb0 = (int32_t)tl.size();
         for (auto xs : split((int)c.nx, v.oxm))
             for (auto ys : split_even((int)c.ny, v.oym))
-                tl.push_back(TbTile{(int32_t)k, xs.first, ys.first, xs.second, ys.second, 0, 0, 0});
+                tl.push_back(TbTile{(int32_t)k, xs.first, ys.first, xs.second, ys.second,
+                                    (int32_t)tl.size(), 0, 0});
         c.ntb = (int32_t)tl.size() - c.tb0;
     }
     if ((int)tl.size() > ctx->tb_capacity) {
@@ -380,6 +389,9 @@ int plan_tb(stm_ctx* ctx, int variant) {
     HIPCHK(ctx, hipMemcpyAsync(ctx->d_tb, tl.data(), sizeof(TbTile) * tl.size(),
                                hipMemcpyHostToDevice, ctx->stream));
     ctx->ntb = (int)tl.size();
+    ctx->h_tb = tl;
+    ctx->tb_cur = ctx->d_tb;
+    ctx->ntb_cur = ctx->ntb;
     ctx->variant = variant;
     return upload_descs(ctx);
 }
@@ -572,12 +584,45 @@ int poll_active(stm_ctx* ctx, bool* any) {
     return STM_OK;
 }
 
+// After a poll: when the channels still running hold under 7/8 of the tiles
+// launched so far, later launches cover only their tiles.  A channel that
+// stopped has already taken its stop decision (that is what the poll read),
+// and every tile of a running channel is kept, so each launch still writes
+// the partials its successors read.  The pinned staging buffer is rewritten
+// only after the next poll, when this copy has completed.
+int compact_tiles(stm_ctx* ctx) {
+    int n = 0;
+    for (int k = 0; k < nch(ctx); k++)
+        if (ctx->h_active[k]) n += ctx->chans[k].ntb;
+    if (n == 0 || 8 * n >= 7 * ctx->ntb_cur) return STM_OK;
+    if (ctx->tbc_capacity < ctx->ntb) {
+        if (ctx->d_tbc) HIPCHK(ctx, hipFree(ctx->d_tbc));
+        if (ctx->h_tbc) HIPCHK(ctx, hipHostFree(ctx->h_tbc));
+        ctx->d_tbc = ctx->h_tbc = nullptr;
+        ctx->tbc_capacity = 0;
+        HIPCHK(ctx, hipMalloc(&ctx->d_tbc, sizeof(TbTile) * ctx->ntb));
+        HIPCHK(ctx, hipHostMalloc(&ctx->h_tbc, sizeof(TbTile) * ctx->ntb, hipHostMallocDefault));
+        ctx->tbc_capacity = ctx->ntb;
+    }
+    int m = 0;
+    for (int k = 0; k < nch(ctx); k++) {
+        if (!ctx->h_active[k]) continue;
+        const HostChan& c = ctx->chans[k];
+        for (int i = c.tb0; i < c.tb0 + c.ntb; i++) ctx->h_tbc[m++] = ctx->h_tb[i];
+    }
+    HIPCHK(ctx, hipMemcpyAsync(ctx->d_tbc, ctx->h_tbc, sizeof(TbTile) * m, hipMemcpyHostToDevice,
+                               ctx->stream));
+    ctx->tb_cur = ctx->d_tbc;
+    ctx->ntb_cur = m;
+    return STM_OK;
+}
+
 // Launch L of the blocked solver on the current variant.
 void launch_blocked(stm_ctx* ctx, int variant, int L, int mode) {
     const int cap = ctx->part_capacity;
     double* prev = ctx->d_tbpart + (size_t)((L + 1) & 1) * cap;
     double* cur = ctx->d_tbpart + (size_t)(L & 1) * cap;
-    launch_rb_tb(ctx->stream, variant, ctx->ntb, ctx->d_cd, ctx->d_cs, ctx->d_tb, ctx->flags,
+    launch_rb_tb(ctx->stream, variant, ctx->ntb_cur, ctx->d_cd, ctx->d_cs, ctx->tb_cur, ctx->flags,
                  ctx->rhs, ctx->p, ctx->p2, prev, cur, ctx->d_active, L, mode);
 }
 
@@ -588,6 +633,8 @@ void launch_blocked(stm_ctx* ctx, int variant, int L, int mode) {
 // once.  check = 0 runs a fixed number of sweeps without the residual.
 int blocked_solve(stm_ctx* ctx, int check) {
     int batch = 2, L = 0;
+    ctx->tb_cur = ctx->d_tb;
+    ctx->ntb_cur = ctx->ntb;
     for (;;) {
         for (int b = 0; b < batch; b++, L++)
             LAUNCH("rb_tb", launch_blocked(ctx, ctx->variant, L, check ? TB_CHECK : 0));
@@ -595,8 +642,11 @@ int blocked_solve(stm_ctx* ctx, int check) {
         int rc = poll_active(ctx, &any);
         if (rc) return rc;
         if (!any) break;
+        if ((rc = compact_tiles(ctx))) return rc;
         if (batch < 64) batch *= 2;
     }
+    ctx->tb_cur = ctx->d_tb;
+    ctx->ntb_cur = ctx->ntb;
     LAUNCH("pbuf_fix", launch_pbuf_fix(ctx->stream, ctx->ntiles, nch(ctx), ctx->d_cd, ctx->d_cs,
                                        ctx->d_tiles, ctx->p2, ctx->p));
     return STM_OK;
@@ -1010,7 +1060,7 @@ void stm_destroy(stm_ctx* ctx) {
     for (hipEvent_t e : ctx->free_events) hipEventDestroy(e);
     void* bufs[] = {ctx->u, ctx->v, ctx->p, ctx->f, ctx->g, ctx->rhs, ctx->scratch, ctx->p2,
                     ctx->flags, ctx->d_cd, ctx->d_cs, ctx->d_be, ctx->d_restore, ctx->d_tiles,
-                    ctx->d_part, ctx->d_active, ctx->d_tbpart, ctx->d_tb, ctx->d_lc, ctx->lres,
+                    ctx->d_part, ctx->d_active, ctx->d_tbpart, ctx->d_tb, ctx->d_tbc, ctx->d_lc, ctx->lres,
                     ctx->lcf, ctx->lprev, ctx->lsnap, ctx->lrc, ctx->lmc, ctx->d_cpart, ctx->d_map,
                     ctx->d_inv, ctx->d_live, ctx->d_rgba};
     for (void* b : bufs)
@@ -1018,6 +1068,7 @@ void stm_destroy(stm_ctx* ctx) {
     if (ctx->h_active) hipHostFree(ctx->h_active);
     if (ctx->h_cs) hipHostFree(ctx->h_cs);
     if (ctx->h_lc) hipHostFree(ctx->h_lc);
+    if (ctx->h_tbc) hipHostFree(ctx->h_tbc);
     if (ctx->h_live_ring) hipHostFree(ctx->h_live_ring);
     for (hipEvent_t e : ctx->poll_evs)
         if (e) hipEventDestroy(e);

@@ -295,7 +295,7 @@ k_rb_strip(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
         const double res2 = block_sum<NT>(acc, wsum);
         const bool conv = check && res2 <= s.tol2;
         if (conv || done >= itermax) {
-            if (blockIdx.x == (unsigned)d.tb0 && tid == 0) {
+            if (t.idx == d.tb0 && tid == 0) {
                 s.stop_launch = L - 1;
                 s.sor_iters = (uint32_t)done;
                 s.pbuf = L & 1;
@@ -325,7 +325,7 @@ k_rb_strip(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
         else if (o1) pout[g + 1] = pv[r][1];
     }
     const double b = block_sum<NT>(acc, wsum);
-    if (tid == 0) part_cur[blockIdx.x] = b;
+    if (tid == 0) part_cur[t.idx] = b;
 }
 
 // After a blocked solve: channels whose latest p is in the second buffer

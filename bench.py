@@ -12,6 +12,7 @@ RHS, pressure solve to the residual, velocity update, next step size).
     python bench.py --workload learned_cavities   # 32 cavities 256x256: plain vs learned
     python bench.py --workload learned_channels   # 64 channels 512x256: plain vs learned
     python bench.py --workload learned_channels_500   # the same, 500 learned steps from rest
+    python bench.py --workload learned_uneven     # the 256 uneven channels: plain vs learned
 
 Prints one JSON line.  `value` = interior cells * K / wall seconds of the K
 timed ticks, inputs resident in HBM.  The roofline entry is for the dominant
@@ -58,7 +59,8 @@ def parse():
     ap.add_argument("--profile", action="store_true",
                     help="short fixed run for hardware counters, no JSON line")
     ap.add_argument("--workload", choices=("cavity", "channels", "learned_cavities",
-                                           "learned_channels", "learned_channels_500", "net"),
+                                           "learned_channels", "learned_channels_500",
+                                           "learned_uneven", "net"),
                     default="cavity",
                     help="cavity: the 1024x1024 lid cavity (default); channels: 256 "
                          "independent channels of uneven size, seed 0; learned_*: plain vs "
@@ -251,6 +253,28 @@ def learned_channels(ticks, log=None):
     return res
 
 
+def learned_uneven(ticks, log=None):
+    """The 256 uneven channels of the batched config (64x32 to 512x256, seed
+    0, eps 1e-5): network trained on the 256 channels of seed 1001, one frame
+    of 64 x 32 at coarsening 8 for every size; both runs whole-tick tuned."""
+    from stroemung_amd import learned, workloads
+    log = log or (lambda *x: None)
+    train = workloads.random_channels(count=256, seed=1001, eps=1e-5)
+    params, info = learned.train_net(train, coarsen=8, capture_ticks=24, points=4,
+                                     capture_sweeps=32, steps=3000, batch=32, dagger_rounds=2,
+                                     dagger_ticks=12, dagger_steps=1500, cycles=64, sweeps=32,
+                                     step="momentum", log=log)
+    test = workloads.random_channels(count=256, seed=0, eps=1e-5)
+    res = learned.compare(test, params, coarsen=8, cycles=64, sweeps=32, warm=1, ticks=ticks,
+                          step="momentum", tune="ticks", log=log)
+    res["training"] = info
+    res["workload"] = ("256 channels, imax 64-512 x jmax 32-256 log-uniform, 1-4 random "
+                       "obstacles, Re 50-1000, inflow u=1, red-black SOR omega=1.7 to relative "
+                       "residual 1e-5, tau=0.5, fp64, seed 0; trained on the 256 channels of "
+                       "seed 1001; measured after 1 plain tick and 8 tuning ticks")
+    return res
+
+
 def learned_channels_500(log=None):
     """Config-5 trajectory: the 64 channels 512 x 256 of seed 1 advanced 500
     learned ticks from rest, with plain ticks timed on the same state at
@@ -330,6 +354,8 @@ def learned_line(a):
     log = lambda *x: print(*x, file=sys.stderr, flush=True)  # noqa: E731
     if a.workload == "learned_cavities":
         res = learned_cavities(a.steps, log)
+    elif a.workload == "learned_uneven":
+        res = learned_uneven(a.steps, log)
     else:
         res = learned_channels(a.steps, log)
     lr = res["learned"]

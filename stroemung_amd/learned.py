@@ -253,19 +253,26 @@ def _run_ticks(ctx, ticks, cells):
 
 
 def compare(grids, params, coarsen=4, cycles=16, sweeps=32, warm=20, ticks=10, graphs=True,
-            step="energy", log=_quiet):
+            step="energy", tune="launches", log=_quiet):
     """Plain red-black solve vs the learned solve on `grids`, each from the
-    same state after `warm` plain ticks (two contexts, same deterministic
-    warm-up).  Every channel of both runs ends each solve at its residual
-    bound.  Returns a dict of throughputs."""
+    same state after `warm` plain ticks and the same tuning (two contexts,
+    same deterministic warm-up; tune = "ticks" uses the whole-tick tuner,
+    whose ticks are real, in both).  Every channel of both runs ends each
+    solve at its residual bound.  Returns a dict of throughputs."""
     cells = [(ct.shape[0] - 2) * (ct.shape[1] - 2) for ct, _, _ in grids]
     H, W = frame_for(grids, coarsen)
     out = {"channels": len(grids), "interior_cells": int(sum(cells)), "ticks": ticks,
            "warm": warm}
+    def tuned(ctx):
+        if tune == "ticks":
+            ctx.autotune_ticks()
+        else:
+            ctx.autotune(8)
+
     ctx = make_context(grids)
     for _ in range(warm):
         ctx.tick()
-    ctx.autotune(8)
+    tuned(ctx)
     el, sw, ok = _run_ticks(ctx, ticks, cells)
     variant = ctx.solver_variant()
     ctx.close()
@@ -279,7 +286,7 @@ def compare(grids, params, coarsen=4, cycles=16, sweeps=32, warm=20, ticks=10, g
         net.set_params(params)
         for _ in range(warm):
             ctx.tick()
-        ctx.autotune(8)
+        tuned(ctx)
         ctx.set_learned(net, cycles=cycles, sweeps=sweeps, step=step, coarsen=coarsen)
         ctx.set_graphs(graphs)
         el, sw, ok = _run_ticks(ctx, ticks, cells)

@@ -402,3 +402,23 @@ def test_trained_network_saves_sweeps_on_unseen_layouts():
                                                     res["learned"]["sweeps_per_solve"]))
     assert res["plain"]["residual_bound_met"] and res["learned"]["residual_bound_met"]
     assert res["learned"]["sweeps_per_solve"] < 0.5 * res["plain"]["sweeps_per_solve"]
+
+
+@pytest.mark.gpu
+def test_trajectory_runs_learned_ticks_with_plain_checkpoints():
+    """The long-run helper: learned ticks from rest, each at its residual
+    bound, with plain ticks timed on the same state at the checkpoints."""
+    from stroemung_amd import learned, workloads
+    grids = workloads.fixed_channels(4, seed=5, imax=64, jmax=64, eps=1e-6)
+    ctx = learned.make_context(grids)
+    net = learned.PressureNet(ctx, 16, 16, 4, seed=3)
+    params = net.get_params()
+    net.close()
+    ctx.close()
+    res = learned.trajectory(grids, params, coarsen=4, cycles=4, sweeps=8, ticks=12,
+                             checkpoints=(5, 12))
+    assert res["learned"]["residual_bound_met"]
+    assert [m["tick"] for m in res["checkpoints"]] == [5, 12]
+    for m in res["checkpoints"]:
+        assert m["plain_ms_per_tick"] > 0 and m["learned_ms_per_tick"] > 0
+        assert m["plain_sweeps_per_solve"] > 0

@@ -126,12 +126,14 @@ k_conv3x3(const bf16* __restrict__ in, int cin, const bf16* __restrict__ w,
 // concatenation).  Pooling, upsampling and concatenation thus happen while
 // staging, without materialising their outputs.
 //
-// The patch is double-buffered: the global loads of slice s + 1 are issued
-// into registers before the MFMAs of slice s, and stored to the other buffer
-// after them, so one barrier per slice suffices and the load latency hides
-// behind the matrix work.  Every thread issues all its loads of a slice
-// before its first LDS store.
-template <int TW, int TH, int WN>
+// With PIPE (chosen for workgroups with at least three input slices) the
+// patch is double-buffered: the global loads of slice s + 1 are issued into
+// registers before the MFMAs of slice s, and stored to the other buffer after
+// them, so one barrier per slice suffices and the load latency hides behind
+// the matrix work; that costs registers, so layers with fewer slices (the
+// large shallow ones, where occupancy hides latency instead) stage each slice
+// and then multiply.
+template <int TW, int TH, int WN, bool POOL, bool PIPE>
 __global__ void __launch_bounds__(256)
 k_conv3x3_tile(ConvIn src, const bf16* __restrict__ w, const float* __restrict__ bias,
                bf16* __restrict__ out, float* __restrict__ out32, int cout, int nb, int H, int W,
@@ -175,75 +177,118 @@ k_conv3x3_tile(ConvIn src, const bf16* __restrict__ w, const float* __restrict__
     const int nsl = cin / 32, z = blockIdx.z, nz = gridDim.z;
     const int sl0 = z * nsl / nz, sl1 = (z + 1) * nsl / nz;
 
-    // the NV staged vectors of this thread for the slice at channel c0
-    bf16x8 xs[NV];
+    // Staging: the loads of a slice are issued unconditionally (addresses
+    // clamped into the image, out-of-image vectors zeroed when stored), with
+    // no branch between them, so the compiler can count them exactly and the
+    // MFMAs wait only for their weights; a pooled input (POOL, src.mode ==
+    // IN_POOL, never with a skip part) keeps its four window values in
+    // registers and takes their max when storing.
+    constexpr bool pool = POOL;
+    constexpr int NR = POOL ? 4 : 1;
+    bf16x8 raw[NV][NR];
+    uint32_t valid = 0;
     auto fetch = [&](int c0) {
         const bool first = c0 < src.ca;
-        const int mode = first ? src.mode : IN_DIRECT;
         const bf16* base = first ? src.a : src.b;
         const int cs = first ? src.ca : src.cb, cc = first ? c0 : c0 - src.ca;
+        const int up = (first && src.mode == IN_UP) ? 1 : 0, dn = pool ? 1 : 0;
+        const int Hs = (H << dn) >> up, Ws = (W << dn) >> up;     // source image
+        valid = 0;
 #pragma unroll
         for (int k = 0; k < NV; k++) {
             const int v = threadIdx.x + k * 256;
-            const int pix = v >> 2, part = v & 3;
+            const int vv = v < NPV ? v : 0;
+            const int pix = vv >> 2, part = vv & 3;
             const int ph = pix / PW, pw = pix - ph * PW;
             const int gh = th0 + ph - 1, gw = tw0 + pw - 1;
-            bf16x8 x = zero8();
-            if (v < NPV && gh >= 0 && gh < H && gw >= 0 && gw < W) {
-                const int ch = cc + part * 8;
-                if (mode == IN_DIRECT) {
-                    x = load8(base + (((int64_t)b * H + gh) * W + gw) * cs + ch);
-                } else if (mode == IN_UP) {
-                    x = load8(base + (((int64_t)b * (H / 2) + gh / 2) * (W / 2) + gw / 2) * cs + ch);
-                } else {   // IN_POOL: 2x2 max from (2H) x (2W)
-                    const int64_t r0 = (((int64_t)b * (2 * H) + 2 * gh) * (2 * W) + 2 * gw) * cs + ch;
-                    const int64_t rs = (int64_t)(2 * W) * cs;
-                    const bf16x8 p0 = load8(base + r0), p1 = load8(base + r0 + cs);
-                    const bf16x8 p2 = load8(base + r0 + rs), p3 = load8(base + r0 + rs + cs);
-#pragma unroll
-                    for (int e = 0; e < 8; e++)
-                        x[e] = (bf16)fmaxf(fmaxf((float)p0[e], (float)p1[e]),
-                                           fmaxf((float)p2[e], (float)p3[e]));
-                }
+            if (v < NPV && gh >= 0 && gh < H && gw >= 0 && gw < W) valid |= 1u << k;
+            const int ghc = min(max(gh, 0), H - 1), gwc = min(max(gw, 0), W - 1);
+            const int64_t r0 = (((int64_t)b * Hs + ((ghc << dn) >> up)) * Ws + ((gwc << dn) >> up))
+                               * cs + cc + part * 8;
+            raw[k][0] = load8(base + r0);
+            if constexpr (POOL) {
+                const int64_t rs = (int64_t)Ws * cs;
+                raw[k][1] = load8(base + r0 + cs);
+                raw[k][2] = load8(base + r0 + rs);
+                raw[k][3] = load8(base + r0 + rs + cs);
             }
-            xs[k] = x;
         }
     };
     auto put = [&](int buf) {
 #pragma unroll
         for (int k = 0; k < NV; k++) {
             const int v = threadIdx.x + k * 256;
-            if (v < NPV) *reinterpret_cast<bf16x8*>(&patch[buf][(v >> 2) * CS + (v & 3) * 8]) = xs[k];
+            if (v >= NPV) continue;
+            bf16x8 x = raw[k][0];
+            if constexpr (POOL) {
+#pragma unroll
+                for (int e = 0; e < 8; e++)
+                    x[e] = (bf16)fmaxf(fmaxf((float)raw[k][0][e], (float)raw[k][1][e]),
+                                       fmaxf((float)raw[k][2][e], (float)raw[k][3][e]));
+            }
+            if (!((valid >> k) & 1u)) x = zero8();
+            *reinterpret_cast<bf16x8*>(&patch[buf][(v >> 2) * CS + (v & 3) * 8]) = x;
         }
     };
 
-    if (sl0 < sl1) {
-        fetch(sl0 * 32);
-        put(0);
-    }
-    __syncthreads();
-    for (int s = sl0; s < sl1; s++) {
-        const int buf = (s - sl0) & 1, c0 = s * 32;
-        if (s + 1 < sl1) fetch(c0 + 32);         // in flight during the MFMAs below
+    // this slice's weights are loaded before the next slice's inputs: the
+    // vector-memory counter completes in issue order, so the MFMAs can wait
+    // for their weights while the prefetch is still in flight
+    bf16x8 bw[9][2];
+    auto weights = [&](int c0) {
+#pragma unroll
+        for (int tap = 0; tap < 9; tap++)
+#pragma unroll
+            for (int u = 0; u < 2; u++)
+                bw[tap][u] = load8(w + ((int64_t)(n_base + u * 16 + row) * 9 + tap) * cin + c0 + kq);
+    };
+    auto mfmas = [&](int buf) {
 #pragma unroll
         for (int tap = 0; tap < 9; tap++) {
             const int dy = tap / 3, dx = tap % 3;    // patch offsets 0..2
-            bf16x8 bv[2];
-#pragma unroll
-            for (int u = 0; u < 2; u++)
-                bv[u] = load8(w + ((int64_t)(n_base + u * 16 + row) * 9 + tap) * cin + c0 + kq);
             const int shift = (dy * PW + dx) * CS;
 #pragma unroll
             for (int i = 0; i < MT; i++) {
                 const bf16x8 a = *reinterpret_cast<const bf16x8*>(&patch[buf][aoff[i] + shift]);
 #pragma unroll
                 for (int u = 0; u < 2; u++)
-                    acc[i][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bv[u], acc[i][u], 0, 0, 0);
+                    acc[i][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw[tap][u], acc[i][u], 0, 0, 0);
             }
         }
-        // the other buffer was last read before the previous barrier
-        if (s + 1 < sl1) put(buf ^ 1);
+    };
+
+    if constexpr (PIPE) {
+        if (sl0 < sl1) {
+            fetch(sl0 * 32);
+            put(0);
+        }
         __syncthreads();
+        // every slice but the last prefetches its successor; no load in the
+        // loop is conditional, so the compiler's wait counts stay exact
+        for (int s = sl0; s + 1 < sl1; s++) {
+            const int buf = (s - sl0) & 1, c0 = s * 32;
+            weights(c0);
+            __builtin_amdgcn_sched_barrier(0);   // keep the issue order: weights, then inputs
+            fetch(c0 + 32);
+            __builtin_amdgcn_sched_barrier(0);
+            mfmas(buf);
+            put(buf ^ 1);       // the other buffer was last read before the previous barrier
+            __syncthreads();
+        }
+        if (sl0 < sl1) {
+            weights((sl1 - 1) * 32);
+            mfmas((sl1 - 1 - sl0) & 1);
+        }
+    } else {
+        // few slices per workgroup: stage, then multiply, in fewer registers
+        for (int s = sl0; s < sl1; s++) {
+            if (s > sl0) __syncthreads();        // the previous slice's MFMAs are done
+            fetch(s * 32);
+            put(0);
+            __syncthreads();
+            weights(s * 32);
+            mfmas(0);
+        }
     }
     // epilogue: lane holds column n = l % 16, rows 4 * (l / 16) + j of each M tile
 #pragma unroll
@@ -620,8 +665,15 @@ static void conv_tile(hipStream_t st, const ConvIn& src, const bf16* w, const fl
     if (src.acc32 && nblocks < 512)
         while (ks < 4 && nblocks * ks < 1024 && nsl / (2 * ks) >= 2) ks *= 2;
     dim3 grid(tiles, cout / (32 * WN), ks);
-    k_conv3x3_tile<TW, TH, WN><<<grid, 256, 0, st>>>(src, w, bias, out, out32, cout, nb, H, W,
-                                                     flags);
+    const bool pool = src.mode == IN_POOL, pipe = nsl / ks >= 3;
+#define STM_CONV_TILE(P, Q)                                                                    \
+    k_conv3x3_tile<TW, TH, WN, P, Q><<<grid, 256, 0, st>>>(src, w, bias, out, out32, cout, nb, H, \
+                                                           W, flags)
+    if (pool && pipe) STM_CONV_TILE(true, true);
+    else if (pool) STM_CONV_TILE(true, false);
+    else if (pipe) STM_CONV_TILE(false, true);
+    else STM_CONV_TILE(false, false);
+#undef STM_CONV_TILE
     if (ks > 1) {
         const int64_t n = (int64_t)nb * H * W * cout;
         k_conv_finish<<<nblk(n), 256, 0, st>>>(src.acc32, bias, out, out32, n, cout, flags,
@@ -683,8 +735,11 @@ void relu_bwd(hipStream_t st, int64_t n, const bf16* y, bf16* g) {
     k_relu_bwd<<<nblk(n), 256, 0, st>>>(n, y, g);
 }
 void bias_grad(hipStream_t st, const bf16* g, int c, int64_t npix, float* db) {
-    // bias channels are 32, 64, 128 or 256; short chunks keep many blocks in flight
-    const int per = 128;
+    // bias channels are 32, 64, 128 or 256.  Each block adds its c sums to db
+    // with atomics, all blocks on the same c addresses, so the block count is
+    // capped near one per CU (the atomics, not the 2-byte reads, set the time)
+    int per = 128;
+    while (npix / per > 256) per *= 2;
     k_bias_grad<<<nblk(npix, per), 256, 0, st>>>(g, c, (int)npix, per, db);
 }
 void maxpool(hipStream_t st, const bf16* x, bf16* y, int nb, int H, int W, int c, const int* live) {

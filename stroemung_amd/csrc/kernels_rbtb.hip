@@ -362,7 +362,8 @@ __global__ void k_pbuf_reset(int nchan, ChanState* __restrict__ cs) {
     V(4, 4, 4, 64, 16)       \
     V(5, 4, 6, 64, 16)       \
     V(6, 8, 6, 64, 16)       \
-    V(7, 2, 2, 64, 16)
+    V(7, 2, 2, 64, 16)       \
+    V(8, 8, 12, 64, 8)
 
 #define V_ENTRY(i, K, R, HY, NS) {K, NS * R - 2 * (2 * K + 4), 2 * HY - 2 * (2 * K + 4)},
 const TbVariant TB_VARIANTS[] = {STM_TB_VARIANTS(V_ENTRY)};

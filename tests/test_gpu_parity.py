@@ -186,7 +186,7 @@ def test_stages_bit_exact_random_fields(seed, nx, ny):
     _cmp(ctx, ch, osim, ("p",), "ghost p")
 
 
-N_VARIANTS = 8
+N_VARIANTS = 9
 
 
 @pytest.mark.parametrize("variant", range(N_VARIANTS))
@@ -275,7 +275,7 @@ def test_whole_tick_autotune_picks_a_permitted_variant_and_keeps_the_bound():
     ctx.tick()
     t0 = [ctx.state(c)["time"] for c in range(len(ws))]
     v, rate = ctx.autotune_ticks()
-    assert 0 <= v < 8 and rate > 0
+    assert 0 <= v < N_VARIANTS and rate > 0
     assert ctx.solver_variant()[0] == v
     for c in range(len(ws)):   # one real tick per variant
         assert ctx.state(c)["time"] > t0[c]

@@ -72,6 +72,20 @@ def parse():
     return ap.parse_args()
 
 
+def host_info():
+    """nproc and CPU model of the machine the CPU baseline ran on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "cpu_model": model}
+
+
 def cpu_baseline(n, sweeps_per_step, nsweeps):
     """Oracle (plain-C port of the reference, one core) on the same grid:
     one tick's non-SOR stages plus `nsweeps` lexicographic SOR sweeps,
@@ -105,6 +119,7 @@ def cpu_baseline(n, sweeps_per_step, nsweeps):
         "unit": "cell-updates/s",
         "cores": 1,
         "kind": "port",
+        "host": host_info(),
         "sample": ("oracle lexicographic SOR (reference algorithm, masked residual test each "
                    "sweep) on the %dx%d cavity: %d sweeps timed (%.3f ms/sweep) plus one "
                    "tick's BC/FG/RHS/ADAP_UV (%.1f ms); extrapolated to %.0f sweeps per "
@@ -211,6 +226,7 @@ def channels_run(a):
 
 
 MFMA_BF16_PEAK_TFLOPS = 2500.0
+MALL_BYTES = 256 << 20          # MI355X Infinity Cache
 
 
 def learned_cavities(ticks=10, log=None):
@@ -439,6 +455,7 @@ def main():
     # one launch advances every interior cell by ksweeps red-black sweeps
     bytes_per_launch = SWEEP_BYTES_PER_CELL * ksweeps * cells
     achieved = bytes_per_launch / avg_s / 1e9 if timed else None
+    working_set = (3 * 8 + 1) * (n + 2) * (n + 2)
     kernels = {}
     for k in ("rb_tb", "pbuf_fix", "solve_begin", "fg", "rhs", "adap_uv", "restore",
               "bc_velocity", "ranges", "delt"):
@@ -485,6 +502,10 @@ def main():
             "avg_launch_us": avg_s * 1e6,
             "timed_launches": timed,
             "measured_copy_gbs": copy_gbs,
+            # p, the second p buffer, rhs and flags: under the 256 MB Infinity Cache
+            # on this grid, so the kernel's data may stay in the MALL between launches
+            "working_set_bytes": working_set,
+            "cache_resident": working_set < MALL_BYTES,
         },
         "kernels": kernels,
     }

@@ -86,6 +86,48 @@ def host_info():
     return {"nproc": os.cpu_count(), "cpu_model": model}
 
 
+def cpu_parallel(n, sweeps_per_step, nsweeps):
+    """Not the reference algorithm: the oracle's red-black sweeps with each
+    half-sweep split over the host threads (OMP_NUM_THREADS, else every core;
+    bit-identical to the serial sweeps), timed for `nsweeps` sweeps on the
+    same grid and extrapolated to the GPU's red-black sweeps per step, plus one
+    tick's other stages on one core."""
+    import numpy as np
+
+    from oracle import pyoracle as O
+    from stroemung_amd import presets
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", 0)) or os.cpu_count()
+    ct, vel = presets.lid_cavity(n, n)
+    sim = O.Sim(ct, vel, delx=1.0 / n, dely=1.0 / n, delt=1e-4, gamma=0.9, reynolds=1000.0,
+                sor_absolute_epsilon=0.0, max_iterations=1, omega=1.8, solver=O.SOLVER_RB)
+    sim.u[...] = np.random.default_rng(0).uniform(-0.1, 0.1, sim.u.shape)
+    assert sim.call("rebuild_boundary_list") == 0
+    t0 = time.perf_counter()
+    sim.call("set_boundary_u_and_v")
+    sim.call("calculate_f_and_g")
+    sim.call("calculate_rhs")
+    sim.set(rhs_mean=O.lib().orc_rhs_mean(sim.s))
+    t1 = time.perf_counter()
+    sim.call("rb_sweeps_mt", nsweeps, threads)
+    t2 = time.perf_counter()
+    sim.call("set_u_and_v")
+    t3 = time.perf_counter()
+    t_stages = (t1 - t0) + (t3 - t2)
+    t_sweep = (t2 - t1) / nsweeps
+    return {
+        "value": n * n / (t_stages + sweeps_per_step * t_sweep),
+        "unit": "cell-updates/s",
+        "cores": threads,
+        "kind": "port, red-black (not the reference algorithm)",
+        "host": host_info(),
+        "sample": ("oracle red-black sweeps on %d threads on the %dx%d cavity: %d sweeps "
+                   "timed (%.3f ms/sweep) plus one tick's other stages on one core (%.1f ms); "
+                   "extrapolated to %.0f sweeps per step as measured on the GPU"
+                   % (threads, n, n, nsweeps, t_sweep * 1e3, t_stages * 1e3, sweeps_per_step)),
+    }
+
+
 def cpu_baseline(n, sweeps_per_step, nsweeps):
     """Oracle (plain-C port of the reference, one core) on the same grid:
     one tick's non-SOR stages plus `nsweeps` lexicographic SOR sweeps,
@@ -511,6 +553,7 @@ def main():
     }
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n, sweeps_per_step, a.cpu_sweeps)
+        out["cpu_parallel"] = cpu_parallel(n, sweeps_per_step, 4 * a.cpu_sweeps)
     ctx.close()
     if not a.no_learned:
         try:

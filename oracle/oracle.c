@@ -677,6 +677,35 @@ double orc_rhs_mean(const orc_sim *s)
     return n ? acc / (double)n : 0.0;
 }
 
+/*
+ * orc_rb_sweeps with each half-sweep's rows of x split over `threads` OpenMP
+ * threads.  A cell of one colour reads only cells of the other colour, so the
+ * result is bit-identical to orc_rb_sweeps.  Not the reference algorithm (that
+ * is the lexicographic orc_solve_sor): a multi-core CPU comparison for the
+ * red-black solver.
+ */
+void orc_rb_sweeps_mt(orc_sim *s, int nsweeps, int threads)
+{
+    const rb_consts c = rb_make(s);
+    double *p = s->p;
+    const int nx = s->nx, ny = s->ny;
+    const double mean = s->rhs_mean;
+    for (int k = 0; k < nsweeps; k++) {
+        orc_ghost_refresh(s, -1);
+        for (int colour = 0; colour < 2; colour++) {
+#pragma omp parallel for num_threads(threads) schedule(static)
+            for (int x = 1; x < nx - 1; x++)
+                for (int y = 1 + ((x + 1 + colour) & 1); y < ny - 1; y += 2) {
+                    const int i = x * ny + y;
+                    if (s->cell_type[i] != ORC_FLUID) continue;
+                    const double t = fma(p[i + ny] + p[i - ny], c.rdx2,
+                                         (p[i + 1] + p[i - 1]) * c.rdy2);
+                    p[i] = fma(c.omw, p[i], c.mid * (t - (s->rhs[i] - mean)));
+                }
+        }
+    }
+}
+
 /* One sweep = ghost refresh of every boundary cell, red half, black half. */
 void orc_rb_sweeps(orc_sim *s, int nsweeps)
 {

@@ -125,6 +125,7 @@ double orc_comp_delt(const orc_sim *s, double umax, double vmax);
 void orc_ghost_refresh(orc_sim *s, int colour);       /* colour -1: all */
 void orc_rb_half_sweep(orc_sim *s, int colour);
 void orc_rb_sweeps(orc_sim *s, int nsweeps);
+void orc_rb_sweeps_mt(orc_sim *s, int nsweeps, int threads);
 double orc_masked_residual_sum(orc_sim *s, double *rhs_sum2);
 double orc_rhs_mean(const orc_sim *s);
 int orc_solve_rb(orc_sim *s, uint32_t *iters, double *norm2);

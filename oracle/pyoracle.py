@@ -117,6 +117,7 @@ def _declare(L):
     L.orc_ghost_refresh.argtypes = [P, C.c_int]
     L.orc_rb_half_sweep.argtypes = [P, C.c_int]
     L.orc_rb_sweeps.argtypes = [P, C.c_int]
+    L.orc_rb_sweeps_mt.argtypes = [P, C.c_int, C.c_int]
     L.orc_masked_residual_sum.argtypes = [P, _D]
     L.orc_masked_residual_sum.restype = C.c_double
     L.orc_rhs_mean.argtypes = [P]

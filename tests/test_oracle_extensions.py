@@ -205,3 +205,27 @@ def test_red_black_and_lexicographic_same_fixed_point():
         sols.append(s.p[fl] - s.p[fl].mean())
     scale = np.abs(sols[0]).max()
     assert np.abs(sols[0] - sols[1]).max() < 1e-8 * scale
+
+
+@pytest.mark.parametrize("threads", [1, 3, 8])
+def test_threaded_red_black_sweeps_equal_the_serial_ones(threads):
+    """The multi-core CPU comparison line: each half-sweep split over threads
+    gives the serial red-black sweeps bit for bit (a half-sweep's cells read
+    only the other colour)."""
+    ct, vel = mixed_grid(17, 45, 38)
+    rng = np.random.default_rng(5)
+    runs = []
+    for mt in (False, True):
+        s = _sim(ct, vel, solver=O.SOLVER_RB, omega=1.7)
+        s.p[...] = rng.standard_normal(ct.shape) if not runs else runs[0][1]
+        s.rhs[...] = rng.standard_normal(ct.shape) if not runs else runs[0][2]
+        p0, r0 = s.p.copy(), s.rhs.copy()
+        assert s.call("rebuild_boundary_list") == 0
+        s.set(rhs_mean=0.25)
+        if mt:
+            s.call("rb_sweeps_mt", 9, threads)
+        else:
+            s.call("rb_sweeps", 9)
+        runs.append((s.p.copy(), p0, r0))
+    np.testing.assert_array_equal(runs[0][0], runs[1][0])
+    assert not np.array_equal(runs[0][0], runs[0][1])    # the sweeps changed p

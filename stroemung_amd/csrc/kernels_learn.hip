@@ -61,17 +61,40 @@ __device__ __forceinline__ double lap_diff(const double* __restrict__ a,
     return fma((e - c) - (c - w), rdx2, ((so - c) - (c - n)) * rdy2);
 }
 
-// Sum of one tile block's values into part[blockIdx.x * NPART + slot].
-__device__ void tile_sum(double v, double* sh, double* __restrict__ part, int slot) {
+// Sums of N values over one tile block into part[blockIdx.x * NPART +
+// slot[j]]: a fixed-order shuffle reduction inside each wave, then the
+// TILE_THREADS / 64 wave sums added in wave order.  sh holds N * that many
+// doubles; a call ends with a barrier, so sh can be reused at once.
+template <int N>
+__device__ void tile_sums(const double (&v)[N], double* sh, double* __restrict__ part,
+                          const int (&slot)[N]) {
+    constexpr int NW = TILE_THREADS / 64;
     const int tid = threadIdx.y * blockDim.x + threadIdx.x;
-    sh[tid] = v;
+    const int lane = tid & 63, wave = tid >> 6;
+    double r[N];
+#pragma unroll
+    for (int j = 0; j < N; j++) r[j] = v[j];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int j = 0; j < N; j++) r[j] = r[j] + __shfl_down(r[j], off, 64);
+    if (lane == 0)
+#pragma unroll
+        for (int j = 0; j < N; j++) sh[j * NW + wave] = r[j];
     __syncthreads();
-    for (int st = TILE_THREADS / 2; st > 0; st >>= 1) {
-        if (tid < st) sh[tid] = sh[tid] + sh[tid + st];
-        __syncthreads();
+    if (tid < N) {
+        double acc = 0.0;
+#pragma unroll
+        for (int w = 0; w < NW; w++) acc = acc + sh[tid * NW + w];
+        part[(int64_t)blockIdx.x * NPART + slot[tid]] = acc;
     }
-    if (tid == 0) part[(int64_t)blockIdx.x * NPART + slot] = sh[0];
     __syncthreads();
+}
+
+__device__ void tile_sum(double v, double* sh, double* __restrict__ part, int slot) {
+    const double vv[1] = {v};
+    const int ss[1] = {slot};
+    tile_sums<1>(vv, sh, part, ss);
 }
 
 // Block sum (256 threads, 1-D) returned to every thread.
@@ -237,16 +260,24 @@ k_learn_nscale(LearnChan* __restrict__ lc, const double2* __restrict__ cpart, in
 
 // The running channels in order: map[s] = channel of batch entry s,
 // inv[c] = batch entry of channel c (-1 if not running), *live = count.
-__global__ void k_learn_compact(const LearnChan* __restrict__ lc, int nch, int32_t* __restrict__ map,
-                                int32_t* __restrict__ inv, int32_t* __restrict__ live)
+__global__ void __launch_bounds__(64)
+k_learn_compact(const LearnChan* __restrict__ lc, int nch, int32_t* __restrict__ map,
+                int32_t* __restrict__ inv, int32_t* __restrict__ live)
 {
-    if (threadIdx.x != 0) return;
-    int s = 0;
-    for (int c = 0; c < nch; c++) {
-        inv[c] = lc[c].run ? s : -1;
-        if (lc[c].run) map[s++] = c;
+    // one wave, 64 channels per step: a ballot of the running flags and the
+    // count of set bits below each lane give the packed positions in order
+    const int lane = threadIdx.x;
+    int base = 0;
+    for (int c0 = 0; c0 < nch; c0 += 64) {
+        const int c = c0 + lane;
+        const bool run = c < nch && lc[c].run;
+        const unsigned long long m = __ballot(run);
+        const int pos = base + __popcll(m & ((1ull << lane) - 1ull));
+        if (c < nch) inv[c] = run ? pos : -1;
+        if (run) map[pos] = c;
+        base += __popcll(m);
     }
-    *live = s;
+    if (lane == 0) *live = base;
 }
 
 // Network input (H x W x 32 bf16 per batch entry: -rc * scale, fluid
@@ -360,12 +391,14 @@ k_learn_dots(const ChanDesc* __restrict__ cd, const Tile* __restrict__ tiles,
             }
         }
     }
-    tile_sum(rz, sh, part, S_RZ);
-    tile_sum(zq, sh, part, S_ZQ);
     if (prev) {
-        tile_sum(rs, sh, part, S_RS);
-        tile_sum(sq, sh, part, S_SQ);
-        tile_sum(zs, sh, part, S_ZS);
+        const double v[5] = {rz, zq, rs, sq, zs};
+        const int slots[5] = {S_RZ, S_ZQ, S_RS, S_SQ, S_ZS};
+        tile_sums<5>(v, sh, part, slots);
+    } else {
+        const double v[2] = {rz, zq};
+        const int slots[2] = {S_RZ, S_ZQ};
+        tile_sums<2>(v, sh, part, slots);
     }
 }
 

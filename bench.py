@@ -13,6 +13,7 @@ RHS, pressure solve to the residual, velocity update, next step size).
     python bench.py --workload learned_channels   # 64 channels 512x256: plain vs learned
     python bench.py --workload learned_channels_500   # the same, 500 learned steps from rest
     python bench.py --workload learned_uneven     # the 256 uneven channels: plain vs learned
+    python bench.py --workload sweeps --n 4096    # the blocked kernel alone (HBM-resident at 4096)
 
 Prints one JSON line.  `value` = interior cells * K / wall seconds of the K
 timed ticks, inputs resident in HBM.  The roofline entry is for the dominant
@@ -61,7 +62,7 @@ def parse():
                     help="short fixed run for hardware counters, no JSON line")
     ap.add_argument("--workload", choices=("cavity", "channels", "learned_cavities",
                                            "learned_channels", "learned_channels_500",
-                                           "learned_uneven", "net"),
+                                           "learned_uneven", "net", "sweeps"),
                     default="cavity",
                     help="cavity: the 1024x1024 lid cavity (default); channels: 256 "
                          "independent channels of uneven size, seed 0; learned_*: plain vs "
@@ -196,6 +197,57 @@ def profile_run(a):
     ctx.synchronize()
     print("profile run: variant %d (K=%d, tile %dx%d), %d sweeps; copy probe %d bytes, %.0f GB/s"
           % (variant, k, tx, ty, 64 * k, COPY_PROBE_BYTES, gbs), flush=True)
+
+
+def sweeps_run(a):
+    """The blocked red-black kernel alone: fixed sweeps on the n x n lid
+    cavity with random p and rhs (no tick), timed with HIP events on the
+    library's stream.  At n = 4096 the solve's working set (p, second p
+    buffer, rhs, flags: 420 MB) exceeds the 256 MB Infinity Cache, so the
+    kernel runs from HBM; n = 1024 (26 MB) is cache-resident."""
+    import numpy as np
+
+    n = a.n
+    ctx, ch = make_context(a)
+    rng = np.random.default_rng(0)
+    ctx.upload(ch, p=rng.standard_normal((n + 2, n + 2)),
+               rhs=rng.standard_normal((n + 2, n + 2)))
+    tuned = ctx.autotune(8)
+    variant, k, tx, ty = ctx.solver_variant()
+    launches = 64
+    ctx.stage("rb_sweeps", 8 * k)                     # warm-up
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.stage("rb_sweeps", launches * k)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    ctx.close()
+    # wall time per working launch: the host polls, the launch that takes
+    # the stop decision and the queued launches after it are included, so
+    # this slightly understates the kernel (the rocprofv3 trace gives it alone)
+    avg_s = (t1 - t0) / launches
+    bytes_per_launch = SWEEP_BYTES_PER_CELL * k * n * n
+    achieved = bytes_per_launch / avg_s / 1e9
+    working_set = (3 * 8 + 1) * (n + 2) * (n + 2)
+    out = {
+        "metric": "blocked red-black SOR sweep, algorithmic bytes per second",
+        "value": achieved, "unit": "GB/s", "n_gpus": 1, "steps": launches, "warmup": 8,
+        "ms_per_step": avg_s * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "%d K-sweep launches on the %dx%d lid cavity, random p and rhs"
+                               % (launches, n, n),
+                   "variant": {"id": variant, "sweeps_per_launch": k, "tile": [tx, ty],
+                               "autotune_us_per_sweep": tuned[1]},
+                   "wall_us_per_sweep": (t1 - t0) * 1e6 / (launches * k),
+                   "timing": "wall time of the sweeps stage / working launches"},
+        "roofline": {"kernel": "k_rb_strip<%d, %s>" % (k, VARIANT_SHAPES.get(variant, "?")),
+                     "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": bytes_per_launch, "avg_launch_us": avg_s * 1e6,
+                     "working_set_bytes": working_set,
+                     "cache_resident": working_set < MALL_BYTES},
+    }
+    print(json.dumps(out), flush=True)
 
 
 def channels_run(a):
@@ -450,6 +502,9 @@ def main():
         return
     if a.workload == "channels":
         channels_run(a)
+        return
+    if a.workload == "sweeps":
+        sweeps_run(a)
         return
     if a.workload == "learned_channels_500":
         learned_500_line(a)

@@ -657,12 +657,13 @@ static void conv_tile(hipStream_t st, const ConvIn& src, const bf16* w, const fl
                       bf16* out, float* out32, int cout, int nb, int H, int W, int flags) {
     const unsigned tiles = (unsigned)(((W + TW - 1) / TW) * ((H + TH - 1) / TH) * nb);
     const unsigned nblocks = tiles * (unsigned)(cout / (32 * WN));
-    // small layers (< 512 workgroups, e.g. the deep levels, or a batch the
-    // learned solve has compacted): split the input channels over up to 4
-    // blocks per tile, each with at least two 32-channel slices
+    // small layers (fewer workgroups than CUs, e.g. the deep levels of a small
+    // batch): split the input channels over up to 4 blocks per tile, each with
+    // at least two 32-channel slices.  From one workgroup per CU up, the FP32
+    // atomics and the finishing pass cost more than the split gains.
     const int nsl = (src.ca + src.cb) / 32;
     int ks = 1;
-    if (src.acc32 && nblocks < 512)
+    if (src.acc32 && nblocks < 256)
         while (ks < 4 && nblocks * ks < 1024 && nsl / (2 * ks) >= 2) ks *= 2;
     dim3 grid(tiles, cout / (32 * WN), ks);
     const bool pool = src.mode == IN_POOL, pipe = nsl / ks >= 3;

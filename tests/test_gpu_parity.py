@@ -264,6 +264,44 @@ def test_batch_equals_single_channels():
             assert_bits_equal(a[n], b[n], "channel %d %s" % (k, n))
 
 
+def test_compacted_launches_equal_single_channels():
+    """Two large channels with a loose tolerance stop long before a small one
+    with a tight tolerance, so later launches cover only the small channel's
+    tiles (compaction); every channel must still equal its own single run, bit
+    for bit, iterations included."""
+    say("test: compacted blocked launches equal single runs")
+    # omega = 1 (Gauss-Seidel) and a tight tolerance keep the small channel
+    # sweeping long after the large ones have met their loose tolerance
+    grids = [(presets.lid_cavity(120, 90), 0.3, 1.8), (mixed_grid(21, 110, 96), 0.3, 1.8),
+             (presets.lid_cavity(40, 30), 1e-10, 1.0)]
+    def params(eps, omega):
+        return make_params(1.0 / 90, 1.0 / 90, 2e-3, 0.9, 300.0, eps, 50000, omega,
+                           solver="rb", check_every=4)
+    batch = Context(0)
+    for (ct, vel), eps, om in grids:
+        batch.add_grid(params(eps, om), ct, vel)
+    batch.construct()
+    singles = []
+    for (ct, vel), eps, om in grids:
+        c = Context(0)
+        c.add_grid(params(eps, om), ct, vel)
+        c.construct()
+        singles.append(c)
+    for step in range(3):
+        its, nrm = batch.tick()
+        assert its[2] > 4 * max(its[0], its[1]), its     # the small channel runs on alone
+        for k, c in enumerate(singles):
+            sit, snrm = c.tick()
+            assert (its[k], nrm[k]) == (sit[0], snrm[0]), (step, k)
+    for k, c in enumerate(singles):
+        a, b = batch.download(k), c.download(0)
+        for n in a:
+            assert_bits_equal(a[n], b[n], "channel %d %s" % (k, n))
+    batch.close()
+    for c in singles:
+        c.close()
+
+
 def test_whole_tick_autotune_picks_a_permitted_variant_and_keeps_the_bound():
     say("test: whole-tick autotune on an uneven batch")
     from stroemung_amd import workloads

@@ -268,7 +268,8 @@ int stm_net_benchmark(stm_net *net, uint32_t nb, uint32_t reps, double *fwd_ms,
  * (a = <r, z> / <z, -Lap z>), STM_STEP_MOMENTUM sets p += a z + b s with
  * (a, b) minimising that norm over span{z, s}, s being the change of p over
  * the previous cycle (a flexible-CG-like acceleration across cycles; the
- * first cycle of a solve takes the energy step).  The stopping test and the
+ * first cycle of a solve takes the energy step); STM_STEP_MOMENTUM2 also keeps
+ * the change of the cycle before (span{z, s1, s2}).  The stopping test and the
  * FP64 arithmetic of
  * the sweeps are those of the plain solver, so the result meets the same
  * residual.  net = NULL or cycles = 0 turns it off.  The net must come from
@@ -278,6 +279,7 @@ int stm_net_benchmark(stm_net *net, uint32_t nb, uint32_t reps, double *fwd_ms,
 #define STM_STEP_ENERGY 0
 #define STM_STEP_NETWORK 1
 #define STM_STEP_MOMENTUM 2
+#define STM_STEP_MOMENTUM2 3
 int stm_set_learned(stm_ctx *ctx, stm_net *net, uint32_t cycles, uint32_t sweeps, int32_t step,
                     uint32_t coarsen);
 /* ---- the GUI image (src/visualization.rs:29-105, render_simulation)

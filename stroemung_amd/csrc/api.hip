@@ -99,6 +99,7 @@ struct stm_ctx {
     LearnChan* h_lc = nullptr;     // pinned
     double *lres = nullptr, *lcf = nullptr;
     double* lprev = nullptr;       // two-direction step: p at the previous cycle
+    double* lprev2 = nullptr;      // three-direction step: p the cycle before
     double* lsnap = nullptr;       // capture: LEARN_POINTS snapshots of p
     float *lrc = nullptr, *lmc = nullptr;   // coarse residual and fluid fraction, nch frames
     double2* d_cpart = nullptr;
@@ -716,7 +717,9 @@ int learn_phase(stm_ctx* ctx, int budget, int final_phase) {
 // residuals, prolongation, ghosts, step length, update.
 int learn_correct(stm_ctx* ctx, Net& net, int f, int first, hipEvent_t ev0, hipEvent_t ev1) {
     const int n = nch(ctx);
-    double* prev = ctx->lstep == STM_STEP_MOMENTUM ? ctx->lprev : nullptr;
+    double* prev = (ctx->lstep == STM_STEP_MOMENTUM || ctx->lstep == STM_STEP_MOMENTUM2)
+                       ? ctx->lprev : nullptr;
+    double* prev2 = ctx->lstep == STM_STEP_MOMENTUM2 ? ctx->lprev2 : nullptr;
     LAUNCH("learn_compact", launch_learn_compact(ctx->stream, n, ctx->d_lc, ctx->d_map,
                                                  ctx->d_inv, ctx->d_live));
     LAUNCH("learn_pack", launch_learn_pack(ctx->stream, n, ctx->d_lc, ctx->lrc, ctx->lmc, net.H,
@@ -733,11 +736,12 @@ int learn_correct(stm_ctx* ctx, Net& net, int f, int first, hipEvent_t ev0, hipE
                                        -1, 0));
     LAUNCH("learn_dots", launch_learn_dots(ctx->stream, ctx->ntiles, ctx->d_cd, ctx->d_tiles,
                                            ctx->d_lc, ctx->flags, ctx->lres, ctx->lcf, ctx->p,
-                                           prev, ctx->d_part));
+                                           prev, prev2, ctx->d_part));
     LAUNCH("learn_alpha", launch_learn_alpha(ctx->stream, n, ctx->d_cd, ctx->d_lc, ctx->d_part,
                                              ctx->lstep, f, first));
     LAUNCH("learn_apply", launch_learn_apply(ctx->stream, ctx->ntiles, ctx->d_cd, ctx->d_tiles,
-                                             ctx->d_lc, ctx->flags, ctx->lcf, ctx->p, prev));
+                                             ctx->d_lc, ctx->flags, ctx->lcf, ctx->p, prev,
+                                             prev2));
     return STM_OK;
 }
 
@@ -890,6 +894,8 @@ int learn_buffers(stm_ctx* ctx, const Net& net, bool capture) {
         HIPCHK(ctx, hipMemset(ctx->lcf, 0, bytes));
         HIPCHK(ctx, hipMalloc(&ctx->lprev, bytes));
         HIPCHK(ctx, hipMemset(ctx->lprev, 0, bytes));
+        HIPCHK(ctx, hipMalloc(&ctx->lprev2, bytes));
+        HIPCHK(ctx, hipMemset(ctx->lprev2, 0, bytes));
         HIPCHK(ctx, hipMalloc(&ctx->d_map, sizeof(int32_t) * nch(ctx)));
         HIPCHK(ctx, hipMalloc(&ctx->d_inv, sizeof(int32_t) * nch(ctx)));
         HIPCHK(ctx, hipMalloc(&ctx->d_live, sizeof(int32_t)));
@@ -1061,7 +1067,7 @@ void stm_destroy(stm_ctx* ctx) {
     void* bufs[] = {ctx->u, ctx->v, ctx->p, ctx->f, ctx->g, ctx->rhs, ctx->scratch, ctx->p2,
                     ctx->flags, ctx->d_cd, ctx->d_cs, ctx->d_be, ctx->d_restore, ctx->d_tiles,
                     ctx->d_part, ctx->d_active, ctx->d_tbpart, ctx->d_tb, ctx->d_tbc, ctx->d_lc, ctx->lres,
-                    ctx->lcf, ctx->lprev, ctx->lsnap, ctx->lrc, ctx->lmc, ctx->d_cpart, ctx->d_map,
+                    ctx->lcf, ctx->lprev, ctx->lprev2, ctx->lsnap, ctx->lrc, ctx->lmc, ctx->d_cpart, ctx->d_map,
                     ctx->d_inv, ctx->d_live, ctx->d_rgba};
     for (void* b : bufs)
         if (b) hipFree(b);
@@ -1547,7 +1553,8 @@ int stm_set_learned(stm_ctx* ctx, stm_net* net, uint32_t cycles, uint32_t sweeps
         return STM_OK;
     }
     if (sweeps == 0) return fail(ctx, STM_ERR_INVALID_ARG, "sweeps per cycle must be positive");
-    if (step != STM_STEP_ENERGY && step != STM_STEP_NETWORK && step != STM_STEP_MOMENTUM)
+    if (step != STM_STEP_ENERGY && step != STM_STEP_NETWORK && step != STM_STEP_MOMENTUM &&
+        step != STM_STEP_MOMENTUM2)
         return fail(ctx, STM_ERR_INVALID_ARG, "unknown step rule %d", step);
     if ((rc = check_net(ctx, net, coarsen))) return rc;
     if ((rc = learn_buffers(ctx, net->net, false))) return rc;

@@ -82,11 +82,12 @@ void launch_learn_prolong(hipStream_t, int ntiles, const ChanDesc*, const Tile*,
                           const float* mc, int H, int W, int f, const int32_t* inv, double* cf);
 void launch_learn_dots(hipStream_t, int ntiles, const ChanDesc*, const Tile*, const LearnChan*,
                        const uint8_t* flags, const double* res, const double* cf, const double* p,
-                       const double* prev, double* part);
+                       const double* prev, const double* prev2, double* part);
 void launch_learn_alpha(hipStream_t, int nch, const ChanDesc*, LearnChan*, const double* part,
                         int step, int f, int first);
 void launch_learn_apply(hipStream_t, int ntiles, const ChanDesc*, const Tile*, const LearnChan*,
-                        const uint8_t* flags, const double* cf, double* p, double* prev);
+                        const uint8_t* flags, const double* cf, double* p, double* prev,
+                        double* prev2);
 void launch_learn_arm(hipStream_t, int nch, const ChanDesc*, ChanState*, const LearnChan*,
                       int32_t* active, int budget);
 void launch_learn_collect(hipStream_t, int nch, ChanState*, LearnChan*, int final_phase);

@@ -123,6 +123,8 @@ __device__ double chan_total(const ChanDesc& d, const double* __restrict__ part,
 constexpr int S_RES2 = 2, S_RZ = 3, S_ZQ = 4, S_DSUM = 5;
 // two-direction step (slots 0 and 1 are free once the solve has begun)
 constexpr int S_RS = 0, S_SQ = 1, S_ZS = 6;
+// three-direction step: s2 = prev - prev2
+constexpr int S_RS2 = 7, S_S2Q = 8, S_ZS2 = 9, S_S1S2 = 10;
 
 // Sum over the interior fluid cells of coarse pixel (hc, wc) of a[g], or of
 // (a[g] - a0[g]) - shift when a0 is given; *nf = the number of those cells.
@@ -194,6 +196,7 @@ k_learn_check(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
     LearnChan& l = lc[c];
     if (first) {
         l.sweeps = 0;
+        l.hist = 0;
         l.run = (d.solver == SOLVER_RB && d.itermax > 0) ? 1 : 0;
     }
     if (l.run && res2 <= s.tol2) {
@@ -204,6 +207,7 @@ k_learn_check(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
     }
     l.alpha = 0.0;
     l.beta = 0.0;
+    l.gamma = 0.0;
 }
 
 // Coarse residual rc and fluid fraction mc of every pixel of an H x W frame
@@ -368,7 +372,7 @@ k_learn_dots(const ChanDesc* __restrict__ cd, const Tile* __restrict__ tiles,
              const LearnChan* __restrict__ lc, const uint8_t* __restrict__ flags,
              const double* __restrict__ res, const double* __restrict__ cf,
              const double* __restrict__ p, const double* __restrict__ prev,
-             double* __restrict__ part)
+             const double* __restrict__ prev2, double* __restrict__ part)
 {
     __shared__ double sh[TILE_THREADS];
     const Tile t = tiles[blockIdx.x];
@@ -376,6 +380,7 @@ k_learn_dots(const ChanDesc* __restrict__ cd, const Tile* __restrict__ tiles,
     const ChanDesc& d = cd[t.chan];
     const int x = t.x0 + threadIdx.y, y = t.y0 + threadIdx.x;
     double rz = 0.0, zq = 0.0, rs = 0.0, sq = 0.0, zs = 0.0;
+    double rs2 = 0.0, s2q = 0.0, zs2 = 0.0, s1s2 = 0.0;
     if (x < d.nx && y < d.ny && interior(d, x, y)) {
         const int64_t i = d.off + x * (int64_t)d.pitch + y;
         if (flag_fluid(flags[i])) {
@@ -388,10 +393,22 @@ k_learn_dots(const ChanDesc* __restrict__ cd, const Tile* __restrict__ tiles,
                 rs = res[i] * sv;
                 sq = sv * ls;
                 zs = cf[i] * ls;
+                if (prev2) {
+                    const double s2 = prev[i] - prev2[i];
+                    const double l2 = lap_diff(prev, prev2, i, d.pitch, d.rdx2, d.rdy2);
+                    rs2 = res[i] * s2;
+                    s2q = s2 * l2;
+                    zs2 = cf[i] * l2;
+                    s1s2 = sv * l2;
+                }
             }
         }
     }
-    if (prev) {
+    if (prev2) {
+        const double v[9] = {rz, zq, rs, sq, zs, rs2, s2q, zs2, s1s2};
+        const int slots[9] = {S_RZ, S_ZQ, S_RS, S_SQ, S_ZS, S_RS2, S_S2Q, S_ZS2, S_S1S2};
+        tile_sums<9>(v, sh, part, slots);
+    } else if (prev) {
         const double v[5] = {rz, zq, rs, sq, zs};
         const int slots[5] = {S_RZ, S_ZQ, S_RS, S_SQ, S_ZS};
         tile_sums<5>(v, sh, part, slots);
@@ -419,21 +436,30 @@ k_learn_alpha(const ChanDesc* __restrict__ cd, LearnChan* __restrict__ lc,
     const ChanDesc& d = cd[c];
     const double rz = chan_total(d, part, S_RZ, sh);
     const double zq = chan_total(d, part, S_ZQ, sh);
-    double rs = 0.0, sq = 0.0, zs = 0.0;
-    if (step == 2 && !first) {
+    // kept previous p's: step 2 has one after the first cycle, step 3 counts
+    // them per channel (the graph-replayed cycles cannot be told apart)
+    const int hist = step == 3 ? lc[c].hist : ((step == 2 && !first) ? 1 : 0);
+    double rs = 0.0, sq = 0.0, zs = 0.0, rs2 = 0.0, s2q = 0.0, zs2 = 0.0, s1s2 = 0.0;
+    if (hist >= 1) {
         rs = chan_total(d, part, S_RS, sh);
         sq = chan_total(d, part, S_SQ, sh);
         zs = chan_total(d, part, S_ZS, sh);
     }
+    if (hist >= 2) {
+        rs2 = chan_total(d, part, S_RS2, sh);
+        s2q = chan_total(d, part, S_S2Q, sh);
+        zs2 = chan_total(d, part, S_ZS2, sh);
+        s1s2 = chan_total(d, part, S_S1S2, sh);
+    }
     if (threadIdx.x) return;
     LearnChan& l = lc[c];
-    double a = 0.0, b = 0.0;
+    double a = 0.0, b = 0.0, g = 0.0;
     if (l.run && l.scale > 0.0) {
         if (step == 1) {
             a = (double)(f * f) * d.delx * d.dely / l.scale;
         } else if (zq < 0.0) {               // <z, Lap z> < 0 for any z not constant
             a = -rz / zq;
-            if (step == 2 && !first && sq < 0.0) {
+            if (hist >= 1 && sq < 0.0) {
                 const double zaz = -zq, sas = -sq, zas = -zs;
                 const double det = zaz * sas - zas * zas;
                 if (det > 1e-10 * zaz * sas) {
@@ -441,10 +467,32 @@ k_learn_alpha(const ChanDesc* __restrict__ cd, LearnChan* __restrict__ lc,
                     b = (zaz * rs - zas * rz) / det;
                 }
             }
+            if (hist >= 2 && sq < 0.0 && s2q < 0.0) {
+                // Cholesky of the 3 x 3 energy matrix over (z, s1, s2); any
+                // pivot below 1e-10 of its diagonal keeps the 2-direction step
+                const double m00 = -zq, m01 = -zs, m02 = -zs2;
+                const double m11 = -sq, m12 = -s1s2, m22 = -s2q;
+                const double l00 = sqrt(m00), l10 = m01 / l00, l20 = m02 / l00;
+                const double d11 = m11 - l10 * l10;
+                if (d11 > 1e-10 * m11) {
+                    const double l11 = sqrt(d11), l21 = (m12 - l20 * l10) / l11;
+                    const double d22 = m22 - l20 * l20 - l21 * l21;
+                    if (d22 > 1e-10 * m22) {
+                        const double l22 = sqrt(d22);
+                        const double y0 = rz / l00, y1 = (rs - l10 * y0) / l11;
+                        const double y2 = (rs2 - l20 * y0 - l21 * y1) / l22;
+                        g = y2 / l22;
+                        b = (y1 - l21 * g) / l11;
+                        a = (y0 - l10 * b - l20 * g) / l00;
+                    }
+                }
+            }
         }
     }
     l.alpha = a;
     l.beta = b;
+    l.gamma = g;
+    if (step == 3 && l.run) l.hist = min(l.hist + 1, 2);   // this cycle's p is kept
 }
 
 // p += alpha * z at interior fluid cells; with prev != nullptr (the
@@ -453,7 +501,8 @@ k_learn_alpha(const ChanDesc* __restrict__ cd, LearnChan* __restrict__ lc,
 __global__ void __launch_bounds__(TILE_THREADS)
 k_learn_apply(const ChanDesc* __restrict__ cd, const Tile* __restrict__ tiles,
               const LearnChan* __restrict__ lc, const uint8_t* __restrict__ flags,
-              const double* __restrict__ cf, double* __restrict__ p, double* __restrict__ prev)
+              const double* __restrict__ cf, double* __restrict__ p, double* __restrict__ prev,
+              double* __restrict__ prev2)
 {
     const Tile t = tiles[blockIdx.x];
     const ChanDesc& d = cd[t.chan];
@@ -465,8 +514,13 @@ k_learn_apply(const ChanDesc* __restrict__ cd, const Tile* __restrict__ tiles,
         if (!lc[t.chan].run || x >= d.nx || y >= d.ny) return;
         const double b = lc[t.chan].beta;
         const int64_t i = d.off + x * (int64_t)d.pitch + y;
-        const double po = p[i];
-        if (interior(d, x, y) && flag_fluid(flags[i])) p[i] = fma(b, po - prev[i], fma(a, cf[i], po));
+        const double po = p[i], pp = prev[i];
+        if (interior(d, x, y) && flag_fluid(flags[i])) {
+            double v = fma(b, po - pp, fma(a, cf[i], po));
+            if (prev2) v = fma(lc[t.chan].gamma, pp - prev2[i], v);
+            p[i] = v;
+        }
+        if (prev2) prev2[i] = pp;
         prev[i] = po;
         return;
     }
@@ -632,8 +686,9 @@ void launch_learn_prolong(hipStream_t st, int ntiles, const ChanDesc* cd, const 
 }
 void launch_learn_dots(hipStream_t st, int ntiles, const ChanDesc* cd, const Tile* tiles,
                        const LearnChan* lc, const uint8_t* flags, const double* res,
-                       const double* cf, const double* p, const double* prev, double* part) {
-    k_learn_dots<<<ntiles, TB, 0, st>>>(cd, tiles, lc, flags, res, cf, p, prev, part);
+                       const double* cf, const double* p, const double* prev,
+                       const double* prev2, double* part) {
+    k_learn_dots<<<ntiles, TB, 0, st>>>(cd, tiles, lc, flags, res, cf, p, prev, prev2, part);
 }
 void launch_learn_alpha(hipStream_t st, int nch, const ChanDesc* cd, LearnChan* lc,
                         const double* part, int step, int f, int first) {
@@ -641,8 +696,8 @@ void launch_learn_alpha(hipStream_t st, int nch, const ChanDesc* cd, LearnChan* 
 }
 void launch_learn_apply(hipStream_t st, int ntiles, const ChanDesc* cd, const Tile* tiles,
                         const LearnChan* lc, const uint8_t* flags, const double* cf, double* p,
-                        double* prev) {
-    k_learn_apply<<<ntiles, TB, 0, st>>>(cd, tiles, lc, flags, cf, p, prev);
+                        double* prev, double* prev2) {
+    k_learn_apply<<<ntiles, TB, 0, st>>>(cd, tiles, lc, flags, cf, p, prev, prev2);
 }
 void launch_learn_arm(hipStream_t st, int nch, const ChanDesc* cd, ChanState* cs,
                       const LearnChan* lc, int32_t* active, int budget) {

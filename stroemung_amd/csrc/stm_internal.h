@@ -68,10 +68,13 @@ struct LearnChan {
     double scale;                // network input scale: 1 / rms of the coarse residual
     double alpha;                // step length of the network's correction
     double beta;                 // two-direction step: length along the previous change
+    double gamma;                // three-direction step: length along the one before
     double dmean;                // capture: mean of the pressure change
     double cscale[8];            // capture: `scale` of each stored residual (LEARN_POINTS)
     int32_t run;                 // the channel has not converged yet
     int32_t sweeps;              // sweeps summed over the phases of one solve
+    int32_t hist;                // kept previous p's of this solve (0, 1, 2)
+    int32_t pad;
 };
 
 // Training pairs one solve can store per channel.
@@ -107,6 +110,6 @@ constexpr int TILE_Y = 64;       // y-cells per tile (one wavefront)
 constexpr int TILE_THREADS = TILE_X * TILE_Y;
 
 // Partial-sum slots per tile.
-constexpr int NPART = 8;
+constexpr int NPART = 16;
 
 }  // namespace stm

@@ -311,7 +311,7 @@ def test_capture_pairs_follow_the_restriction(f):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("f,step", [(1, "network"), (4, "network"), (4, "energy"),
-                                    (4, "momentum")])
+                                    (4, "momentum"), (4, "momentum2")])
 def test_capture_train_and_learned_solve_reach_the_same_residual(f, step):
     from stroemung_amd.learned import PressureNet
     n, count = 64, 4
@@ -355,7 +355,7 @@ def test_capture_train_and_learned_solve_reach_the_same_residual(f, step):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("step", ["energy", "momentum"])
+@pytest.mark.parametrize("step", ["energy", "momentum", "momentum2"])
 def test_graph_replayed_cycles_are_bit_identical_to_launches(step):
     """Learned cycles replayed from a captured hipGraph give the same fields,
     bit for bit, as the same cycles launched one by one -- over several ticks,

@@ -23,7 +23,9 @@ lexicographic SOR (the reference algorithm, one core) timed on a bounded
 sample of the same grid.  The default line also carries "learned": the
 plain solve against the learned two-grid solve on 32 random-obstacle lid
 cavities of 256 x 256 (the network trained first, on the GPU, from pairs the
-solver produces on other layouts), both to the same residual.
+solver produces on other layouts), both to the same residual; and
+"learned_same_grid": that network's learned solve on the benchmark's own grid,
+the same ticks timed as for `value`, each solve to the same residual bound.
 """
 import argparse
 import json
@@ -323,16 +325,24 @@ MFMA_BF16_PEAK_TFLOPS = 2500.0
 MALL_BYTES = 256 << 20          # MI355X Infinity Cache
 
 
-def learned_cavities(ticks=10, log=None):
+def train_cavity_net(log=None):
+    """The network of the learned cavity comparisons: trained on the pairs
+    the solver produces on 64 random-obstacle lid cavities of 256 x 256
+    (seed 0), coarsening 4.  Returns (params, info)."""
+    from stroemung_amd import learned, workloads
+    train = workloads.random_cavities(64, n=256, seed=0, eps=1e-6)
+    return learned.train_net(train, coarsen=4, capture_ticks=48, points=4, capture_sweeps=32,
+                             steps=3000, batch=32, dagger_rounds=2, dagger_ticks=16,
+                             dagger_steps=1500, cycles=32, sweeps=16, step="momentum",
+                             log=log or (lambda *x: None))
+
+
+def learned_cavities(ticks=10, log=None, trained=None):
     """32 random-obstacle lid cavities of 256 x 256 cells, Re 50-1000, eps
     1e-6: network trained on 64 seed-0 layouts, compared on seed-100 layouts."""
     from stroemung_amd import learned, workloads
     log = log or (lambda *x: None)
-    train = workloads.random_cavities(64, n=256, seed=0, eps=1e-6)
-    params, info = learned.train_net(train, coarsen=4, capture_ticks=48, points=4,
-                                     capture_sweeps=32, steps=3000, batch=32, dagger_rounds=2,
-                                     dagger_ticks=16, dagger_steps=1500, cycles=32, sweeps=16,
-                                     step="momentum", log=log)
+    params, info = trained or train_cavity_net(log)
     test = workloads.random_cavities(32, n=256, seed=100, eps=1e-6)
     res = learned.compare(test, params, coarsen=4, cycles=32, sweeps=16, warm=20, ticks=ticks,
                           step="momentum", log=log)
@@ -342,6 +352,52 @@ def learned_cavities(ticks=10, log=None):
                        "trained on 64 seed-0 layouts (20 480 pairs), measured on seed 100 "
                        "after 20 plain ticks")
     return res
+
+
+def learned_same_grid(a, params, plain_value, plain_sweeps, log=None):
+    """The learned solve on the benchmark's own grid (the n x n lid cavity),
+    with the network trained on 256 x 256 cavities (fully convolutional, so it
+    runs on an (n/8) x (n/8) frame at coarsening 8): the same warm-up tick and
+    tuner as the plain run, then the same ticks timed, each solve to the same
+    residual bound."""
+    from stroemung_amd.learned import PressureNet
+    n, f = a.n, 8
+    ctx, ch = make_context(a)
+    net = PressureNet(ctx, n // f, n // f, 1, seed=1)
+    try:
+        net.set_params(params)
+        for _ in range(a.warmup):
+            ctx.tick()
+        ctx.autotune(8)
+        ctx.set_graphs(True)
+        ctx.set_learned(net, cycles=256, sweeps=32, step="momentum", coarsen=f)
+        ctx.synchronize()
+        sweeps, bound = 0, True
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            its, _ = ctx.tick()
+            sweeps += its[ch]
+            st = ctx.state(ch)
+            bound = bound and st["residual_squared"] <= st["tolerance_squared"] \
+                and not st["exhausted"]
+        ctx.synchronize()
+        el = time.perf_counter() - t0
+        ctx.set_learned(None)
+    finally:
+        net.close()
+        ctx.close()
+    value = n * n * a.steps / el
+    if log:
+        log("same grid: learned %.2f ms/tick, %.0f sweeps/step, %.1fx the plain run"
+            % (el * 1e3 / a.steps, sweeps / a.steps, value / plain_value))
+    return {"workload": "the %dx%d lid cavity above, same ticks, same residual bound; network "
+                        "trained on 64 random-obstacle 256x256 cavities (coarsening 4), run "
+                        "here on a %dx%d frame at coarsening 8" % (n, n, n // f, n // f),
+            "cell_updates_per_s": value, "ms_per_tick": el * 1e3 / a.steps,
+            "sweeps_per_step": sweeps / a.steps, "plain_sweeps_per_step": plain_sweeps,
+            "speedup_vs_plain": value / plain_value, "residual_bound_met": bound,
+            "cycles": 256, "sweeps_per_cycle": 32, "step": "momentum", "coarsen": f,
+            "graphs": True}
 
 
 def learned_channels(ticks, log=None):
@@ -611,9 +667,12 @@ def main():
         out["cpu_parallel"] = cpu_parallel(n, sweeps_per_step, 4 * a.cpu_sweeps)
     ctx.close()
     if not a.no_learned:
+        log = lambda *x: print("learned:", *x, file=sys.stderr, flush=True)  # noqa: E731
         try:
-            out["learned"] = learned_cavities(
-                10, lambda *x: print("learned:", *x, file=sys.stderr, flush=True))
+            trained = train_cavity_net(log)
+            out["learned_same_grid"] = learned_same_grid(a, trained[0], value, sweeps_per_step,
+                                                         log)
+            out["learned"] = learned_cavities(10, log, trained)
         except Exception as e:   # the plain measurement stands on its own
             out["learned"] = {"error": "%s: %s" % (type(e).__name__, e)}
     print(json.dumps(out), flush=True)

@@ -402,6 +402,15 @@ def test_trained_network_saves_sweeps_on_unseen_layouts():
                                                     res["learned"]["sweeps_per_solve"]))
     assert res["plain"]["residual_bound_met"] and res["learned"]["residual_bound_met"]
     assert res["learned"]["sweeps_per_solve"] < 0.5 * res["plain"]["sweeps_per_solve"]
+    # the network is fully convolutional: the same weights on larger grids and
+    # frames (128 x 128 cavities, 32 x 32 frames) still cut the sweeps
+    big = workloads.random_cavities(2, n=128, seed=43, eps=1e-6)
+    res = learned.compare(big, params, coarsen=4, cycles=24, sweeps=16, warm=3, ticks=2,
+                          step="momentum")
+    print("128x128: plain %.1f sweeps/solve, learned %.1f" % (
+        res["plain"]["sweeps_per_solve"], res["learned"]["sweeps_per_solve"]))
+    assert res["plain"]["residual_bound_met"] and res["learned"]["residual_bound_met"]
+    assert res["learned"]["sweeps_per_solve"] < 0.5 * res["plain"]["sweeps_per_solve"]
 
 
 @pytest.mark.gpu

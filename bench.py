@@ -357,11 +357,11 @@ def learned_cavities(ticks=10, log=None, trained=None):
 def learned_same_grid(a, params, plain_value, plain_sweeps, log=None):
     """The learned solve on the benchmark's own grid (the n x n lid cavity),
     with the network trained on 256 x 256 cavities (fully convolutional, so it
-    runs on an (n/8) x (n/8) frame at coarsening 8): the same warm-up tick and
+    runs on an (n/16) x (n/16) frame at coarsening 16): the same warm-up tick and
     tuner as the plain run, then the same ticks timed, each solve to the same
     residual bound."""
     from stroemung_amd.learned import PressureNet
-    n, f = a.n, 8
+    n, f = a.n, 16
     ctx, ch = make_context(a)
     net = PressureNet(ctx, n // f, n // f, 1, seed=1)
     try:
@@ -392,7 +392,7 @@ def learned_same_grid(a, params, plain_value, plain_sweeps, log=None):
             % (el * 1e3 / a.steps, sweeps / a.steps, value / plain_value))
     return {"workload": "the %dx%d lid cavity above, same ticks, same residual bound; network "
                         "trained on 64 random-obstacle 256x256 cavities (coarsening 4), run "
-                        "here on a %dx%d frame at coarsening 8" % (n, n, n // f, n // f),
+                        "here on a %dx%d frame at coarsening %d" % (n, n, n // f, n // f, f),
             "cell_updates_per_s": value, "ms_per_tick": el * 1e3 / a.steps,
             "sweeps_per_step": sweeps / a.steps, "plain_sweeps_per_step": plain_sweeps,
             "speedup_vs_plain": value / plain_value, "residual_bound_met": bound,

@@ -258,8 +258,8 @@ int stm_net_benchmark(stm_net *net, uint32_t nb, uint32_t reps, double *fwd_ms,
 /* ---- learned pressure solve (no reference counterpart)
  * With a net set, every red-black solve of the context runs `cycles`
  * two-grid cycles first and then the plain red-black solve to the same eps.
- * A cycle restricts the residual r to a grid coarser by `coarsen` (1, 2, 4
- * or 8: pixel = mean of r over its coarsen^2 interior cells), normalises it,
+ * A cycle restricts the residual r to a grid coarser by `coarsen` (1, 2, 4,
+ * 8 or 16: pixel = mean of r over its coarsen^2 interior cells), normalises it,
  * runs the network there, prolongs its output y bilinearly to a correction z,
  * sets p += a z and runs `sweeps` red-black sweeps.  The network is trained
  * to output the restricted error times rms(R r) / h_c^2 (h_c^2 = coarsen^2

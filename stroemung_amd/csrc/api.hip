@@ -928,8 +928,8 @@ int learn_buffers(stm_ctx* ctx, const Net& net, bool capture) {
 int check_net(stm_ctx* ctx, stm_net* net, uint32_t f) {
     const Net& m = net->net;
     if (m.ctx != ctx) return fail(ctx, STM_ERR_INVALID_ARG, "the net belongs to another context");
-    if (f != 1 && f != 2 && f != 4 && f != 8)
-        return fail(ctx, STM_ERR_INVALID_ARG, "coarsening %u is not 1, 2, 4 or 8", f);
+    if (f != 1 && f != 2 && f != 4 && f != 8 && f != 16)
+        return fail(ctx, STM_ERR_INVALID_ARG, "coarsening %u is not 1, 2, 4, 8 or 16", f);
     if (m.cap < nch(ctx))
         return fail(ctx, STM_ERR_INVALID_ARG, "net batch %d below the channel count %d", m.cap,
                     nch(ctx));

@@ -4,7 +4,7 @@
 // ghosts) the pressure equation is A p = b, b = -(rhs - mean), and the error
 // e = p* - p solves A e = r with r = b - A p = Lap(p) - (rhs - mean).
 //
-// The network works on a grid coarsened by f (1, 2, 4 or 8) and acts as the
+// The network works on a grid coarsened by f (1, 2, 4, 8 or 16) and acts as the
 // coarse solver of a two-grid cycle whose smoother is red-black SOR:
 //   r   = Lap(p) - (rhs - mean)             fine interior fluid cells, ghosts current
 //   rc  = R r                               coarse pixel = sum of r over its f x f

@@ -199,7 +199,7 @@ class Context:
         None: off).  step: "network" takes the network's magnitude, "energy"
         a line search, "momentum" the energy minimum over the correction and
         the previous cycle's change of p, "momentum2" also over the change of
-        the cycle before; coarsen: 1, 2, 4 or 8."""
+        the cycle before; coarsen: 1, 2, 4, 8 or 16."""
         rule = {"energy": 0, "network": 1, "momentum": 2, "momentum2": 3}[step]
         self._check(lib().stm_set_learned(self._ctx, net._net if net else None, cycles, sweeps,
                                           rule, coarsen))

@@ -267,13 +267,13 @@ def _restrict(a, f):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("f", [1, 4])
+@pytest.mark.parametrize("f", [1, 4, 16])
 def test_capture_pairs_follow_the_restriction(f):
     """The stored pair of the first capture point, recomputed from the fields:
     residual from ghost-refreshed p, restriction, scale, fluid fraction and
     the target from the converged p."""
     from stroemung_amd.learned import PressureNet
-    n, count = 64, 2
+    n, count = (128 if f == 16 else 64), 2       # frames are at least 8 x 8
     ctx, grids = _cavity_context(count, n, seed=21)
     for _ in range(3):
         ctx.tick()
@@ -311,10 +311,10 @@ def test_capture_pairs_follow_the_restriction(f):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("f,step", [(1, "network"), (4, "network"), (4, "energy"),
-                                    (4, "momentum"), (4, "momentum2")])
+                                    (4, "momentum"), (4, "momentum2"), (16, "momentum")])
 def test_capture_train_and_learned_solve_reach_the_same_residual(f, step):
     from stroemung_amd.learned import PressureNet
-    n, count = 64, 4
+    n, count = (128 if f == 16 else 64), 4       # frames are at least 8 x 8
     ctx, _ = _cavity_context(count, n, seed=11)
     net = PressureNet(ctx, n // f, n // f, 32, seed=12)
     net.reserve(64)

@@ -664,7 +664,9 @@ def main():
     }
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n, sweeps_per_step, a.cpu_sweeps)
-        out["cpu_parallel"] = cpu_parallel(n, sweeps_per_step, 4 * a.cpu_sweeps)
+        # beside the reference-order port: the red-black sweeps on all host threads
+        out["cpu_baseline"]["parallel_red_black"] = cpu_parallel(n, sweeps_per_step,
+                                                                 4 * a.cpu_sweeps)
     ctx.close()
     if not a.no_learned:
         log = lambda *x: print("learned:", *x, file=sys.stderr, flush=True)  # noqa: E731

@@ -18,7 +18,8 @@ struct ConvIn {
     const bf16* b;
     int cb;
     const int* live = nullptr;   // device count: images at or beyond it are skipped
-    float* acc32 = nullptr;      // zeroed FP32 scratch of nb*H*W*cout: allows split K
+    float* acc32 = nullptr;      // FP32 scratch for the partial sums of split K ...
+    int64_t acc_cap = 0;         // ... holding this many floats (split s writes [s*n, s*n + n))
 };
 
 // 3x3 convolution, padding 1, NHWC bf16, fp32 accumulation on the matrix

@@ -303,8 +303,8 @@ k_conv3x3_tile(ConvIn src, const bf16* __restrict__ w, const float* __restrict__
                 const int gh = th0 + m / TW, gw = tw0 + m % TW;
                 if (gh >= H || gw >= W) continue;
                 const int64_t o = (((int64_t)b * H + gh) * W + gw) * cout + n;
-                if (nz > 1) {                   // partial sum; k_conv_finish completes it
-                    atomicAdd(&src.acc32[o], acc[i][u][j]);
+                if (nz > 1) {   // split z's partial sum in its own slice; k_conv_finish adds them
+                    src.acc32[(int64_t)z * ((int64_t)nb * H * W * cout) + o] = acc[i][u][j];
                     continue;
                 }
                 float v = acc[i][u][j] + bvs;
@@ -315,18 +315,19 @@ k_conv3x3_tile(ConvIn src, const bf16* __restrict__ w, const float* __restrict__
     }
 }
 
-// Completes a split-K convolution: bias, ReLU and BF16 from the FP32 partial
-// sums, which are zeroed again for the next use.
+// Completes a split-K convolution: the ks partial sums added in split order
+// (the same bits on every run), then bias, ReLU and BF16.
 __global__ void __launch_bounds__(256)
-k_conv_finish(float* __restrict__ acc32, const float* __restrict__ bias, bf16* __restrict__ out,
-              float* __restrict__ out32, int64_t n, int cout, int flags,
+k_conv_finish(const float* __restrict__ acc32, int ks, const float* __restrict__ bias,
+              bf16* __restrict__ out, float* __restrict__ out32, int64_t n, int cout, int flags,
               const int* __restrict__ live, int64_t per_image)
 {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    if (live && k >= (int64_t)*live * per_image) return;   // never written: still zero
-    float v = acc32[k] + ((flags & 1) ? bias[k % cout] : 0.f);
-    acc32[k] = 0.f;
+    if (live && k >= (int64_t)*live * per_image) return;   // images the splits skipped
+    float v = acc32[k];
+    for (int z = 1; z < ks; z++) v = v + acc32[(int64_t)z * n + k];
+    v = v + ((flags & 1) ? bias[k % cout] : 0.f);
     if (flags & 2) v = v > 0.f ? v : 0.f;
     if (out) out[k] = (bf16)v;
     if (out32) out32[k] = v;
@@ -659,12 +660,16 @@ static void conv_tile(hipStream_t st, const ConvIn& src, const bf16* w, const fl
     const unsigned nblocks = tiles * (unsigned)(cout / (32 * WN));
     // small layers (fewer workgroups than CUs, e.g. the deep levels of a small
     // batch): split the input channels over up to 4 blocks per tile, each with
-    // at least two 32-channel slices.  From one workgroup per CU up, the FP32
-    // atomics and the finishing pass cost more than the split gains.
+    // at least two 32-channel slices, writing its partial sums to its own
+    // slice of the scratch.  From one workgroup per CU up, the finishing pass
+    // costs more than the split gains.
     const int nsl = (src.ca + src.cb) / 32;
+    const int64_t nout = (int64_t)nb * H * W * cout;
     int ks = 1;
     if (src.acc32 && nblocks < 256)
-        while (ks < 4 && nblocks * ks < 1024 && nsl / (2 * ks) >= 2) ks *= 2;
+        while (ks < 4 && nblocks * ks < 1024 && nsl / (2 * ks) >= 2 &&
+               2 * ks * nout <= src.acc_cap)
+            ks *= 2;
     dim3 grid(tiles, cout / (32 * WN), ks);
     const bool pool = src.mode == IN_POOL, pipe = nsl / ks >= 3;
 #define STM_CONV_TILE(P, Q)                                                                    \
@@ -675,11 +680,9 @@ static void conv_tile(hipStream_t st, const ConvIn& src, const bf16* w, const fl
     else if (pipe) STM_CONV_TILE(false, true);
     else STM_CONV_TILE(false, false);
 #undef STM_CONV_TILE
-    if (ks > 1) {
-        const int64_t n = (int64_t)nb * H * W * cout;
-        k_conv_finish<<<nblk(n), 256, 0, st>>>(src.acc32, bias, out, out32, n, cout, flags,
-                                               src.live, (int64_t)H * W * cout);
-    }
+    if (ks > 1)
+        k_conv_finish<<<nblk(nout), 256, 0, st>>>(src.acc32, ks, bias, out, out32, nout, cout,
+                                                  flags, src.live, (int64_t)H * W * cout);
 }
 
 bool conv_tiles_fit(int H, int W) { return W % 8 == 0 && H % 8 == 0; }

@@ -104,8 +104,8 @@ int Net::init(stm_ctx* c, int H_, int W_, int cap_, uint64_t seed) {
     NCHK(hipMalloc(&wflip, sizeof(bf16) * maxw));
     // the largest conv output is level 0's: cap x H x W x 32 (levels below have
     // a quarter of the pixels and twice the channels)
-    NCHK(hipMalloc(&splitk, sizeof(float) * (size_t)cap * H * W * 32));
-    NCHK(hipMemset(splitk, 0, sizeof(float) * (size_t)cap * H * W * 32));
+    NCHK(hipMalloc(&splitk, sizeof(float) * (size_t)splitk_cap()));
+    NCHK(hipMemset(splitk, 0, sizeof(float) * (size_t)splitk_cap()));
     for (auto& a : acts) {
         const size_t n = (size_t)cap * h(a.level) * w(a.level) * a.c;
         NCHK(hipMalloc(&a.v, sizeof(bf16) * n));
@@ -198,6 +198,7 @@ void Net::forward_infer(int nb, const int* live) {
         ConvIn src = in;
         src.live = live;
         src.acc32 = splitk;
+        src.acc_cap = splitk_cap();
         nn::conv3x3_in(stream, src, pb + c.woff, p32 + c.boff, acts[c.y].v, nullptr, c.cout, nb,
                    h(c.level), w(c.level), 3);
     };

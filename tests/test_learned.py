@@ -431,3 +431,21 @@ def test_trajectory_runs_learned_ticks_with_plain_checkpoints():
     for m in res["checkpoints"]:
         assert m["plain_ms_per_tick"] > 0 and m["learned_ms_per_tick"] > 0
         assert m["plain_sweeps_per_solve"] > 0
+
+
+@pytest.mark.gpu
+def test_small_batch_forward_is_deterministic():
+    """One image of 64 x 64 splits the deep layers' input channels over
+    several workgroups; their partial sums are added in a fixed order, so
+    repeated forwards give the same bits."""
+    from stroemung_amd.learned import PressureNet
+    ctx, _ = _cavity_context(1, 64, seed=51)
+    net = PressureNet(ctx, 64, 64, 1, seed=52)
+    x = np.random.default_rng(53).standard_normal((1, 64, 64, 2)).astype(np.float32)
+    x[..., 1] = 1.0
+    ys = [net.forward(x) for _ in range(4)]
+    for y in ys[1:]:
+        np.testing.assert_array_equal(y, ys[0])
+    assert np.isfinite(ys[0]).all() and np.abs(ys[0]).max() > 0
+    net.close()
+    ctx.close()

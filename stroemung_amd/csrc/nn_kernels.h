@@ -35,11 +35,19 @@ void conv3x3_in(hipStream_t, const ConvIn& src, const bf16* w, const float* bias
                 float* out32, int cout, int nb, int H, int W, int flags);
 // w'[ci][8 - tap][co] = w[co][tap][ci]: the backward-input weights
 void weight_flip(hipStream_t, const bf16* w, bf16* wt, int cin, int cout);
+// The gradient and loss reductions leave per-block partial sums in `part`
+// (device scratch, reused launch after launch on one stream) and add them in
+// a fixed order, so a gradient is the same from run to run.  Sizes in floats:
+// grad_partials for conv3x3_dw and bias_grad on npix pixels, loss_partials
+// for loss and head_bwd.
+int64_t grad_partials(int cin, int cout, int64_t npix);
+int64_t loss_partials(int64_t npix_per, int nb);
 // dw32[co][tap][ci] += sum_pix gz[pix][co] * in[pix + off(tap)][ci]
 void conv3x3_dw(hipStream_t, const bf16* gz, int cout, const bf16* in, int cin, float* dw32, int nb,
-                int H, int W);
+                int H, int W, float* part);
 void relu_bwd(hipStream_t, int64_t n, const bf16* y, bf16* g);
-void bias_grad(hipStream_t, const bf16* g, int c, int64_t npix, float* db);
+// db[c] += sum_pix g[pix][c]
+void bias_grad(hipStream_t, const bf16* g, int c, int64_t npix, float* db, float* part);
 void maxpool(hipStream_t, const bf16* x, bf16* y, int nb, int H, int W, int c,
              const int* live = nullptr);
 // accumulates into gx
@@ -54,10 +62,12 @@ void channel_copy(hipStream_t, const bf16* src, int cs, int s0, bf16* dst, int c
                   int64_t npix, int add, const int* live = nullptr, int64_t hw = 1);
 void head(hipStream_t, const bf16* x, int c, const float* w, const float* b, float* y,
           int64_t npix, const int* live = nullptr, int64_t hw = 1);
+// gx = gy w; dw += sum gy x; db += sum gy
 void head_bwd(hipStream_t, const bf16* x, int c, const float* w, const float* gy, bf16* gx,
-              float* dw, float* db, int64_t npix);
+              float* dw, float* db, int64_t npix, float* part);
+// denom[s] = sum m t^2 of sample s; loss_out += the masked, normalised MSE; gy its gradient
 void loss(hipStream_t, const float* y, const float* t, const float* m, float* denom, float* gy,
-          float* loss_out, int64_t npix_per, int nb);
+          float* loss_out, int64_t npix_per, int nb, float* part);
 void adam(hipStream_t, int64_t n, float* p, const float* g, float* m, float* v, bf16* pb, float lr,
           float b1, float b2, float eps, int step, float clip);
 void to_bf16(hipStream_t, int64_t n, const float* x, bf16* y);

@@ -12,6 +12,8 @@
 // column l % 16, rows 4 * (l / 16) + 0..3 of the f32 result.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "nn_kernels.h"
 
 namespace stm {
@@ -345,20 +347,26 @@ __global__ void k_weight_flip(const bf16* __restrict__ w, bf16* __restrict__ wt,
     wt[((int64_t)ci * 9 + (8 - tap)) * cout + co] = w[k];
 }
 
-// dw[co][tap][ci] += sum over a pixel chunk of gz[pix][co] * in[pix+off][ci].
+// dw[co][tap][ci] over a pixel chunk: sum of gz[pix][co] * in[pix+off][ci].
 // Block: 256 threads, a tile of up to 64 co x 64 ci for one tap; chunks of
 // 128 pixels are staged transposed in LDS ([channel][pixel]) so each lane's 8
 // consecutive pixels (the MFMA's k) are one 16-byte read.  The four waves
 // cover the tile in 32 x 32 pieces and split the chunk's four 32-pixel k
 // steps among themselves when the tile is narrower than 64 x 64, so no wave
-// multiplies padding.  Partial sums go to dw32 with atomics.
+// multiplies padding; those waves then add their sums in LDS in wave order.
+// With one pixel chunk (chunk_stride 0) the block adds its sums into dw; with
+// several, chunk z writes its own slice out[z * chunk_stride + ...] and
+// k_chunk_sum adds the slices in chunk order, so the result does not depend
+// on the order in which blocks finish.
 __global__ void __launch_bounds__(256)
 k_conv3x3_dw(const bf16* __restrict__ gz, int cout, const bf16* __restrict__ in, int cin,
-             float* __restrict__ dw32, int nb, int H, int W, int pix_per_block)
+             float* __restrict__ out, int64_t chunk_stride, int nb, int H, int W,
+             int pix_per_block)
 {
     constexpr int KP = 128, LS = KP + 8;
-    __shared__ __attribute__((aligned(16))) bf16 sa[64][LS];   // [co][pix]
-    __shared__ __attribute__((aligned(16))) bf16 sb[64][LS];   // [ci][pix]
+    __shared__ __attribute__((aligned(16))) bf16 smem[2][64][LS];
+    bf16 (*sa)[LS] = smem[0];   // [co][pix]
+    bf16 (*sb)[LS] = smem[1];   // [ci][pix]
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int tap = blockIdx.z % 9;
     const int dy = tap / 3 - 1, dx = tap % 3 - 1;
@@ -411,14 +419,69 @@ k_conv3x3_dw(const bf16* __restrict__ gz, int cout, const bf16* __restrict__ in,
                     acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[t], bb[u], acc[t][u], 0, 0, 0);
         }
     }
+    if (ks > 1) {
+        // wave w + g * wo * wi holds k group g of wave w's piece: 16 floats a
+        // lane, 4 KB a wave, in the staging memory (34 KB) once every wave is
+        // past its last MFMA read
+        float* red = reinterpret_cast<float*>(&smem[0][0][0]);
+        __syncthreads();
+        if (kg > 0)
+            for (int t = 0; t < 2; t++)
+                for (int u = 0; u < 2; u++)
+                    for (int i = 0; i < 4; i++)
+                        red[(wave * 16 + (t * 2 + u) * 4 + i) * 64 + lane] = acc[t][u][i];
+        __syncthreads();
+        if (kg == 0)
+            for (int g = 1; g < ks; g++) {
+                const int w2 = wave + g * wo * wi;
+                for (int t = 0; t < 2; t++)
+                    for (int u = 0; u < 2; u++)
+                        for (int i = 0; i < 4; i++)
+                            acc[t][u][i] += red[(w2 * 16 + (t * 2 + u) * 4 + i) * 64 + lane];
+            }
+    }
+    if (kg != 0) return;
+    float* dst = out + (int64_t)(blockIdx.z / 9) * chunk_stride;
     for (int t = 0; t < 2; t++)
         for (int u = 0; u < 2; u++) {
             const int ci = ci0 + ni * 32 + u * 16 + row;
             for (int i = 0; i < 4; i++) {
                 const int co = co0 + mo * 32 + t * 16 + (lane >> 4) * 4 + i;
-                atomicAdd(&dw32[((int64_t)co * 9 + tap) * cin + ci], acc[t][u][i]);
+                const int64_t o = ((int64_t)co * 9 + tap) * cin + ci;
+                dst[o] = chunk_stride ? acc[t][u][i] : dst[o] + acc[t][u][i];
             }
         }
+}
+
+// dst[i] += part[i] + part[n + i] + ... + part[(chunks - 1) n + i], in that order
+__global__ void k_chunk_sum(const float* __restrict__ part, int chunks, int64_t n,
+                            float* __restrict__ dst)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float s = part[i];
+    for (int c = 1; c < chunks; c++) s += part[(int64_t)c * n + i];
+    dst[i] += s;
+}
+
+// Block j: the sum over k of part[k * stride + col0 + j], k in [0, nparts):
+// thread t adds k = t, t + 256, ... in order, then a fixed LDS tree.  dst[j]
+// gets it added (add) or assigned.
+__global__ void __launch_bounds__(256)
+k_block_sum(const float* __restrict__ part, int nparts, int stride, int col0,
+            float* __restrict__ dst, int add)
+{
+    __shared__ float red[256];
+    const int j = blockIdx.x;
+    float s = 0.f;
+    for (int k = threadIdx.x; k < nparts; k += 256) s += part[(int64_t)k * stride + col0 + j];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) dst[j] = add ? dst[j] + red[0] : red[0];
 }
 
 // ------------------------------------------------------------ elementwise
@@ -430,12 +493,12 @@ __global__ void k_relu_bwd(int64_t n, const bf16* __restrict__ y, bf16* __restri
     if (k < n && !((float)y[k] > 0.f)) g[k] = (bf16)0.f;
 }
 
-// db[c] += sum over pixels of g[pix][c]
+// part[block][c] = sum over the block's pixels of g[pix][c]
 // c divides 256: thread t sums channel t % c over pixels t / c, t / c + 256 / c,
 // ... of the block's chunk (consecutive threads read consecutive channels),
 // then the 256 / c partials of each channel are added in LDS.
 __global__ void __launch_bounds__(256)
-k_bias_grad(const bf16* __restrict__ g, int c, int npix, int per, float* __restrict__ db)
+k_bias_grad(const bf16* __restrict__ g, int c, int npix, int per, float* __restrict__ part)
 {
     __shared__ float red[256];
     const int ch = threadIdx.x % c, lanes = 256 / c;
@@ -447,7 +510,7 @@ k_bias_grad(const bf16* __restrict__ g, int c, int npix, int per, float* __restr
     if (threadIdx.x < c) {
         float t = 0.f;
         for (int k = threadIdx.x; k < 256; k += c) t += red[k];
-        atomicAdd(&db[threadIdx.x], t);
+        part[(int64_t)blockIdx.x * c + threadIdx.x] = t;
     }
 }
 
@@ -553,10 +616,11 @@ __global__ void k_head(const bf16* __restrict__ x, int c, const float* __restric
     y[p] = s;
 }
 
-// head backward: gx[pix, c] = gy[pix] * w[c]; dw[c] += sum gy * x; db += sum gy
+// head backward: gx[pix, c] = gy[pix] * w[c]; per block, part[block][0] = sum gy
+// and part[block][1 + c] = sum gy * x[c] over the block's pixels
 __global__ void k_head_bwd(const bf16* __restrict__ x, int c, const float* __restrict__ w,
                            const float* __restrict__ gy, bf16* __restrict__ gx,
-                           float* __restrict__ dw, float* __restrict__ db, int64_t npix)
+                           float* __restrict__ part, int64_t npix)
 {
     __shared__ float red[4][33];
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -574,7 +638,7 @@ __global__ void k_head_bwd(const bf16* __restrict__ x, int c, const float* __res
     if (threadIdx.x <= c) {
         const float t = (red[0][threadIdx.x] + red[1][threadIdx.x]) +
                         (red[2][threadIdx.x] + red[3][threadIdx.x]);
-        atomicAdd(threadIdx.x == 0 ? db : &dw[threadIdx.x - 1], t);
+        part[(int64_t)blockIdx.x * (c + 1) + threadIdx.x] = t;
     }
 }
 
@@ -583,7 +647,7 @@ __global__ void k_head_bwd(const bf16* __restrict__ x, int c, const float* __res
 // solve that made no change) has weight zero.
 __global__ void k_loss(const float* __restrict__ y, const float* __restrict__ t,
                        const float* __restrict__ m, const float* __restrict__ denom,
-                       float* __restrict__ gy, float* __restrict__ loss, int64_t npix_per,
+                       float* __restrict__ gy, float* __restrict__ part, int64_t npix_per,
                        int nb)
 {
     __shared__ float red[256];
@@ -603,12 +667,12 @@ __global__ void k_loss(const float* __restrict__ y, const float* __restrict__ t,
         if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
         __syncthreads();
     }
-    if (threadIdx.x == 0) atomicAdd(loss, red[0]);
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
-// per-sample sum of m * t^2 (+1e-20)
+// per-sample sum of m * t^2 (+1e-30), in parts: part[block x][sample]
 __global__ void k_target_norm(const float* __restrict__ t, const float* __restrict__ m,
-                              float* __restrict__ denom, int64_t npix_per)
+                              float* __restrict__ part, int64_t npix_per)
 {
     __shared__ float red[256];
     const int s = blockIdx.y;
@@ -623,7 +687,8 @@ __global__ void k_target_norm(const float* __restrict__ t, const float* __restri
         if (threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
         __syncthreads();
     }
-    if (threadIdx.x == 0) atomicAdd(&denom[s], red[0] + (blockIdx.x == 0 ? 1e-30f : 0.f));
+    if (threadIdx.x == 0)
+        part[(int64_t)blockIdx.x * gridDim.y + s] = red[0] + (blockIdx.x == 0 ? 1e-30f : 0.f);
 }
 
 // Adam on fp32 master weights; refreshes the bf16 copy used by the kernels
@@ -725,26 +790,55 @@ void weight_flip(hipStream_t st, const bf16* w, bf16* wt, int cin, int cout) {
     const int64_t n = (int64_t)cin * 9 * cout;
     k_weight_flip<<<nblk(n), 256, 0, st>>>(w, wt, cin, cout);
 }
-void conv3x3_dw(hipStream_t st, const bf16* gz, int cout, const bf16* in, int cin, float* dw32,
-                int nb, int H, int W) {
-    const int64_t npix = (int64_t)nb * H * W;
-    const int ci_blocks = (cin + 63) / 64;
+static int dw_pixels_per_chunk(int64_t npix) {
     int per = 4096;
     while (npix / per > 64 && per < (1 << 20)) per *= 2;
+    return per;
+}
+static int bias_pixels_per_block(int64_t npix) {
+    int per = 128;
+    while (npix / per > 256) per *= 2;
+    return per;
+}
+// Bounds over every pixel count up to npix (the chunk counts are not monotone
+// in the pixel count: at most 65 chunks of 4096 << k pixels until k reaches
+// 8, at most 257 bias blocks of 128 << k).
+int64_t grad_partials(int cin, int cout, int64_t npix) {
+    const int64_t chunks =
+        std::min((npix + 4095) / 4096, std::max<int64_t>(65, (npix + (1 << 20) - 1) >> 20));
+    const int64_t dw = chunks > 1 ? chunks * cout * 9 * cin : 0;
+    const int64_t db = std::min<int64_t>((npix + 127) / 128, 257) * cout;
+    return std::max(dw, db);
+}
+int64_t loss_partials(int64_t npix_per, int nb) {
+    const int64_t npix = npix_per * nb;
+    return std::max<int64_t>({(int64_t)64 * nb, (int64_t)nblk(npix), (int64_t)nblk(npix) * 33});
+}
+void conv3x3_dw(hipStream_t st, const bf16* gz, int cout, const bf16* in, int cin, float* dw32,
+                int nb, int H, int W, float* part) {
+    const int64_t npix = (int64_t)nb * H * W;
+    const int ci_blocks = (cin + 63) / 64;
+    const int per = dw_pixels_per_chunk(npix);
     const int chunks = (int)((npix + per - 1) / per);
+    const int64_t nw = (int64_t)cout * 9 * cin;
     dim3 grid(ci_blocks, (cout + 63) / 64, 9 * chunks);
-    k_conv3x3_dw<<<grid, 256, 0, st>>>(gz, cout, in, cin, dw32, nb, H, W, per);
+    if (chunks == 1) {
+        k_conv3x3_dw<<<grid, 256, 0, st>>>(gz, cout, in, cin, dw32, 0, nb, H, W, per);
+        return;
+    }
+    k_conv3x3_dw<<<grid, 256, 0, st>>>(gz, cout, in, cin, part, nw, nb, H, W, per);
+    k_chunk_sum<<<nblk(nw), 256, 0, st>>>(part, chunks, nw, dw32);
 }
 void relu_bwd(hipStream_t st, int64_t n, const bf16* y, bf16* g) {
     k_relu_bwd<<<nblk(n), 256, 0, st>>>(n, y, g);
 }
-void bias_grad(hipStream_t st, const bf16* g, int c, int64_t npix, float* db) {
-    // bias channels are 32, 64, 128 or 256.  Each block adds its c sums to db
-    // with atomics, all blocks on the same c addresses, so the block count is
-    // capped near one per CU (the atomics, not the 2-byte reads, set the time)
-    int per = 128;
-    while (npix / per > 256) per *= 2;
-    k_bias_grad<<<nblk(npix, per), 256, 0, st>>>(g, c, (int)npix, per, db);
+void bias_grad(hipStream_t st, const bf16* g, int c, int64_t npix, float* db, float* part) {
+    // bias channels are 32, 64, 128 or 256; each block leaves c partial sums
+    // and at most about one block per CU runs, so the block sums stay short
+    const int per = bias_pixels_per_block(npix);
+    const int nblocks = (int)nblk(npix, per);
+    k_bias_grad<<<nblocks, 256, 0, st>>>(g, c, (int)npix, per, part);
+    k_block_sum<<<c, 256, 0, st>>>(part, nblocks, c, 0, db, 1);
 }
 void maxpool(hipStream_t st, const bf16* x, bf16* y, int nb, int H, int W, int c, const int* live) {
     k_maxpool<<<nblk((int64_t)nb * H * W * c / 4), 256, 0, st>>>(x, y, nb, H, W, c, live);
@@ -771,14 +865,19 @@ void head(hipStream_t st, const bf16* x, int c, const float* w, const float* b, 
     k_head<<<nblk(npix), 256, 0, st>>>(x, c, w, b, y, npix, live, hw);
 }
 void head_bwd(hipStream_t st, const bf16* x, int c, const float* w, const float* gy, bf16* gx,
-              float* dw, float* db, int64_t npix) {
-    k_head_bwd<<<nblk(npix), 256, 0, st>>>(x, c, w, gy, gx, dw, db, npix);
+              float* dw, float* db, int64_t npix, float* part) {
+    const int nblocks = (int)nblk(npix);
+    k_head_bwd<<<nblocks, 256, 0, st>>>(x, c, w, gy, gx, part, npix);
+    k_block_sum<<<1, 256, 0, st>>>(part, nblocks, c + 1, 0, db, 1);
+    k_block_sum<<<c, 256, 0, st>>>(part, nblocks, c + 1, 1, dw, 1);
 }
 void loss(hipStream_t st, const float* y, const float* t, const float* m, float* denom, float* gy,
-          float* loss_out, int64_t npix_per, int nb) {
-    hipMemsetAsync(denom, 0, sizeof(float) * nb, st);
-    k_target_norm<<<dim3(64, nb), 256, 0, st>>>(t, m, denom, npix_per);
-    k_loss<<<nblk(npix_per * nb), 256, 0, st>>>(y, t, m, denom, gy, loss_out, npix_per, nb);
+          float* loss_out, int64_t npix_per, int nb, float* part) {
+    k_target_norm<<<dim3(64, nb), 256, 0, st>>>(t, m, part, npix_per);
+    k_block_sum<<<nb, 256, 0, st>>>(part, 64, nb, 0, denom, 0);
+    const int nblocks = (int)nblk(npix_per * nb);
+    k_loss<<<nblocks, 256, 0, st>>>(y, t, m, denom, gy, part, npix_per, nb);
+    k_block_sum<<<1, 256, 0, st>>>(part, nblocks, 1, 0, loss_out, 1);
 }
 void adam(hipStream_t st, int64_t n, float* p, const float* g, float* m, float* v, bf16* pb,
           float lr, float b1, float b2, float eps, int step, float clip) {

@@ -48,9 +48,13 @@ struct Net {
     float *p32 = nullptr, *g32 = nullptr, *m32 = nullptr, *v32 = nullptr;
     nn::bf16* pb = nullptr;      // bf16 copy of the parameters
     nn::bf16* wflip = nullptr;   // scratch for the backward-input weights
-    float* splitk = nullptr;     // FP32 partial sums of split-K convolutions, one slice per split
-    // its size in floats: two batches of the first level's 32-channel output,
-    // four splits of any deeper level's (cout doubles as the sides halve)
+    // FP32 partial sums: one slice per split of a split-K convolution in the
+    // forward pass, the per-block sums of the gradient and loss reductions in
+    // training (scratch_floats in all, at least splitk_cap)
+    float* splitk = nullptr;
+    int64_t scratch_floats = 0;
+    // the split-K share in floats: two batches of the first level's 32-channel
+    // output, four splits of any deeper level's (cout doubles as the sides halve)
     int64_t splitk_cap() const { return (int64_t)2 * cap * H * W * 32; }
     float* y = nullptr;          // head output, cap x H x W (fp32)
     float* gy = nullptr;

@@ -2,6 +2,7 @@
 // the device-resident training set and the C ABI of the network.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -104,8 +105,13 @@ int Net::init(stm_ctx* c, int H_, int W_, int cap_, uint64_t seed) {
     NCHK(hipMalloc(&wflip, sizeof(bf16) * maxw));
     // the largest conv output is level 0's: cap x H x W x 32 (levels below have
     // a quarter of the pixels and twice the channels)
-    NCHK(hipMalloc(&splitk, sizeof(float) * (size_t)splitk_cap()));
-    NCHK(hipMemset(splitk, 0, sizeof(float) * (size_t)splitk_cap()));
+    scratch_floats = std::max(splitk_cap(), nn::loss_partials((int64_t)H * W, cap));
+    for (auto& cv : convs)
+        scratch_floats = std::max(
+            scratch_floats,
+            nn::grad_partials(cv.cin, cv.cout, (int64_t)cap * h(cv.level) * w(cv.level)));
+    NCHK(hipMalloc(&splitk, sizeof(float) * (size_t)scratch_floats));
+    NCHK(hipMemset(splitk, 0, sizeof(float) * (size_t)scratch_floats));
     for (auto& a : acts) {
         const size_t n = (size_t)cap * h(a.level) * w(a.level) * a.c;
         NCHK(hipMalloc(&a.v, sizeof(bf16) * n));
@@ -261,8 +267,9 @@ void Net::backward(int nb) {
         const int hh = h(c.level), ww = w(c.level);
         const int64_t np = npix(c.level);
         nn::relu_bwd(stream, np * c.cout, acts[c.y].v, acts[c.y].g);
-        nn::conv3x3_dw(stream, acts[c.y].g, c.cout, acts[c.x].v, c.cin, g32 + c.woff, nb, hh, ww);
-        nn::bias_grad(stream, acts[c.y].g, c.cout, np, g32 + c.boff);
+        nn::conv3x3_dw(stream, acts[c.y].g, c.cout, acts[c.x].v, c.cin, g32 + c.woff, nb, hh, ww,
+                       splitk);
+        nn::bias_grad(stream, acts[c.y].g, c.cout, np, g32 + c.boff, splitk);
         if (need_gx) {
             nn::weight_flip(stream, pb + c.woff, wflip, c.cin, c.cout);
             nn::conv3x3(stream, acts[c.y].g, c.cout, wflip, nullptr, acts[c.x].g, nullptr, c.cin,
@@ -270,7 +277,7 @@ void Net::backward(int nb) {
         }
     };
     nn::head_bwd(stream, acts[20].v, 32, p32 + head_w, gy, acts[20].g, g32 + head_w,
-                 g32 + head_b, npix(0));
+                 g32 + head_b, npix(0), splitk);
     conv_bwd(13, true);
     conv_bwd(12, true);
     nn::upsample_bwd(stream, acts[18].g, acts[17].g, nb, h(0), w(0), 64, 96, 0);
@@ -300,7 +307,7 @@ void Net::backward(int nb) {
 int Net::train_step(int nb, float lr) {
     NCHK(hipMemsetAsync(g32, 0, sizeof(float) * nparam, stream));
     forward(nb);
-    nn::loss(stream, y, target, mask, denom, gy, loss, (int64_t)H * W, nb);
+    nn::loss(stream, y, target, mask, denom, gy, loss, (int64_t)H * W, nb, splitk);
     backward(nb);
     adam_step++;
     nn::adam(stream, nparam, p32, g32, m32, v32, pb, lr, 0.9f, 0.999f, 1e-8f, adam_step, 1.0f);
@@ -448,7 +455,7 @@ int stm_net_gradients(stm_net* n, uint32_t nb, const float* x, const float* t, f
     hipMemsetAsync(net.loss, 0, sizeof(float), net.stream);
     net.forward((int)nb);
     nn::loss(net.stream, net.y, net.target, net.mask, net.denom, net.gy, net.loss,
-             (int64_t)net.H * net.W, (int)nb);
+             (int64_t)net.H * net.W, (int)nb, net.splitk);
     net.backward((int)nb);
     hipMemcpyAsync(loss, net.loss, sizeof(float), hipMemcpyDeviceToHost, net.stream);
     hipMemcpyAsync(grads, net.g32, sizeof(float) * net.nparam, hipMemcpyDeviceToHost, net.stream);

@@ -449,3 +449,44 @@ def test_small_batch_forward_is_deterministic():
     assert np.isfinite(ys[0]).all() and np.abs(ys[0]).max() > 0
     net.close()
     ctx.close()
+
+
+@pytest.mark.gpu
+def test_gradients_and_training_are_bitwise_reproducible():
+    """The weight, bias and head gradients and the loss are sums of per-block
+    partials added in a fixed order.  At 8 images of 64 x 64 the first
+    level's weight gradient has eight pixel chunks and the bias and head
+    reductions several blocks each, so repeated gradients and two trainings
+    from the same start and the same captured set give the same bits."""
+    from stroemung_amd.learned import PressureNet
+    ctx = _gpu_ctx()
+    H, W, nb = 64, 64, 8
+    net = PressureNet(ctx, H, W, nb, seed=61)
+    x, t = _inputs(nb, H, W, 62)
+    runs = [net.gradients(x, t) for _ in range(3)]
+    for loss, g in runs[1:]:
+        assert loss == runs[0][0]
+        np.testing.assert_array_equal(g, runs[0][1])
+    assert np.isfinite(runs[0][1]).all() and np.abs(runs[0][1]).max() > 0
+    net.close()
+    ctx.close()
+
+    trained = []
+    for _ in range(2):
+        ctx, _ = _cavity_context(2, 64, seed=63)
+        for _ in range(2):
+            ctx.tick()
+        net = PressureNet(ctx, 16, 16, 4, seed=64)
+        net.reserve(16)
+        ctx.set_capture(net, points=4, sweeps=8, coarsen=4)
+        for _ in range(2):
+            ctx.tick()
+        ctx.set_capture(None)
+        loss = net.train(24, 4, lr=1e-3, seed=65)
+        trained.append((net.dataset_size(), loss, net.get_params()))
+        net.close()
+        ctx.close()
+    (n0, l0, p0), (n1, l1, p1) = trained
+    assert n0 == n1 > 4
+    assert l0 == l1 and np.isfinite(l0)
+    np.testing.assert_array_equal(p0, p1)

@@ -124,24 +124,39 @@ struct Blk {
         }
     }
 
-    // red-black update of own fluid cells of colour c.  Branch-free: every
-    // own cell of the colour is computed and the cells that are not updated
-    // (ghosts, solids, the region's border) keep their value; bit 2r+q of
-    // `upd` says which are updated.
-    template <int c, int r = 0>
+    // Red-black update of own fluid cells of colour c.  Every LDS read of the
+    // phase is of the other colour, which the phase does not change, so they
+    // are issued first as one batch (x[r]: N or S of row r; xe / xw: E of the
+    // last row and W of the first, across the strip ends) and the wave waits
+    // once.  Branch-free: every own cell of the colour is computed and the
+    // cells that are not updated (ghosts, solids, the region's border) keep
+    // their value; bit 2r+q of `upd` says which are updated.
+    template <int c>
     __device__ static __forceinline__ void update(double (*P)[PS], double (&pv)[R][2],
                                                   const double (&rr)[R][2], uint32_t upd,
                                                   int lx0, int m, double omw, double mid,
                                                   double rdx2, double rdy2) {
-        if constexpr (r < R) {
-            constexpr int q = c ^ RP0 ^ (r & 1);
-            double pn, ps, pe, pw;
-            nbrs<r, q>(P, pv, lx0, m, pn, ps, pe, pw);
+        constexpr int o = c ^ 1;
+        double x[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int q = c ^ RP0 ^ (r & 1);
+            x[r] = P[o][OFF + (lx0 + r) * HY + m - 1 + 2 * q];
+        }
+        const double xe = P[o][OFF + (lx0 + R) * HY + m];
+        const double xw = P[o][OFF + (lx0 - 1) * HY + m];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int q = c ^ RP0 ^ (r & 1);
+            // cells (r, q ^ 1), (r +- 1, q) have colour c ^ 1: unchanged here
+            const double pn = q ? pv[r][0] : x[r];
+            const double ps = q ? x[r] : pv[r][1];
+            const double pe = r + 1 < R ? pv[r + 1 < R ? r + 1 : r][q] : xe;
+            const double pw = r >= 1 ? pv[r >= 1 ? r - 1 : r][q] : xw;
             const double tt = fma(pe + pw, rdx2, (ps + pn) * rdy2);
             const double nv = fma(omw, pv[r][q], mid * (tt - rr[r][q]));
             pv[r][q] = ((upd >> (2 * r + q)) & 1u) ? nv : pv[r][q];
             P[c][OFF + (lx0 + r) * HY + m] = pv[r][q];
-            update<c, r + 1>(P, pv, rr, upd, lx0, m, omw, mid, rdx2, rdy2);
         }
     }
 
@@ -173,7 +188,8 @@ struct Blk {
                                                  const uint8_t (&cc)[R][2], int lx0, int m,
                                                  int nsw, bool check, const ChanDesc& d) {
         const double omw = d.omw, mid = d.mid, rdx2 = d.rdx2, rdy2 = d.rdy2;
-        uint32_t upd = 0, ghost = 0;
+        uint32_t upd = 0;
+        int ghost = 0;
 #pragma unroll
         for (int r = 0; r < R; r++)
 #pragma unroll
@@ -182,8 +198,9 @@ struct Blk {
                 if (cc[r][q] & C_UPDATE) upd |= 1u << (2 * r + q);
                 if (cc[r][q] >> C_EDGE_SHIFT) ghost = 1;
             }
+        // ghost refreshes only in waves that own a ghost cell (wave-uniform)
+        ghost = __ballot(ghost) != 0;
         __syncthreads();
-        // ghost refreshes only in waves that own a ghost cell
         if (ghost) refresh<1>(P, pv, cc, lx0, m);   // black ghosts, for the first red half
         __syncthreads();
         for (int sw = 0; sw < nsw; sw++) {

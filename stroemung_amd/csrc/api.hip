@@ -51,6 +51,7 @@ struct KStat {
 struct PendingTiming {
     KStat* stat;
     hipEvent_t a, b;
+    uint32_t count;   // launches between a and b
 };
 
 }  // namespace
@@ -117,6 +118,7 @@ struct stm_ctx {
     std::vector<int64_t> cycle_key;  // what the captured cycle depends on
     // profiling
     uint32_t sample = 0;
+    bool span_open = false;   // a batch is timed as a whole: no per-launch events
     std::unordered_map<std::string, KStat> stats;
     std::vector<PendingTiming> pending;
     std::vector<hipEvent_t> free_events;
@@ -490,7 +492,7 @@ int drain(stm_ctx* ctx) {
         float ms = 0.f;
         HIPCHK(ctx, hipEventElapsedTime(&ms, t.a, t.b));
         t.stat->ms += ms;
-        t.stat->timed += 1;
+        t.stat->timed += t.count;
         ctx->free_events.push_back(t.a);
         ctx->free_events.push_back(t.b);
     }
@@ -512,7 +514,7 @@ hipEvent_t take_event(stm_ctx* ctx) {
 template <class F>
 int launch(stm_ctx* ctx, const char* name, F&& fn) {
     KStat& k = ctx->stats[name];
-    const bool timed = ctx->sample && (k.launches % ctx->sample) == 0;
+    const bool timed = ctx->sample && !ctx->span_open && (k.launches % ctx->sample) == 0;
     k.launches++;
     hipEvent_t a = nullptr, b = nullptr;
     if (timed) {
@@ -524,7 +526,7 @@ int launch(stm_ctx* ctx, const char* name, F&& fn) {
     fn();
     if (timed) {
         HIPCHK(ctx, hipEventRecord(b, ctx->stream));
-        ctx->pending.push_back(PendingTiming{&k, a, b});
+        ctx->pending.push_back(PendingTiming{&k, a, b, 1});
         if (ctx->pending.size() >= 2048) return drain(ctx);
     }
     return STM_OK;
@@ -637,8 +639,27 @@ int blocked_solve(stm_ctx* ctx, int check) {
     ctx->tb_cur = ctx->d_tb;
     ctx->ntb_cur = ctx->ntb;
     for (;;) {
+        // when profiling, batches of 8 or more launches are also timed as a
+        // whole ("rb_tb_batch"): one pair of events around back-to-back
+        // launches, whose cost per launch is then negligible
+        hipEvent_t ea = nullptr;
+        const bool span = ctx->sample && batch >= 8;
+        if (span) {
+            ea = take_event(ctx);
+            if (!ea) return fail(ctx, STM_ERR_HIP, "hipEventCreate failed");
+            HIPCHK(ctx, hipEventRecord(ea, ctx->stream));
+            ctx->span_open = true;
+        }
         for (int b = 0; b < batch; b++, L++)
             LAUNCH("rb_tb", launch_blocked(ctx, ctx->variant, L, check ? TB_CHECK : 0));
+        if (span) {
+            ctx->span_open = false;
+            hipEvent_t eb = take_event(ctx);
+            if (!eb) return fail(ctx, STM_ERR_HIP, "hipEventCreate failed");
+            HIPCHK(ctx, hipEventRecord(eb, ctx->stream));
+            ctx->pending.push_back(PendingTiming{&ctx->stats["rb_tb_batch"], ea, eb,
+                                                 (uint32_t)batch});
+        }
         bool any = false;
         int rc = poll_active(ctx, &any);
         if (rc) return rc;

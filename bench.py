@@ -45,7 +45,7 @@ COPY_PROBE_BYTES = 256 << 20
 # template arguments (R, HY, NS) of each blocked-solver variant, after K
 VARIANT_SHAPES = {0: "4, 32, 16", 1: "4, 32, 16", 2: "4, 64, 16", 3: "4, 32, 16",
                   4: "4, 64, 16", 5: "6, 64, 16", 6: "6, 64, 16", 7: "2, 64, 16",
-                  8: "12, 64, 8"}
+                  8: "12, 64, 8", 9: "6, 48, 16"}
 
 
 def parse():

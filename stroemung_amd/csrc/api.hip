@@ -337,50 +337,59 @@ int upload_descs(stm_ctx* ctx) {
     return STM_OK;
 }
 
-// Split n cells into ceil(n / tmax) near-equal consecutive parts.
-std::vector<std::pair<int, int>> split(int n, int tmax) {
-    const int parts = (n + tmax - 1) / tmax;
-    std::vector<std::pair<int, int>> out;
+// Output parts of one axis of n cells for the blocked solver: region E
+// cells, halo H, part starts and region origins multiples of q (1 in x, 2 in
+// y).  A middle part has H cells of halo on both sides (at most E - 2H
+// cells).  The first part's region starts q cells before the channel and the
+// last part's region reaches past its end, where nothing feeds the tile, so
+// those parts hold up to E - q - H and E - 1 - H cells.
+struct AxisPart {
+    int start, len, origin;
+};
+std::vector<AxisPart> plan_axis(int n, int E, int H, int q) {
+    std::vector<AxisPart> out;
+    if (n <= E - q - 1) {
+        out.push_back(AxisPart{0, n, -q});
+        return out;
+    }
+    const int o = E - 2 * H, e0 = E - q - H, e1 = E - 1 - H;
+    int P = 2;
+    while (e0 + (P - 2) * o + e1 < n) P++;
+    // every part but the last at its largest; the last gets the rest, which
+    // is at most e1 (P is enough) and more than H - 1 (P is the least: with
+    // one part fewer, n > e0 + (P - 3) o + e1, so the rest exceeds e1 - o)
     int start = 0;
-    for (int k = 0; k < parts; k++) {
-        const int len = n / parts + (k < n % parts ? 1 : 0);
-        out.emplace_back(start, len);
+    for (int k = 0; k < P - 1; k++) {
+        const int len = k == 0 ? e0 : o;
+        out.push_back(AxisPart{start, len, k == 0 ? -q : start - H});
         start += len;
     }
+    out.push_back(AxisPart{start, n - start, start - H});
     return out;
-}
-
-// As split(), with every part starting at an even index (tmax even): the
-// blocked solver loads y as aligned pairs.
-std::vector<std::pair<int, int>> split_even(int n, int tmax) {
-    auto pairs = split((n + 1) / 2, tmax / 2);
-    for (auto& p : pairs) {
-        p.first *= 2;
-        p.second *= 2;
-    }
-    pairs.back().second = n - pairs.back().first;
-    return pairs;
 }
 
 int count_tb(stm_ctx* ctx, int variant) {
     const TbVariant& v = TB_VARIANTS[variant];
+    const int H = 2 * v.k + 4;
     int n = 0;
     for (auto& c : ctx->chans)
-        n += (int)(split((int)c.nx, v.oxm).size() * split_even((int)c.ny, v.oym).size());
+        n += (int)(plan_axis((int)c.nx, v.ex, H, 1).size() *
+                   plan_axis((int)c.ny, v.ey, H, 2).size());
     return n;
 }
 
 // Output tiles of the blocked solver for `variant`, channel by channel.
 int plan_tb(stm_ctx* ctx, int variant) {
     const TbVariant& v = TB_VARIANTS[variant];
+    const int H = 2 * v.k + 4;
     std::vector<TbTile> tl;
     for (size_t k = 0; k < ctx->chans.size(); k++) {
         HostChan& c = ctx->chans[k];
         c.tb0 = (int32_t)tl.size();
-        for (auto xs : split((int)c.nx, v.oxm))
-            for (auto ys : split_even((int)c.ny, v.oym))
-                tl.push_back(TbTile{(int32_t)k, xs.first, ys.first, xs.second, ys.second,
-                                    (int32_t)tl.size(), 0, 0});
+        for (auto xs : plan_axis((int)c.nx, v.ex, H, 1))
+            for (auto ys : plan_axis((int)c.ny, v.ey, H, 2))
+                tl.push_back(TbTile{(int32_t)k, xs.start, ys.start, xs.len, ys.len,
+                                    (int32_t)tl.size(), xs.origin, ys.origin});
         c.ntb = (int32_t)tl.size() - c.tb0;
     }
     if ((int)tl.size() > ctx->tb_capacity) {

@@ -41,7 +41,8 @@ void launch_lex_solve(hipStream_t, int nchan, const ChanDesc*, ChanState*, const
 // kernels_rbtb.hip: temporally blocked red-black SOR
 struct TbVariant {
     int k;          // sweeps per launch
-    int oxm, oym;   // largest output tile
+    int oxm, oym;   // largest output tile away from a channel's edges
+    int ex, ey;     // loaded region; the halo is (ex - oxm) / 2 = 2k + 4
 };
 extern const TbVariant TB_VARIANTS[];
 extern const int N_TB_VARIANTS;

@@ -7,6 +7,10 @@
 // ghost refresh, so after K sweeps every cell at distance >= 2K + 1 from the
 // loaded region is exact; the remaining halo covers the final ghost refresh
 // and the residual (H is even so that loaded rows start on a 16-byte pair).
+// Nothing beyond a channel's edge feeds a tile, so on an edge side the region
+// needs no halo: it starts one cell outside the channel (two in y, keeping
+// pairs aligned), which leaves the edge tiles H - 1 more output cells (each
+// tile carries its region origin).
 // The result is bit-identical to K separate global sweeps of
 //     refresh all ghosts; red half-sweep; black half-sweep
 // (same operations on the same operands) while the cache / HBM traffic drops
@@ -252,7 +256,7 @@ k_rb_strip(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
     const double* __restrict__ pin = (L & 1) ? pb : pa;
     double* __restrict__ pout = (L & 1) ? pa : pb;
 
-    const int ex0 = t.x0 - H, ey0 = t.y0 - H;    // ey0 is even
+    const int ex0 = t.rx0, ey0 = t.ry0;          // region origin; ey0 is even
     const int nx = d.nx, ny = d.ny;
     const int64_t pitch = d.pitch, base = d.off;
     const int strip = tid / HY, m = tid - (tid / HY) * HY;
@@ -295,7 +299,8 @@ k_rb_strip(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
                 if (inner && flag_fluid(fl) && gx >= 1 && gx <= nx - 2 && gyq >= 1 && gyq <= ny - 2)
                     code |= C_UPDATE;
                 if (inner && !flag_fluid(fl)) code |= (uint8_t)(flag_edge(fl) << C_EDGE_SHIFT);
-                if (lx >= H && lx < H + t.xn && ly >= H && ly < H + t.yn) code |= C_OUT;
+                if (gx >= t.x0 && gx < t.x0 + t.xn && gyq >= t.y0 && gyq < t.y0 + t.yn)
+                    code |= C_OUT;
             }
             cc[r][q] = code;
         }
@@ -380,9 +385,11 @@ __global__ void k_pbuf_reset(int nchan, ChanState* __restrict__ cs) {
     V(5, 4, 6, 64, 16)       \
     V(6, 8, 6, 64, 16)       \
     V(7, 2, 2, 64, 16)       \
-    V(8, 8, 12, 64, 8)
+    V(8, 8, 12, 64, 8)       \
+    V(9, 6, 6, 48, 16)
 
-#define V_ENTRY(i, K, R, HY, NS) {K, NS * R - 2 * (2 * K + 4), 2 * HY - 2 * (2 * K + 4)},
+#define V_ENTRY(i, K, R, HY, NS) \
+    {K, NS * R - 2 * (2 * K + 4), 2 * HY - 2 * (2 * K + 4), NS * R, 2 * HY},
 const TbVariant TB_VARIANTS[] = {STM_TB_VARIANTS(V_ENTRY)};
 #undef V_ENTRY
 const int N_TB_VARIANTS = sizeof(TB_VARIANTS) / sizeof(TB_VARIANTS[0]);

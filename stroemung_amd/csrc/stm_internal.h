@@ -100,9 +100,9 @@ struct Tile {
 
 // Output tile of the blocked red-black solver: cells [x0, x0+xn) x [y0, y0+yn).
 struct TbTile {
-    int32_t chan, x0, y0, xn, yn;
+    int32_t chan, x0, y0, xn, yn;   // output cells [x0, x0 + xn) x [y0, y0 + yn)
     int32_t idx;                 // position in the full tile list (residual partial slot)
-    int32_t pad1, pad2;
+    int32_t rx0, ry0;            // origin of the loaded region (ry0 even)
 };
 
 constexpr int TILE_X = 4;        // x-rows per tile

@@ -186,7 +186,7 @@ def test_stages_bit_exact_random_fields(seed, nx, ny):
     _cmp(ctx, ch, osim, ("p",), "ghost p")
 
 
-N_VARIANTS = 9
+N_VARIANTS = 10
 
 
 @pytest.mark.parametrize("variant", range(N_VARIANTS))

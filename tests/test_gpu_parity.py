@@ -215,6 +215,31 @@ def test_rb_sweeps_bit_exact(seed, nx, ny, variant):
     _cmp(ctx, ch, osim, ("p",), "19 sweeps")
 
 
+# Grid sides (boundary ring included) on either side of the sizes where the
+# blocked tiling goes from one tile per axis to two: a single tile's region
+# starts outside the channel on both sides (x: side <= region - 2; y, in
+# aligned pairs: side <= region - 3), two tiles use the edge-side room.  The
+# regions are 32, 64 or 96 cells in x and 64, 96 or 128 in y.
+@pytest.mark.parametrize("variant", range(N_VARIANTS))
+@pytest.mark.parametrize("nx,ny", [(30, 125), (31, 126), (62, 61), (63, 62), (94, 93),
+                                   (95, 94)])
+def test_rb_sweeps_bit_exact_at_tile_edges(nx, ny, variant):
+    say("test: red-black sweeps at tile edges", nx, ny, "variant", variant)
+    seed = 20 + nx
+    ct, vel = mixed_grid(seed, nx, ny)
+    fields = random_fields(seed, (nx, ny))
+    kw = dict(delx=0.3, dely=0.2, delt=0.01, gamma=0.9, reynolds=100.0,
+              sor_absolute_epsilon=1e-6, max_iterations=50, omega=1.7, solver=O.SOLVER_RB,
+              check_every=0)
+    ctx, ch, osim = _pair(ct, vel, fields=fields, **kw)
+    assert ctx.solver_variant(variant)[0] == variant
+    assert osim.call("rebuild_boundary_list") == 0
+    ctx.stage("rb_sweeps", 13)
+    osim.set(rhs_mean=ctx.state(ch)["rhs_mean"])
+    osim.call("rb_sweeps", 13)
+    _cmp(ctx, ch, osim, ("p",), "13 sweeps")
+
+
 def test_rb_solve_matches_oracle_iterations():
     say("test: red-black solve, iterations vs oracle")
     ct, vel = presets.lid_cavity(48, 48)

@@ -62,6 +62,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile", action="store_true",
                     help="short fixed run for hardware counters, no JSON line")
+    ap.add_argument("--variant", type=int, default=-1,
+                    help="with --profile: this blocked-solver variant instead of the "
+                         "autotuner's choice (counter collection changes the timings the "
+                         "tuner sees)")
     ap.add_argument("--workload", choices=("cavity", "channels", "learned_cavities",
                                            "learned_channels", "learned_channels_500",
                                            "learned_uneven", "net", "sweeps"),
@@ -192,7 +196,10 @@ def profile_run(a):
     one copy of COPY_PROBE_BYTES (the byte-counter reference)."""
     ctx, ch = make_context(a)
     ctx.tick()
-    ctx.autotune(8)
+    if a.variant >= 0:
+        ctx.solver_variant(a.variant)
+    else:
+        ctx.autotune(8)
     variant, k, tx, ty = ctx.solver_variant()
     ctx.stage("rb_sweeps", 64 * k)
     gbs = ctx.stream_copy(COPY_PROBE_BYTES, 1)

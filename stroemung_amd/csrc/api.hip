@@ -659,8 +659,18 @@ int blocked_solve(stm_ctx* ctx, int check) {
             HIPCHK(ctx, hipEventRecord(ea, ctx->stream));
             ctx->span_open = true;
         }
-        for (int b = 0; b < batch; b++, L++)
-            LAUNCH("rb_tb", launch_blocked(ctx, ctx->variant, L, check ? TB_CHECK : 0));
+        for (int b = 0; b < batch; b++, L++) {
+            const int rc = launch(ctx, "rb_tb", [&] {
+                launch_blocked(ctx, ctx->variant, L, check ? TB_CHECK : 0);
+            });
+            if (rc) {
+                if (span) {
+                    ctx->span_open = false;
+                    ctx->free_events.push_back(ea);
+                }
+                return rc;
+            }
+        }
         if (span) {
             ctx->span_open = false;
             hipEvent_t eb = take_event(ctx);

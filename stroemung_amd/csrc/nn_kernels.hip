@@ -810,9 +810,10 @@ int64_t grad_partials(int cin, int cout, int64_t npix) {
     const int64_t db = std::min<int64_t>((npix + 127) / 128, 257) * cout;
     return std::max(dw, db);
 }
-int64_t loss_partials(int64_t npix_per, int nb) {
+int64_t loss_partials(int64_t npix_per, int nb, int head_c) {
     const int64_t npix = npix_per * nb;
-    return std::max<int64_t>({(int64_t)64 * nb, (int64_t)nblk(npix), (int64_t)nblk(npix) * 33});
+    return std::max<int64_t>(
+        {(int64_t)64 * nb, (int64_t)nblk(npix), (int64_t)nblk(npix) * (head_c + 1)});
 }
 void conv3x3_dw(hipStream_t st, const bf16* gz, int cout, const bf16* in, int cin, float* dw32,
                 int nb, int H, int W, float* part) {

@@ -105,7 +105,7 @@ int Net::init(stm_ctx* c, int H_, int W_, int cap_, uint64_t seed) {
     NCHK(hipMalloc(&wflip, sizeof(bf16) * maxw));
     // the largest conv output is level 0's: cap x H x W x 32 (levels below have
     // a quarter of the pixels and twice the channels)
-    scratch_floats = std::max(splitk_cap(), nn::loss_partials((int64_t)H * W, cap));
+    scratch_floats = std::max(splitk_cap(), nn::loss_partials((int64_t)H * W, cap, 32));
     for (auto& cv : convs)
         scratch_floats = std::max(
             scratch_floats,

@@ -432,7 +432,7 @@ def learned_channels(ticks, log=None):
 
 
 def learned_uneven(ticks, log=None):
-    """The 256 uneven channels of the batched config (64x32 to 512x256, seed
+    """The 256 uneven channels of the batched workload (64x32 to 512x256, seed
     0, eps 1e-5): network trained on the 256 channels of seed 1001, one frame
     of 64 x 32 at coarsening 8 for every size; both runs whole-tick tuned."""
     from stroemung_amd import learned, workloads
@@ -454,7 +454,7 @@ def learned_uneven(ticks, log=None):
 
 
 def learned_channels_500(log=None):
-    """Config-5 trajectory: the 64 channels 512 x 256 of seed 1 advanced 500
+    """The learned + batched trajectory: the 64 channels 512 x 256 of seed 1 advanced 500
     learned ticks from rest, with plain ticks timed on the same state at
     checkpoints; network trained as for learned_channels."""
     from stroemung_amd import learned, workloads

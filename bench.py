@@ -56,7 +56,7 @@ def parse():
     ap.add_argument("--n", type=int, default=1024, help="interior cells per side")
     ap.add_argument("--eps", type=float, default=1e-6)
     ap.add_argument("--check-every", type=int, default=0,
-                    help="sweeps per blocked launch (1/2/4/8); 0 = autotuned")
+                    help="sweeps per blocked launch (1/2/4/6/8); 0 = autotuned")
     ap.add_argument("--sample", type=int, default=16, help="time every n-th launch")
     ap.add_argument("--cpu-sweeps", type=int, default=150)
     ap.add_argument("--no-cpu", action="store_true")

@@ -169,7 +169,7 @@ int stm_rb_sweeps(stm_ctx *ctx, uint32_t n);
 /* ---- red-black solver configuration (no reference counterpart)
  * The red-black solve runs as temporally blocked launches of K sweeps each,
  * with the residual checked after every launch; the variant fixes K and the
- * output tile.  params.check_every = K (1, 2, 4, 8) restricts the choice to
+ * output tile.  params.check_every = K (1, 2, 4, 6, 8) restricts the choice to
  * that K; check_every <= 0 lets the tuner pick any variant. */
 /* variant >= 0 selects it; -1 only queries.  Reports the variant in use. */
 int stm_solver_variant(stm_ctx *ctx, int32_t variant, int32_t *current, int32_t *sweeps,

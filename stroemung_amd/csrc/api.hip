@@ -409,8 +409,9 @@ int plan_tb(stm_ctx* ctx, int variant) {
 }
 
 // The variant for the requested check interval (the sweeps per launch):
-// check_every = K picks the K-sweep variant with 64x64 tiles (48x48 for 8);
-// check_every <= 0 leaves the choice to the tuner, starting from K = 4.
+// check_every = K picks the first K-sweep variant of the list (the tuners
+// may pick another with the same K); check_every <= 0 leaves the choice to
+// the tuners, starting from the first K = 4 variant.
 int default_variant(stm_ctx* ctx) {
     const int want = ctx->chans.empty() ? 4 : ctx->chans[0].prm.check_every;
     for (int v = 0; v < N_TB_VARIANTS; v++)
@@ -1055,7 +1056,7 @@ int check_params(stm_ctx* ctx, const stm_params* q) {
         for (int v = 0; v < N_TB_VARIANTS; v++) ok = ok || TB_VARIANTS[v].k == q->check_every;
         if (!ok)
             return fail(ctx, STM_ERR_INVALID_ARG,
-                        "check_every must be 1, 2, 4 or 8 (or <= 0 to let the tuner pick)");
+                        "check_every must be 1, 2, 4, 6 or 8 (or <= 0 to let the tuner pick)");
     }
     return STM_OK;
 }

@@ -392,3 +392,27 @@ def test_rb_trajectory_within_1e6_of_converged_oracle(name, grid, steps, kw):
             _rel(pm(got["p"]), pm(truth.p), fl))
     say("  rel L2 vs 1e-12 oracle: u %.2e v %.2e p %.2e; sweeps/tick %s" % (errs + (gpu_its,)))
     assert max(errs) < 1e-6
+
+
+def test_check_every_takes_the_sweep_counts_of_the_variants():
+    """params.check_every = K pins the blocked solver to a K-sweep variant;
+    a K that no variant has is refused when the grid is added."""
+    from stroemung_amd import _lib
+    say("test: check_every values")
+    ct, vel = presets.lid_cavity(40, 40)
+    for k in (1, 2, 4, 6, 8):
+        prm = make_params(1 / 40, 1 / 40, 1e-3, 0.9, 100.0, 1e-6, 100, 1.7, solver="rb",
+                          check_every=k)
+        ctx = Context(0)
+        ch = ctx.add_grid(prm, ct, vel)
+        ctx.construct()
+        assert ctx.solver_variant()[1] == k
+        ctx.tick()
+        ctx.close()
+    for k in (3, 5, 16):
+        prm = make_params(1 / 40, 1 / 40, 1e-3, 0.9, 100.0, 1e-6, 100, 1.7, solver="rb",
+                          check_every=k)
+        ctx = Context(0)
+        with pytest.raises(_lib.StmError, match="check_every"):
+            ctx.add_grid(prm, ct, vel)
+        ctx.close()

@@ -149,10 +149,14 @@ int stm_construct(stm_ctx *ctx);
 
 /* Simulation::run_simulation_tick (src/simulation.rs:324-333) for every
  * channel.  sor_iterations / norm_squared receive one value per channel
- * (either may be NULL). */
+ * (either may be NULL).  Returns STM_ERR_NAN, after the tick and with the
+ * outputs filled, when a channel's residual norm is not a number; the
+ * red-black solve stops at the first such residual, where the reference's
+ * loop runs on to itermax and returns Ok with the NaN norm. */
 int stm_run_simulation_tick(stm_ctx *ctx, uint32_t *sor_iterations, double *norm_squared);
 
-/* nticks ticks without reading results back between them. */
+/* nticks ticks without reading results back between them; STM_ERR_NAN as
+ * above for the last tick. */
 int stm_run_ticks(stm_ctx *ctx, uint32_t nticks);
 
 /* ---- the stages of a tick, for callers that drive them one by one */

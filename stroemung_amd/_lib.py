@@ -32,6 +32,11 @@ class InvalidGridError(StmError):
     """The reference's SimulationGridError::BoundaryTooThinError."""
 
 
+class NanError(StmError):
+    """A channel's pressure residual is not a number after a tick (the
+    state is left as computed)."""
+
+
 class Params(C.Structure):
     _fields_ = [
         ("delx", C.c_double), ("dely", C.c_double), ("delt", C.c_double),
@@ -157,4 +162,6 @@ def check(rc, ctx=None, net=None):
         raise NoDeviceError(rc, msg)
     if rc == -3:
         raise InvalidGridError(rc, msg)
+    if rc == -5:
+        raise NanError(rc, msg)
     raise StmError(rc, msg)

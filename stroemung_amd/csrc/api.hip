@@ -1292,6 +1292,18 @@ int stm_construct(stm_ctx* ctx) {
     return read_states(ctx);
 }
 
+// After ticks: STM_ERR_NAN when a channel's last residual is not a number.
+int check_nan(stm_ctx* ctx) {
+    for (int k = 0; k < nch(ctx); k++) {
+        const double r = ctx->h_cs[k].norm2;
+        if (r != r)
+            return fail(ctx, STM_ERR_NAN,
+                        "channel %d: the pressure residual is not a number after %u sweeps", k,
+                        ctx->h_cs[k].sor_iters);
+    }
+    return STM_OK;
+}
+
 int stm_run_simulation_tick(stm_ctx* ctx, uint32_t* sor_iterations, double* norm_squared) {
     int rc = ready(ctx);
     if (rc || (rc = tick(ctx)) || (rc = read_states(ctx))) return rc;
@@ -1299,7 +1311,8 @@ int stm_run_simulation_tick(stm_ctx* ctx, uint32_t* sor_iterations, double* norm
         if (sor_iterations) sor_iterations[k] = ctx->h_cs[k].sor_iters;
         if (norm_squared) norm_squared[k] = ctx->h_cs[k].norm2;
     }
-    return drain(ctx);
+    if ((rc = drain(ctx))) return rc;
+    return check_nan(ctx);
 }
 
 int stm_run_ticks(stm_ctx* ctx, uint32_t nticks) {
@@ -1307,9 +1320,8 @@ int stm_run_ticks(stm_ctx* ctx, uint32_t nticks) {
     if (rc) return rc;
     for (uint32_t t = 0; t < nticks; t++)
         if ((rc = tick(ctx))) return rc;
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-    if ((rc = check_launch(ctx))) return rc;
-    return drain(ctx);
+    if ((rc = read_states(ctx)) || (rc = drain(ctx))) return rc;
+    return check_nan(ctx);
 }
 
 int stm_set_boundary_u_and_v(stm_ctx* ctx) {

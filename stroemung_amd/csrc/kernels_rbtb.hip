@@ -316,7 +316,10 @@ k_rb_strip(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
             for (int i = d.tb0 + tid; i < d.tb0 + d.ntb; i += NT) acc = acc + part_prev[i];
         const double res2 = block_sum<NT>(acc, wsum);
         const bool conv = check && res2 <= s.tol2;
-        if (conv || done >= itermax) {
+        // a residual that is not a number never meets the test: stop there
+        // (the tick then reports it) instead of sweeping to itermax
+        const bool nan = check && res2 != res2;
+        if (conv || nan || done >= itermax) {
             if (t.idx == d.tb0 && tid == 0) {
                 s.stop_launch = L - 1;
                 s.sor_iters = (uint32_t)done;

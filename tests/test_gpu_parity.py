@@ -416,3 +416,33 @@ def test_check_every_takes_the_sweep_counts_of_the_variants():
         with pytest.raises(_lib.StmError, match="check_every"):
             ctx.add_grid(prm, ct, vel)
         ctx.close()
+
+
+@pytest.mark.parametrize("solver", ["rb", "lex"])
+def test_nan_residual_ends_the_solve_and_is_reported(solver):
+    """A NaN in p makes the residual NaN: the tick completes, reports
+    STM_ERR_NAN, and the red-black solve stops at its first residual test
+    instead of sweeping to itermax."""
+    from stroemung_amd import _lib
+    say("test: NaN residual", solver)
+    n = 40
+    ct, vel = presets.lid_cavity(n, n)
+    itermax = 200000 if solver == "rb" else 60
+    prm = make_params(1 / n, 1 / n, 1e-3, 0.9, 100.0, 1e-6, itermax, 1.7, solver=solver,
+                      check_every=0)
+    ctx = Context(0)
+    ch = ctx.add_grid(prm, ct, vel)
+    ctx.construct()
+    ctx.tick()
+    p = ctx.download(ch, ("p",))["p"]
+    p[n // 2, n // 2] = np.nan
+    ctx.upload(ch, p=p)
+    with pytest.raises(_lib.NanError, match="not a number"):
+        ctx.tick()
+    st = ctx.state(ch)
+    assert np.isnan(st["norm_squared"])
+    if solver == "rb":
+        assert st["sor_iterations"] <= 16, st["sor_iterations"]
+    else:   # the reference's loop: no test passes, so it runs to itermax
+        assert st["sor_iterations"] >= itermax - 1
+    ctx.close()

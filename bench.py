@@ -63,9 +63,9 @@ def parse():
     ap.add_argument("--profile", action="store_true",
                     help="short fixed run for hardware counters, no JSON line")
     ap.add_argument("--variant", type=int, default=-1,
-                    help="with --profile: this blocked-solver variant instead of the "
-                         "autotuner's choice (counter collection changes the timings the "
-                         "tuner sees)")
+                    help="with --profile or --workload sweeps: this blocked-solver variant "
+                         "instead of the autotuner's choice (counter collection changes the "
+                         "timings the tuner sees)")
     ap.add_argument("--workload", choices=("cavity", "channels", "learned_cavities",
                                            "learned_channels", "learned_channels_500",
                                            "learned_uneven", "net", "sweeps"),
@@ -221,7 +221,11 @@ def sweeps_run(a):
     rng = np.random.default_rng(0)
     ctx.upload(ch, p=rng.standard_normal((n + 2, n + 2)),
                rhs=rng.standard_normal((n + 2, n + 2)))
-    tuned = ctx.autotune(8)
+    if a.variant >= 0:
+        ctx.solver_variant(a.variant)
+        tuned = (a.variant, None)
+    else:
+        tuned = ctx.autotune(8)
     variant, k, tx, ty = ctx.solver_variant()
     launches = 64
     ctx.stage("rb_sweeps", 8 * k)                     # warm-up

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "nn_sizes.h"
+
 namespace stm {
 namespace nn {
 
@@ -38,10 +40,7 @@ void weight_flip(hipStream_t, const bf16* w, bf16* wt, int cin, int cout);
 // The gradient and loss reductions leave per-block partial sums in `part`
 // (device scratch, reused launch after launch on one stream) and add them in
 // a fixed order, so a gradient is the same from run to run.  Sizes in floats:
-// grad_partials for conv3x3_dw and bias_grad on npix pixels, loss_partials
-// for loss and for head_bwd with head_c input channels.
-int64_t grad_partials(int cin, int cout, int64_t npix);
-int64_t loss_partials(int64_t npix_per, int nb, int head_c);
+// grad_partials and loss_partials (nn_sizes.h).
 // dw32[co][tap][ci] += sum_pix gz[pix][co] * in[pix + off(tap)][ci]
 void conv3x3_dw(hipStream_t, const bf16* gz, int cout, const bf16* in, int cin, float* dw32, int nb,
                 int H, int W, float* part);

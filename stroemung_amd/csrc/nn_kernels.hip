@@ -790,37 +790,12 @@ void weight_flip(hipStream_t st, const bf16* w, bf16* wt, int cin, int cout) {
     const int64_t n = (int64_t)cin * 9 * cout;
     k_weight_flip<<<nblk(n), 256, 0, st>>>(w, wt, cin, cout);
 }
-static int dw_pixels_per_chunk(int64_t npix) {
-    int per = 4096;
-    while (npix / per > 64 && per < (1 << 20)) per *= 2;
-    return per;
-}
-static int bias_pixels_per_block(int64_t npix) {
-    int per = 128;
-    while (npix / per > 256) per *= 2;
-    return per;
-}
-// Bounds over every pixel count up to npix (the chunk counts are not monotone
-// in the pixel count: at most 65 chunks of 4096 << k pixels until k reaches
-// 8, at most 257 bias blocks of 128 << k).
-int64_t grad_partials(int cin, int cout, int64_t npix) {
-    const int64_t chunks =
-        std::min((npix + 4095) / 4096, std::max<int64_t>(65, (npix + (1 << 20) - 1) >> 20));
-    const int64_t dw = chunks > 1 ? chunks * cout * 9 * cin : 0;
-    const int64_t db = std::min<int64_t>((npix + 127) / 128, 257) * cout;
-    return std::max(dw, db);
-}
-int64_t loss_partials(int64_t npix_per, int nb, int head_c) {
-    const int64_t npix = npix_per * nb;
-    return std::max<int64_t>(
-        {(int64_t)64 * nb, (int64_t)nblk(npix), (int64_t)nblk(npix) * (head_c + 1)});
-}
 void conv3x3_dw(hipStream_t st, const bf16* gz, int cout, const bf16* in, int cin, float* dw32,
                 int nb, int H, int W, float* part) {
     const int64_t npix = (int64_t)nb * H * W;
     const int ci_blocks = (cin + 63) / 64;
     const int per = dw_pixels_per_chunk(npix);
-    const int chunks = (int)((npix + per - 1) / per);
+    const int chunks = (int)dw_chunks(npix);   // partials: grad_partials (nn_sizes.h)
     const int64_t nw = (int64_t)cout * 9 * cin;
     dim3 grid(ci_blocks, (cout + 63) / 64, 9 * chunks);
     if (chunks == 1) {
@@ -837,7 +812,7 @@ void bias_grad(hipStream_t st, const bf16* g, int c, int64_t npix, float* db, fl
     // bias channels are 32, 64, 128 or 256; each block leaves c partial sums
     // and at most about one block per CU runs, so the block sums stay short
     const int per = bias_pixels_per_block(npix);
-    const int nblocks = (int)nblk(npix, per);
+    const int nblocks = (int)bias_blocks(npix);   // partials: grad_partials (nn_sizes.h)
     k_bias_grad<<<nblocks, 256, 0, st>>>(g, c, (int)npix, per, part);
     k_block_sum<<<c, 256, 0, st>>>(part, nblocks, c, 0, db, 1);
 }

@@ -124,3 +124,18 @@ def test_bench_refuses_more_than_one_gpu():
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr
+
+
+def test_bench_cpu_baseline_legs_run_on_a_small_grid():
+    """The CPU legs of bench.py (oracle timing, no GPU) on a 48x48 cavity:
+    the fields the JSON line carries, positive rates, and the threaded
+    red-black leg labelled as not the reference algorithm."""
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    sys.path.insert(0, root)
+    import bench
+    base = bench.cpu_baseline(48, 5000, 5)
+    assert base["value"] > 0 and base["unit"] == "cell-updates/s" and base["cores"] == 1
+    assert base["kind"] == "port" and "48x48" in base["sample"]
+    par = bench.cpu_parallel(48, 5000, 5)
+    assert par["value"] > 0 and par["cores"] >= 1 and "not the reference" in par["kind"]

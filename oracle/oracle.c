@@ -21,9 +21,16 @@
  *       initial_norm_squared = 899.9547140394143 (deserialize-2.snap:17).
  *       Diagonal edge kinds and the other Inflow/Outflow orientations appear
  *       in no reference golden; they are pinned by wall invariants (normal
- *       face velocity equals the wall velocity, the ghost-cell average of the
- *       tangential velocity equals the wall velocity, ghost pressure equals
- *       the neighbour mean) and by mirror-image grids (tests/test_oracle_*).
+ *       face velocity equals the wall velocity; ghost plus fluid tangential
+ *       velocity is 0, or twice the wall velocity of a moving wall, for the
+ *       straight and the diagonal kinds; an outflow cell copies its fluid
+ *       faces; ghost pressure equals the neighbour mean) and by the net
+ *       effect of the restore list after ADAP_UV, checked cell by cell
+ *       (tests/test_oracle_extensions.py).
+ *   calculate_pressure_range / calculate_speed_range
+ *       pinned: a numpy brute force of the reference's folds, including an
+ *       all-negative p (the max keeps its 0.0 start) and a grid with no
+ *       fluid cell (tests/test_oracle_extensions.py).
  *   Extensions with no reference counterpart (moving wall, max_uv,
  *   comp_delt, red-black SOR, masked residual, solve_rb)
  *       pinned by closed forms, a manufactured Poisson problem with a known

@@ -229,3 +229,139 @@ def test_threaded_red_black_sweeps_equal_the_serial_ones(threads):
         runs.append((s.p.copy(), p0, r0))
     np.testing.assert_array_equal(runs[0][0], runs[1][0])
     assert not np.array_equal(runs[0][0], runs[0][1])    # the sweeps changed p
+
+
+# ------------------------------------------------ diagonal kinds and restore list
+
+NE_, SW_, NW_ = 2, 6, 8
+
+
+def _wall_velocity(t, vel, x, y):
+    return tuple(vel[x, y]) if t in (O.INFLOW, O.MOVING_WALL) else (0.0, 0.0)
+
+
+def test_setbcond_diagonal_tangential_ghosts():
+    """The tangential ghost of each diagonal kind (src/grid/mod.rs:446-488,
+    544-586): NorthEast v[b] = -v[E], SouthWest u[b] = -u[S], NorthWest
+    u[b] = -u[N] and v[b] = -v[W] for NoSlip and Inflow (the sum of ghost and
+    fluid value is exactly 0); 2 * wall velocity for the moving wall.  The
+    N, S and E sources hold their pre-pass values when the cell is visited;
+    v[W] may already have been set by the wall cell at (x-1, y+1).  A rule
+    reading any other neighbour fails: the fields are random."""
+    seen = {}
+    for seed in range(8):
+        ct, vel = mixed_grid(300 + seed, 44, 34, n_rect=10)
+        s = _sim(ct, vel, seed)
+        u0, v0 = s.u.copy(), s.v.copy()
+        assert s.call("set_boundary_u_and_v") == 0
+        u, v, edge = s.u, s.v, s.edge
+        for x in range(ct.shape[0]):
+            for y in range(ct.shape[1]):
+                t, e = int(ct[x, y]), int(edge[x, y])
+                if t in (O.FLUID, O.OUTFLOW) or e not in (NE_, SW_, NW_):
+                    continue
+                bu, bv = _wall_velocity(t, vel, x, y)
+                mv = t == O.MOVING_WALL
+                sums = []
+                if e == NE_:
+                    sums.append((v[x, y] + v0[x + 1, y], bv))
+                if e == SW_:
+                    sums.append((u[x, y] + u0[x, y + 1], bu))
+                if e == NW_:
+                    sums.append((u[x, y] + u0[x, y - 1], bu))
+                    sums.append((v[x, y] + v[x - 1, y], bv))
+                for got, w in sums:
+                    if mv:
+                        assert got == pytest.approx(2 * w, abs=1e-15), (seed, x, y, e)
+                    else:
+                        assert got == 0.0, (seed, x, y, e, t)
+                seen[(e, t)] = seen.get((e, t), 0) + 1
+    for e in (NE_, SW_, NW_):
+        for t in (O.NOSLIP, O.INFLOW, O.MOVING_WALL):
+            assert seen.get((e, t), 0) > 0, ("kind never exercised", e, t)
+
+
+def test_restore_list_net_effect_after_adap_uv():
+    """src/simulation.rs:287-322 with the list of src/grid/mod.rs:595-648.
+    After ADAP_UV every cell of [0, nx-2] x [0, ny-2] that is not a boundary
+    cell holds the plain ADAP_UV value (fluid faces written by SETBCOND are
+    not reset).  A boundary cell holds what SETBCOND left in it, except that
+    a cell whose kind has a West side (SW, W, NW) receives u of its west
+    fluid face and one with a North side (N, NE, NW) v of its north fluid
+    face, as SETBCOND left them (the list carries the face's value under the
+    boundary cell's own index).  A restore written to the face, or the
+    wrong face's value, fails."""
+    checked = {}
+    for seed in range(6):
+        ct, vel = mixed_grid(400 + seed, 42, 31, n_rect=10)
+        s = _sim(ct, vel, seed, delt=0.0173, delx=0.31, dely=0.23)
+        rng = np.random.default_rng(seed + 50)
+        s.f[...] = rng.uniform(-1, 1, ct.shape)
+        s.g[...] = rng.uniform(-1, 1, ct.shape)
+        assert s.call("set_boundary_u_and_v") == 0
+        us, vs = s.u.copy(), s.v.copy()
+        f, g, p = s.f.copy(), s.g.copy(), s.p.copy()
+        s.call("set_u_and_v")
+        dt = s.s.delt
+        eu, ev = us.copy(), vs.copy()
+        eu[:-1, :-1] = f[:-1, :-1] - ((dt / 0.31) * (p[1:, :-1] - p[:-1, :-1]))
+        ev[:-1, :-1] = g[:-1, :-1] - ((dt / 0.23) * (p[:-1, 1:] - p[:-1, :-1]))
+        for x in range(ct.shape[0]):
+            for y in range(ct.shape[1]):
+                if ct[x, y] == O.FLUID:
+                    continue
+                e = int(s.edge[x, y])
+                eu[x, y] = us[x - 1, y] if e in (SW_, 7, NW_) else us[x, y]
+                ev[x, y] = vs[x, y - 1] if e in (1, NE_, NW_) else vs[x, y]
+                checked[e] = checked.get(e, 0) + 1
+        np.testing.assert_array_equal(s.u.view(np.uint64), eu.view(np.uint64))
+        np.testing.assert_array_equal(s.v.view(np.uint64), ev.view(np.uint64))
+    assert all(checked.get(e, 0) > 0 for e in range(9)), checked
+
+
+def _ranges_brute(ct, u, v, p):
+    """The reference's folds (src/grid/mod.rs:237-268): over fluid cells,
+    accumulators start at (f64::MAX, 0.0); the speed range takes sqrt at the
+    end."""
+    fl = ct == O.FLUID
+    big = np.finfo(np.float64).max
+    if fl.any():
+        sp = u[fl] * u[fl] + v[fl] * v[fl]
+        pr = (min(big, float(p[fl].min())), max(0.0, float(p[fl].max())))
+        sr = (math.sqrt(min(big, float(sp.min()))), math.sqrt(max(0.0, float(sp.max()))))
+    else:
+        pr, sr = (big, 0.0), (math.sqrt(big), 0.0)
+    return pr, sr
+
+
+@pytest.mark.parametrize("case", ["mixed", "all_negative_p", "all_positive_p", "no_fluid",
+                                  "one_fluid_cell"])
+def test_ranges_brute_force(case):
+    """calculate_pressure_range / calculate_speed_range against a numpy
+    brute force, including an all-negative p (max stays at the 0.0 start)
+    and an all-positive p (min comes from the data, not f64::MAX)."""
+    ct, vel = mixed_grid(77, 36, 28)
+    if case == "no_fluid":
+        ct = np.full_like(ct, O.NOSLIP)
+    if case == "one_fluid_cell":
+        ct = np.full_like(ct, O.NOSLIP)
+        ct[10, 10] = O.FLUID
+    rng = np.random.default_rng(9)
+    nx, ny = ct.shape
+    s = O.Sim(ct, vel, delx=0.1, dely=0.1, delt=0.01, gamma=0.9, reynolds=100.0,
+              sor_absolute_epsilon=1e-3, max_iterations=10, omega=1.7,
+              u=rng.uniform(-2, 2, (nx, ny)), v=rng.uniform(-2, 2, (nx, ny)),
+              p=rng.uniform(-5, 5, (nx, ny)))
+    if case == "all_negative_p":
+        s.p[...] = -rng.uniform(1, 5, (nx, ny))
+    if case == "all_positive_p":
+        s.p[...] = rng.uniform(1, 5, (nx, ny))
+    s.call("calculate_pressure_range")
+    s.call("calculate_speed_range")
+    pr, sr = _ranges_brute(ct, s.u, s.v, s.p)
+    assert tuple(s.s.pressure_range) == pr
+    assert tuple(s.s.speed_range) == sr
+    if case == "all_negative_p":
+        assert pr[1] == 0.0 and pr[0] < -1.0
+    if case == "all_positive_p":
+        assert pr[0] >= 1.0

@@ -124,7 +124,10 @@ int stm_add_grid(stm_ctx *ctx, const stm_params *params, uint32_t nx, uint32_t n
                  const uint8_t *cell_type, const double *cell_velocity, uint32_t *channel_out);
 
 /* Replaces the cell types of a channel after an edit and rebuilds its
- * boundary list (the GUI edit path, src/lib.rs:38-78). */
+ * boundary list: the GUI edit path draw_cells (src/lib.rs:38-78).  As there,
+ * u, v and p of every cell whose type changes are set to 0 (on the device),
+ * and a too-thin boundary (STM_ERR_INVALID_GRID, BoundaryTooThinError) leaves
+ * the grid and the fields as they were. */
 int stm_set_cells(stm_ctx *ctx, uint32_t channel, const uint8_t *cell_type,
                   const double *cell_velocity);
 

@@ -64,7 +64,7 @@ struct stm_ctx {
     std::vector<HostChan> chans;
     bool finalized = false;
     int64_t total = 0;             // elements per packed field
-    int ntiles = 0, nbtot = 0, check_every = 1;
+    int ntiles = 0, nbtot = 0;
     bool any_rb = false, any_lex = false;
     double *u = nullptr, *v = nullptr, *p = nullptr, *f = nullptr, *g = nullptr, *rhs = nullptr;
     double* scratch = nullptr;
@@ -113,10 +113,15 @@ struct stm_ctx {
     // running-channel count of recent learned cycles, read back without draining the queue
     int32_t* h_live_ring = nullptr;  // pinned, POLL_RING entries
     hipEvent_t poll_evs[4] = {};
-    // learned cycles replayed from a captured hipGraph
+    // learned cycles replayed from a captured hipGraph.  A captured launch
+    // holds its pointers and counts by value, so every change of a buffer or
+    // count a cycle uses bumps graph_gen (and drops the graph), and the key
+    // also names the net by its serial number, not its address
     bool use_graphs = false;
     hipGraphExec_t cycle_exec = nullptr;
     std::vector<int64_t> cycle_key;  // what the captured cycle depends on
+    uint64_t graph_gen = 0;
+    std::vector<stm_net*> nets;      // nets created on this context, detached on destroy
     // profiling
     uint32_t sample = 0;
     bool span_open = false;   // a batch is timed as a whole: no per-launch events
@@ -147,6 +152,18 @@ int fail(stm_ctx* ctx, int code, const char* fmt, ...) {
     } while (0)
 
 int64_t round_up(int64_t a, int64_t b) { return (a + b - 1) / b * b; }
+
+// A buffer or count that captured learned cycles use is about to change:
+// drop the graph (after the launches queued from it) and move the key on.
+void invalidate_graphs(stm_ctx* ctx) {
+    if (ctx->cycle_exec) {
+        hipStreamSynchronize(ctx->stream);
+        hipGraphExecDestroy(ctx->cycle_exec);
+        ctx->cycle_exec = nullptr;
+    }
+    ctx->cycle_key.clear();
+    ctx->graph_gen++;
+}
 
 // src/grid/mod.rs:167-200 and 270-332: the edge kind of boundary cell (x, y)
 // from which of its four neighbours are fluid; -2 if too thin.
@@ -293,6 +310,7 @@ void fill_entries(const HostChan& c, int chan, std::vector<BEntry>& out) {
 }
 
 int upload_boundaries(stm_ctx* ctx) {
+    invalidate_graphs(ctx);   // ghost refreshes capture d_be and nbtot
     std::vector<BEntry> be;
     int boff = 0;
     for (size_t k = 0; k < ctx->chans.size(); k++) {
@@ -350,6 +368,7 @@ int count_tb(stm_ctx* ctx, int variant) {
 
 // Output tiles of the blocked solver for `variant`, channel by channel.
 int plan_tb(stm_ctx* ctx, int variant) {
+    invalidate_graphs(ctx);   // blocked launches capture d_tb and its length
     const TbVariant& v = TB_VARIANTS[variant];
     const int H = 2 * v.k + 4;
     std::vector<TbTile> tl;
@@ -389,6 +408,15 @@ int default_variant(stm_ctx* ctx) {
     return 2;
 }
 
+// Which solvers the channels use (a tick launches only what some channel needs).
+void update_solvers(stm_ctx* ctx) {
+    ctx->any_rb = ctx->any_lex = false;
+    for (const auto& c : ctx->chans) {
+        if (c.prm.solver == STM_SOLVER_RB) ctx->any_rb = true;
+        else ctx->any_lex = true;
+    }
+}
+
 bool variant_allowed(stm_ctx* ctx, int v) {
     const int want = ctx->chans[0].prm.check_every;
     return want <= 0 || TB_VARIANTS[v].k == want;
@@ -400,7 +428,6 @@ int finalize(stm_ctx* ctx) {
     int64_t off = 0;
     int tiles = 0;
     std::vector<Tile> tl;
-    ctx->any_rb = ctx->any_lex = false;
     for (size_t k = 0; k < ctx->chans.size(); k++) {
         HostChan& c = ctx->chans[k];
         c.pitch = (uint32_t)round_up(c.ny, 16);
@@ -412,9 +439,8 @@ int finalize(stm_ctx* ctx) {
                 tl.push_back(Tile{(int32_t)k, (int32_t)x0, (int32_t)y0, 0});
         c.ntiles = (int)tl.size() - tiles;
         tiles = (int)tl.size();
-        if (c.prm.solver == STM_SOLVER_RB) ctx->any_rb = true; else ctx->any_lex = true;
     }
-    ctx->check_every = std::max(1, ctx->chans[0].prm.check_every);
+    update_solvers(ctx);
     ctx->total = off;
     ctx->ntiles = tiles;
     const size_t bytes = sizeof(double) * (size_t)off;
@@ -703,16 +729,20 @@ int learn_phase(stm_ctx* ctx, int budget, int final_phase) {
     LAUNCH("learn_arm", launch_learn_arm(ctx->stream, nch(ctx), ctx->d_cd, ctx->d_cs, ctx->d_lc,
                                          ctx->d_active, budget));
     if (budget > 0) {
+        // no channel sweeps past its itermax (k_learn_arm clamps each one's
+        // budget), so launches beyond the largest itermax would only return
+        int most = 0;
+        for (const auto& c : ctx->chans) most = std::max(most, (int)c.prm.max_iterations);
         const int K = TB_VARIANTS[ctx->variant].k;
-        const int nl = (budget + K - 1) / K;
+        const int nl = (std::min(budget, most) + K - 1) / K;
         for (int L = 0; L < nl; L++)
             LAUNCH("rb_tb", launch_blocked(ctx, ctx->variant, L, 0));
         LAUNCH("learn_phase_end", launch_learn_phase_end(ctx->stream, nch(ctx), ctx->d_cs,
-                                                         ctx->d_lc, nl, budget));
-        // an even number of launches leaves p in its own buffer
-        if (nl & 1)
-            LAUNCH("pbuf_fix", launch_pbuf_fix(ctx->stream, ctx->ntiles, nch(ctx), ctx->d_cd,
-                                               ctx->d_cs, ctx->d_tiles, ctx->p2, ctx->p));
+                                                         ctx->d_lc, nl));
+        // the channels whose latest p is in the second buffer copy it back
+        // (after an odd number of launches, or a clamped budget's stop)
+        LAUNCH("pbuf_fix", launch_pbuf_fix(ctx->stream, ctx->ntiles, nch(ctx), ctx->d_cd,
+                                           ctx->d_cs, ctx->d_tiles, ctx->p2, ctx->p));
         if (!final_phase) return STM_OK;   // the sweeps are counted by learn_phase_end
     } else {
         int rc = blocked_solve(ctx, 1);
@@ -769,8 +799,9 @@ int learned_cycle(stm_ctx* ctx, Net& net, int f) {
 }
 
 int graph_cycle(stm_ctx* ctx, Net& net, int f) {
-    const std::vector<int64_t> key = {(int64_t)(intptr_t)&net, f, ctx->lsweeps, ctx->lstep,
-                                      ctx->variant, net.H, net.W, nch(ctx)};
+    const std::vector<int64_t> key = {(int64_t)ctx->graph_gen, (int64_t)net.serial, f,
+                                      ctx->lsweeps, ctx->lstep, ctx->variant, net.H, net.W,
+                                      nch(ctx), (int64_t)(intptr_t)ctx->tb_cur, ctx->ntb_cur};
     if (!ctx->cycle_exec || key != ctx->cycle_key) {
         if (ctx->cycle_exec) HIPCHK(ctx, hipGraphExecDestroy(ctx->cycle_exec));
         ctx->cycle_exec = nullptr;
@@ -871,6 +902,9 @@ int phased_solve(stm_ctx* ctx) {
         if ((rc = learn_phase(ctx, budget, 0))) return rc;
         if (L && (rc = post_live(ctx, k))) return rc;
     }
+    // the residual after the last budgeted phase: channels that converged
+    // there, or used up itermax, end before the final phase
+    if ((rc = learn_prepare(ctx, 0))) return rc;
     if ((rc = learn_phase(ctx, 0, 1))) return rc;
     for (int j = 0; j < corrected; j++) {   // the final phase has synchronised
         float ms = 0.f;
@@ -894,6 +928,7 @@ int phased_solve(stm_ctx* ctx) {
 
 // Buffers of the learned solve for a net of H x W (and of capture).
 int learn_buffers(stm_ctx* ctx, const Net& net, bool capture) {
+    invalidate_graphs(ctx);   // the coarse frames may be reallocated below
     const size_t bytes = sizeof(double) * (size_t)ctx->total;
     if (!ctx->d_lc) {
         HIPCHK(ctx, hipMalloc(&ctx->d_lc, sizeof(LearnChan) * nch(ctx)));
@@ -1073,6 +1108,11 @@ void stm_destroy(stm_ctx* ctx) {
     if (!ctx) return;
     hipSetDevice(ctx->device);
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    // nets outlive their context only to be destroyed: they no longer refer to it
+    for (stm_net* n : ctx->nets) {
+        n->net.ctx = nullptr;
+        n->net.stream = nullptr;
+    }
     drain(ctx);
     for (hipEvent_t e : ctx->free_events) hipEventDestroy(e);
     void* bufs[] = {ctx->u, ctx->v, ctx->p, ctx->f, ctx->g, ctx->rhs, ctx->scratch, ctx->p2,
@@ -1145,15 +1185,37 @@ int stm_set_cells(stm_ctx* ctx, uint32_t ch, const uint8_t* cell_type,
     const size_t n = (size_t)c.nx * c.ny;
     trial.type.assign(cell_type, cell_type + n);
     if (cell_velocity) trial.vel.assign(cell_velocity, cell_velocity + 2 * n);
-    if ((rc = classify_chan(ctx, trial))) return rc;   // the old grid stays on error
+    // the old grid and fields stay on error (draw_cells' rollback)
+    if ((rc = classify_chan(ctx, trial))) return rc;
+    // cells whose type changes start from rest: u = v = p = 0 (draw_cells,
+    // src/lib.rs:52-64); before the first upload every field is still zero
+    std::vector<int64_t> changed;
+    if (ctx->finalized)
+        for (uint32_t x = 0; x < c.nx; x++)
+            for (uint32_t y = 0; y < c.ny; y++)
+                if (trial.type[(size_t)x * c.ny + y] != c.type[(size_t)x * c.ny + y])
+                    changed.push_back(c.off + (int64_t)x * c.pitch + y);
     c = std::move(trial);
     if (!ctx->finalized) return STM_OK;
     if ((rc = ready(ctx))) return rc;
     if ((rc = upload_boundaries(ctx))) return rc;
     if ((rc = upload_flags(ctx, c))) return rc;
     if ((rc = upload_descs(ctx))) return rc;
+    if (!changed.empty()) {
+        int64_t* d_idx = nullptr;
+        HIPCHK(ctx, hipMalloc(&d_idx, sizeof(int64_t) * changed.size()));
+        hipError_t e = hipMemcpyAsync(d_idx, changed.data(), sizeof(int64_t) * changed.size(),
+                                      hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess) {
+            launch_zero_cells(ctx->stream, (int)changed.size(), d_idx, ctx->u, ctx->v, ctx->p);
+            e = hipStreamSynchronize(ctx->stream);
+        }
+        hipFree(d_idx);
+        if (e != hipSuccess)
+            return fail(ctx, STM_ERR_HIP, "zeroing edited cells: %s", hipGetErrorString(e));
+    }
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-    return STM_OK;
+    return check_launch(ctx);
 }
 
 int stm_set_params(stm_ctx* ctx, uint32_t ch, const stm_params* params) {
@@ -1162,13 +1224,18 @@ int stm_set_params(stm_ctx* ctx, uint32_t ch, const stm_params* params) {
     if ((rc = check_params(ctx, params))) return rc;
     if (params->check_every != ctx->chans[0].prm.check_every && ctx->chans.size() > 1)
         return fail(ctx, STM_ERR_INVALID_ARG, "all channels must share check_every");
-    if (ctx->finalized && (params->solver == STM_SOLVER_RB) != (ctx->chans[ch].prm.solver == STM_SOLVER_RB))
-        ctx->any_rb = ctx->any_lex = true;
+    const bool new_k = params->check_every != ctx->chans[ch].prm.check_every;
     ctx->chans[ch].prm = *params;
-    if (ch == 0) ctx->check_every = std::max(1, params->check_every);
+    update_solvers(ctx);
     if (!ctx->finalized) return STM_OK;
     if ((rc = ready(ctx))) return rc;
-    if ((rc = upload_descs(ctx))) return rc;
+    // a new check interval takes the blocked-solver variant with that many
+    // sweeps per launch (<= 0: the first K = 4 one, until a tuner picks)
+    if (new_k) {
+        if ((rc = plan_tb(ctx, default_variant(ctx)))) return rc;
+    } else if ((rc = upload_descs(ctx))) {
+        return rc;
+    }
     if ((rc = read_states(ctx))) return rc;
     ChanState s = ctx->h_cs[ch];
     s.delt = params->delt;
@@ -1570,6 +1637,7 @@ int stm_set_learned(stm_ctx* ctx, stm_net* net, uint32_t cycles, uint32_t sweeps
                     uint32_t coarsen) {
     int rc = ready(ctx);
     if (rc) return rc;
+    invalidate_graphs(ctx);
     if (!net || cycles == 0) {
         ctx->lnet = nullptr;
         ctx->lcycles = 0;
@@ -1616,6 +1684,7 @@ int stm_render(stm_ctx* ctx, uint32_t ch, int32_t mode, uint8_t* rgba) {
 int stm_set_graphs(stm_ctx* ctx, int32_t on) {
     int rc = ready(ctx);
     if (rc) return rc;
+    invalidate_graphs(ctx);
     ctx->use_graphs = on != 0;
     return STM_OK;
 }
@@ -1624,6 +1693,7 @@ int stm_set_capture(stm_ctx* ctx, stm_net* net, uint32_t points, uint32_t sweeps
                     uint32_t coarsen) {
     int rc = ready(ctx);
     if (rc) return rc;
+    invalidate_graphs(ctx);
     if (!net) {
         ctx->cnet = nullptr;
         return STM_OK;
@@ -1657,4 +1727,18 @@ int stm_reset_stats(stm_ctx* ctx) {
 namespace stm {
 hipStream_t ctx_stream(stm_ctx* ctx) { return ctx->stream; }
 int ctx_device(stm_ctx* ctx) { return ctx->device; }
+
+void ctx_attach_net(stm_ctx* ctx, stm_net* net) { ctx->nets.push_back(net); }
+
+// A net is being destroyed: the context stops using it (learned solve and
+// capture off if they used it) and drops graphs that captured its buffers.
+void ctx_detach_net(stm_ctx* ctx, stm_net* net) {
+    ctx->nets.erase(std::remove(ctx->nets.begin(), ctx->nets.end(), net), ctx->nets.end());
+    if (ctx->lnet == net) {
+        ctx->lnet = nullptr;
+        ctx->lcycles = 0;
+    }
+    if (ctx->cnet == net) ctx->cnet = nullptr;
+    invalidate_graphs(ctx);
+}
 }  // namespace stm

@@ -23,6 +23,8 @@ void launch_ranges(hipStream_t, int ntiles, int nchan, const ChanDesc*, ChanStat
 void launch_delt(hipStream_t, int nchan, const ChanDesc*, ChanState*);
 void launch_advance_time(hipStream_t, int nchan, ChanState*);
 void launch_fill(hipStream_t, int64_t n, double* a, double val);
+void launch_zero_cells(hipStream_t, int n, const int64_t* idx, double* u, double* v,
+                       double* p);
 void launch_copy(hipStream_t, int64_t bytes, const double* src, double* dst);
 
 // kernels_pressure.hip
@@ -92,8 +94,7 @@ void launch_learn_apply(hipStream_t, int ntiles, const ChanDesc*, const Tile*, c
 void launch_learn_arm(hipStream_t, int nch, const ChanDesc*, ChanState*, const LearnChan*,
                       int32_t* active, int budget);
 void launch_learn_collect(hipStream_t, int nch, ChanState*, LearnChan*, int final_phase);
-void launch_learn_phase_end(hipStream_t, int nch, ChanState*, LearnChan*, int launches,
-                            int budget);
+void launch_learn_phase_end(hipStream_t, int nch, ChanState*, LearnChan*, int launches);
 void launch_learn_target(hipStream_t, int ntiles, int nch, const ChanDesc*, const Tile*,
                          LearnChan*, const uint8_t* flags, const double* p, const double* p0,
                          double* part, int H, int W, int f, float* dt, int slot0, int dcap,

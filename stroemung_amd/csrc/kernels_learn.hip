@@ -204,6 +204,13 @@ k_learn_check(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
         s.res2 = res2;
         s.norm2 = res2 / d.fluid_cells;
         s.exhausted = 0;
+    } else if (l.run && l.sweeps >= d.itermax) {
+        // the budgeted phases used up itermax: the solve ends here, exhausted,
+        // with this residual (the plain solve's result at itermax)
+        l.run = 0;
+        s.res2 = res2;
+        s.norm2 = res2 / d.fluid_cells;
+        s.exhausted = 1;
     }
     l.alpha = 0.0;
     l.beta = 0.0;
@@ -542,9 +549,11 @@ __global__ void k_learn_arm(const ChanDesc* __restrict__ cd, ChanState* __restri
     const LearnChan& l = lc[c];
     s.pbuf = 0;
     s.sor_iters = 0;
-    if (l.run) {
-        const int rest = cd[c].itermax - l.sweeps;
-        s.budget = budget > 0 ? budget : (rest > 1 ? rest : 1);
+    // the sweeps left below itermax bound every phase (k_learn_check ends a
+    // solve whose phases used them all before it is armed again)
+    const int rest = cd[c].itermax - l.sweeps;
+    if (l.run && rest > 0) {
+        s.budget = (budget > 0 && budget < rest) ? budget : rest;
         s.active = 1;
         s.stop_launch = -1;
         s.exhausted = 0;
@@ -576,22 +585,27 @@ __global__ void k_learn_collect(ChanState* __restrict__ cs, LearnChan* __restric
     }
 }
 
-// End of a budgeted phase run as `launches` unchecked blocked launches: the
-// running channels swept `budget` more times (counted here) and their latest
-// p is in the second buffer when `launches` is odd.  They stay running: the
-// next cycle's residual decides whether they have converged.
+// End of a budgeted phase run as `launches` unchecked blocked launches: each
+// running channel swept its armed budget (the phase's, or less where itermax
+// is near; counted here).  A channel whose budget ended inside the phase was
+// stopped by the launch that took that decision, which recorded where its
+// latest p is; otherwise it is in the second buffer when `launches` is odd.
+// They stay running: the next residual test decides whether they have
+// converged (or used up itermax).
 __global__ void k_learn_phase_end(ChanState* __restrict__ cs, LearnChan* __restrict__ lc,
-                                  int nch, int launches, int budget)
+                                  int nch, int launches)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nch || !lc[c].run) return;
     ChanState& s = cs[c];
-    lc[c].sweeps += budget;
-    s.pbuf = launches & 1;
-    s.sor_iters = (uint32_t)budget;
+    lc[c].sweeps += s.budget;
+    if (s.stop_launch < 0) {
+        s.pbuf = launches & 1;
+        s.stop_launch = launches - 1;
+    }
+    s.sor_iters = (uint32_t)s.budget;
     s.exhausted = 1;
     s.active = 0;
-    s.stop_launch = launches - 1;
 }
 
 // Capture: slot 5 = sum of p - p0 over interior fluid cells.
@@ -704,8 +718,8 @@ void launch_learn_arm(hipStream_t st, int nch, const ChanDesc* cd, ChanState* cs
     k_learn_arm<<<nb256(nch), 256, 0, st>>>(cd, cs, lc, active, nch, budget);
 }
 void launch_learn_phase_end(hipStream_t st, int nch, ChanState* cs, LearnChan* lc,
-                            int launches, int budget) {
-    k_learn_phase_end<<<nb256(nch), 256, 0, st>>>(cs, lc, nch, launches, budget);
+                            int launches) {
+    k_learn_phase_end<<<nb256(nch), 256, 0, st>>>(cs, lc, nch, launches);
 }
 void launch_learn_collect(hipStream_t st, int nch, ChanState* cs, LearnChan* lc, int final_phase) {
     k_learn_collect<<<nb256(nch), 256, 0, st>>>(cs, lc, nch, final_phase);

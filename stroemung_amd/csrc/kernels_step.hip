@@ -395,6 +395,19 @@ void launch_copy(hipStream_t st, int64_t bytes, const double* src, double* dst) 
                                      reinterpret_cast<double2*>(dst));
 }
 
+// u = v = p = 0 at the listed cells (a grid edit's changed cells).
+__global__ void __launch_bounds__(256)
+k_zero_cells(int n, const int64_t* __restrict__ idx, double* __restrict__ u,
+             double* __restrict__ v, double* __restrict__ p)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int64_t i = idx[k];
+    u[i] = 0.0;
+    v[i] = 0.0;
+    p[i] = 0.0;
+}
+
 __global__ void __launch_bounds__(256)
 k_fill(int64_t n, double* __restrict__ a, double val)
 {
@@ -439,6 +452,10 @@ void launch_delt(hipStream_t st, int nchan, const ChanDesc* cd, ChanState* cs) {
 }
 void launch_advance_time(hipStream_t st, int nchan, ChanState* cs) {
     k_advance_time<<<(nchan + 63) / 64, 64, 0, st>>>(nchan, cs);
+}
+void launch_zero_cells(hipStream_t st, int n, const int64_t* idx, double* u, double* v,
+                       double* p) {
+    if (n > 0) k_zero_cells<<<(n + 255) / 256, 256, 0, st>>>(n, idx, u, v, p);
 }
 void launch_fill(hipStream_t st, int64_t n, double* a, double val) {
     if (n > 0) k_fill<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, a, val);

@@ -40,6 +40,7 @@ struct Act {
 
 struct Net {
     stm_ctx* ctx = nullptr;
+    uint64_t serial = 0;         // unique per net in the process (graph keys)
     hipStream_t stream = nullptr;
     int H = 0, W = 0, cap = 0;   // spatial size and batch capacity
     std::vector<Conv> convs;
@@ -93,3 +94,8 @@ struct Net {
 struct stm_net {
     stm::Net net;
 };
+
+namespace stm {
+void ctx_attach_net(stm_ctx* ctx, stm_net* net);
+void ctx_detach_net(stm_ctx* ctx, stm_net* net);
+}  // namespace stm

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -375,12 +376,16 @@ int stm_net_create(stm_ctx* ctx, uint32_t height, uint32_t width, uint32_t batch
         delete n;
         return rc;
     }
+    static std::atomic<uint64_t> next_serial{1};
+    n->net.serial = next_serial++;
+    stm::ctx_attach_net(ctx, n);
     *out = n;
     return STM_OK;
 }
 
 void stm_net_destroy(stm_net* n) {
     if (!n) return;
+    if (n->net.ctx) stm::ctx_detach_net(n->net.ctx, n);
     hipStreamSynchronize(n->net.stream);
     n->net.release();
     delete n;

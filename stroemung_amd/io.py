@@ -146,11 +146,12 @@ def serde_float(text):
     return _f64_from_parts(positive, significand, exponent)
 
 
-def load_reference_json(fh_or_text):
+def load_reference_json(fh_or_text, exact=False):
     """Read JSON written for the reference (e.g. a `--sim-file`), parsing
-    floats as the reference's serde_json does."""
+    floats as the reference's serde_json does (exact=True: correctly rounded,
+    so shortest round-trip decimals read back bit for bit)."""
     text = fh_or_text if isinstance(fh_or_text, str) else fh_or_text.read()
-    return json.loads(text, parse_float=serde_float)
+    return json.loads(text, parse_float=float if exact else serde_float)
 
 
 # ------------------------------------------------------------ NaSt2D output

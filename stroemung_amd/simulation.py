@@ -260,9 +260,16 @@ class Simulation:
 
     # --------------------------------------------------------------- I/O
     @classmethod
-    def from_reader(cls, fh, **ext):
-        """Simulation::from_reader (src/simulation.rs:117-120): serde JSON."""
-        d = sio.load_reference_json(fh)
+    def from_reader(cls, fh, exact=False, **ext):
+        """Simulation::from_reader (src/simulation.rs:117-120): serde JSON.
+
+        Floats are parsed as the reference's serde_json parses them, which
+        can move a 17-digit decimal by one ulp; exact=True parses them
+        correctly rounded instead, so a state written by to_json() resumes
+        bit for bit (f, g and rhs are not in the JSON: construction
+        recomputes them, and a tick recomputes every value of them it
+        reads)."""
+        d = sio.load_reference_json(fh, exact=exact)
         g = d["grid"]
         ct, vel = sio.decode_cells(g["cell_type"])
         return cls(size=d["size"], cell_size=d["cell_size"], delt=d["delt"], gamma=d["gamma"],
@@ -307,8 +314,12 @@ class Simulation:
         self._cache = None
 
     def set_cell_type(self, cell_type, cell_velocity=None):
-        """Edit the grid and rebuild the boundary list (src/lib.rs:38-78)."""
+        """Edit the grid and rebuild the boundary list, as draw_cells does
+        (src/lib.rs:38-78): u, v and p of every cell whose type changes
+        become 0; a too-thin boundary raises InvalidGridError and leaves the
+        grid and the fields unchanged."""
         self.ctx.set_cells(self.channel, cell_type, cell_velocity)
+        self._cache = None
         self.cell_type = np.ascontiguousarray(cell_type, dtype=np.uint8)
         if cell_velocity is not None:
             self.cell_velocity = _f64(cell_velocity, self.size + (2,))

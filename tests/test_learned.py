@@ -490,3 +490,145 @@ def test_gradients_and_training_are_bitwise_reproducible():
     assert n0 == n1 > 4
     assert l0 == l1 and np.isfinite(l0)
     np.testing.assert_array_equal(p0, p1)
+
+
+def _masked_residual(ct, p, rhs, mean, dx, dy):
+    """The oracle's sum of r^2 over interior fluid cells (ghosts refreshed)."""
+    from oracle import pyoracle as O
+    sim = O.Sim(ct, delx=dx, dely=dy, delt=0.0, gamma=0.0, reynolds=1.0,
+                sor_absolute_epsilon=1.0, max_iterations=1, omega=1.8, p=p)
+    sim.rhs[...] = rhs
+    sim.set(rhs_mean=mean)
+    assert sim.call("rebuild_boundary_list") == 0
+    return O.lib().orc_masked_residual_sum(sim.s, None)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("itermax,cycles,sweeps", [(10, 3, 16), (36, 3, 16), (37, 8, 6)])
+def test_learned_solve_stops_at_itermax(itermax, cycles, sweeps):
+    """A learned solve never sweeps past max_iterations: a phase whose budget
+    would pass it is cut to the sweeps left.  With 4 sweeps per launch the
+    cut phases end after an odd number of sweep-doing launches while the
+    phase queued an even number (36 = 16 + 16 + 4; 37 = 6 * 6 + 1), so p's
+    latest buffer must come from the launch that stopped the channel.  Each
+    channel reports exactly itermax sweeps, exhausted, and the residual it
+    reports is that of the p it leaves."""
+    from stroemung_amd import workloads
+    from stroemung_amd.learned import PressureNet
+    from stroemung_amd.simulation import Context
+    grids = workloads.random_cavities(3, n=64, seed=71, eps=1e-13, check_every=4,
+                                      itermax=itermax)
+    for graphs in (False, True):
+        ctx = Context(0)
+        for ct, vel, prm in grids:
+            ctx.add_grid(prm, ct, vel)
+        ctx.construct()
+        net = PressureNet(ctx, 16, 16, 3, seed=72)
+        ctx.set_learned(net, cycles=cycles, sweeps=sweeps, step="momentum", coarsen=4)
+        ctx.set_graphs(graphs)
+        for tick in range(2):
+            its, _ = ctx.tick()
+            for c, (ct, _, prm) in enumerate(grids):
+                st = ctx.state(c)
+                assert its[c] == itermax and st["sor_iterations"] == itermax, (graphs, tick, c)
+                assert st["exhausted"], (graphs, tick, c)
+                fld = ctx.download(c, ("p", "rhs"))
+                r2 = _masked_residual(ct, fld["p"], fld["rhs"], st["rhs_mean"], prm.delx,
+                                      prm.dely)
+                assert st["residual_squared"] == pytest.approx(r2, rel=1e-10), (graphs, tick, c)
+        ctx.set_learned(None)
+        net.close()
+        ctx.close()
+
+
+@pytest.mark.gpu
+def test_capture_phases_stop_at_itermax():
+    """Capture phases run 8, 16, 32, ... sweeps; with itermax 20 the second is
+    cut to 12 and the solve ends exhausted after 20 sweeps -- the same p, u
+    and v, bit for bit, as the plain solve stopped at 20 sweeps."""
+    from stroemung_amd import workloads
+    from stroemung_amd.learned import PressureNet
+    from stroemung_amd.simulation import Context
+    grids = workloads.random_cavities(2, n=64, seed=73, eps=1e-13, check_every=4, itermax=20)
+    runs = []
+    for capture in (False, True):
+        ctx = Context(0)
+        for ct, vel, prm in grids:
+            ctx.add_grid(prm, ct, vel)
+        ctx.construct()
+        net = PressureNet(ctx, 16, 16, 2, seed=74)
+        net.reserve(16)
+        if capture:
+            ctx.set_capture(net, points=4, sweeps=8, coarsen=4)
+        its = [ctx.tick()[0] for _ in range(2)]
+        for c in range(len(grids)):
+            assert ctx.state(c)["exhausted"]
+        runs.append((its, [ctx.download(c, ("u", "v", "p")) for c in range(len(grids))]))
+        ctx.set_capture(None)
+        net.close()
+        ctx.close()
+    assert runs[0][0] == runs[1][0] == [[20, 20], [20, 20]]
+    for a, b in zip(runs[0][1], runs[1][1]):
+        for k in ("u", "v", "p"):
+            np.testing.assert_array_equal(a[k].view(np.uint64), b[k].view(np.uint64))
+
+
+def _free_block(ct, rng, size=2, margin=2):
+    """Origin of a size x size block with only fluid within `margin` cells."""
+    nx, ny = ct.shape
+    for _ in range(1000):
+        x = int(rng.integers(margin + 1, nx - size - margin - 1))
+        y = int(rng.integers(margin + 1, ny - size - margin - 1))
+        if (ct[x - margin:x + size + margin, y - margin:y + size + margin] == 0).all():
+            return x, y
+    raise AssertionError("no free block")
+
+
+@pytest.mark.gpu
+def test_graph_replay_after_grid_edits_and_net_swaps():
+    """Captured learned cycles hold the boundary list, tile lists and net
+    buffers by value; after a grid edit (an obstacle added: a longer list,
+    then removed: a shorter one) and after the net is destroyed and another
+    of the same shape set, the replayed cycles must equal the same sequence
+    launched one by one, bit for bit."""
+    from stroemung_amd import workloads
+    from stroemung_amd.learned import PressureNet
+    from stroemung_amd.simulation import Context
+    n, count, f = 64, 3, 4
+    grids = workloads.random_cavities(count, n=n, seed=81, eps=1e-6)
+    runs = []
+    for graphs in (False, True):
+        rng = np.random.default_rng(82)
+        ctx = Context(0)
+        for ct, vel, prm in grids:
+            ctx.add_grid(prm, ct, vel)
+        ctx.construct()
+        net = PressureNet(ctx, n // f, n // f, count, seed=83)
+        ctx.set_graphs(graphs)
+        ctx.set_learned(net, cycles=6, sweeps=8, step="momentum", coarsen=f)
+        log = []
+        log.append([ctx.tick()[0] for _ in range(2)])
+        ct0, vel0, _ = grids[1]
+        x, y = _free_block(ct0, rng)
+        edited = ct0.copy()
+        edited[x:x + 2, y:y + 2] = 1
+        ctx.set_cells(1, edited, vel0)                  # more boundary cells
+        log.append([ctx.tick()[0] for _ in range(2)])
+        ctx.set_cells(1, ct0, vel0)                     # fewer again
+        log.append([ctx.tick()[0] for _ in range(2)])
+        net.close()                                     # the context stops using it
+        log.append([ctx.tick()[0] for _ in range(1)])
+        net = PressureNet(ctx, n // f, n // f, count, seed=84)
+        ctx.set_learned(net, cycles=6, sweeps=8, step="momentum", coarsen=f)
+        log.append([ctx.tick()[0] for _ in range(2)])
+        for c in range(count):
+            s = ctx.state(c)
+            assert s["residual_squared"] <= s["tolerance_squared"] and not s["exhausted"]
+        runs.append((log, [ctx.download(c, ("u", "v", "p")) for c in range(count)]))
+        ctx.set_learned(None)
+        net.close()
+        ctx.close()
+    assert runs[0][0] == runs[1][0]
+    for a, b in zip(runs[0][1], runs[1][1]):
+        for k in ("u", "v", "p"):
+            np.testing.assert_array_equal(a[k].view(np.uint64), b[k].view(np.uint64))

@@ -38,9 +38,14 @@ sys.path.insert(0, ROOT)
 
 METRIC = "cell-updates/s at fixed Poisson residual: plain GPU SOR vs learned+batched"
 HBM_PEAK_GBS = 8000.0
-# algorithmic bytes per interior cell of one red-black sweep: each half-sweep
-# reads own p, other p, own rhs and writes own p at 4 B/cell, plus 1 B flags
+# bytes per interior cell of one red-black sweep streamed once per sweep
+# (each half-sweep reads own p, other p, own rhs and writes own p at 4 B/cell,
+# plus 1 B flags): the rate a kernel without temporal blocking would have to
+# stream for the same progress -- reported as "effective_sweep_gbs", not as
+# the roofline's achieved rate
 SWEEP_BYTES_PER_CELL = 33.0
+# stage kernels: algorithmic bytes per cell (f64 8 B, flags 1 B, each value once)
+STAGE_BYTES_PER_CELL = {"fg": 33.0, "rhs": 24.0, "adap_uv": 40.0, "ranges": 25.0}
 COPY_PROBE_BYTES = 256 << 20
 # template arguments (R, HY, NS) of each blocked-solver variant, after K
 VARIANT_SHAPES = {0: "4, 32, 16", 1: "4, 32, 16", 2: "4, 64, 16", 3: "4, 32, 16",
@@ -234,16 +239,17 @@ def sweeps_run(a):
     ctx.stage("rb_sweeps", launches * k)
     ctx.synchronize()
     t1 = time.perf_counter()
-    ctx.close()
     # wall time per working launch: the host polls, the launch that takes
     # the stop decision and the queued launches after it are included, so
     # this slightly understates the kernel (the rocprofv3 trace gives it alone)
     avg_s = (t1 - t0) / launches
-    bytes_per_launch = SWEEP_BYTES_PER_CELL * k * n * n
-    achieved = bytes_per_launch / avg_s / 1e9
+    unique, loaded = ctx.solver_bytes()
+    achieved = unique / avg_s / 1e9
+    effective = SWEEP_BYTES_PER_CELL * k * n * n / avg_s / 1e9
     working_set = (3 * 8 + 1) * (n + 2) * (n + 2)
+    ctx.close()
     out = {
-        "metric": "blocked red-black SOR sweep, algorithmic bytes per second",
+        "metric": "blocked red-black SOR launch, algorithmic bytes per second",
         "value": achieved, "unit": "GB/s", "n_gpus": 1, "steps": launches, "warmup": 8,
         "ms_per_step": avg_s * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
@@ -256,8 +262,9 @@ def sweeps_run(a):
         "roofline": {"kernel": "k_rb_strip<%d, %s>" % (k, VARIANT_SHAPES.get(variant, "?")),
                      "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "bytes_per_launch": bytes_per_launch, "avg_launch_us": avg_s * 1e6,
-                     "working_set_bytes": working_set,
+                     "bytes_per_launch": unique, "loaded_bytes_per_launch": loaded,
+                     "halo_factor": loaded / unique, "effective_sweep_gbs": effective,
+                     "avg_launch_us": avg_s * 1e6, "working_set_bytes": working_set,
                      "cache_resident": working_set < MALL_BYTES},
     }
     print(json.dumps(out), flush=True)
@@ -303,7 +310,11 @@ def channels_run(a):
     if not timed:
         ms, timed = sms, stimed
     avg_s = ms / 1e3 / max(timed, 1)
-    achieved = SWEEP_BYTES_PER_CELL * swept / (avg_s * launches) / 1e9 if timed else None
+    # every launch moves at most each cell of every channel once (25 B); launches
+    # after compaction move only the running channels', so this is an upper bound
+    unique, loaded = ctx.solver_bytes()
+    achieved = unique / avg_s / 1e9 if timed else None
+    effective = SWEEP_BYTES_PER_CELL * swept / (avg_s * launches) / 1e9 if timed else None
     flat = [i for t in per_tick for i in t]
     out = {
         "metric": METRIC,
@@ -329,6 +340,11 @@ def channels_run(a):
             "kernel": "k_rb_strip<%d, %s>" % (ksweeps, VARIANT_SHAPES.get(variant, "?")),
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": None,
+            "bytes_per_launch": unique, "loaded_bytes_per_launch": loaded,
+            "halo_factor": loaded / unique,
+            "model": "25 B per cell of every channel per launch (an upper bound: launches "
+                     "after compaction cover only the running channels)",
+            "effective_sweep_gbs": effective,
             "avg_launch_us": avg_s * 1e6, "launches": launches,
         },
     }
@@ -630,9 +646,12 @@ def main():
         ms, timed = sms, stimed
     _, _, launches = ctx.kernel_stats(kname)
     avg_s = ms / 1e3 / max(timed, 1)
-    # one launch advances every interior cell by ksweeps red-black sweeps
-    bytes_per_launch = SWEEP_BYTES_PER_CELL * ksweeps * cells
-    achieved = bytes_per_launch / avg_s / 1e9 if timed else None
+    # one launch advances every cell by ksweeps red-black sweeps on chip: it
+    # must move each cell's p, rhs and flags in and p out once (25 B per cell);
+    # its tiles' overlapping halos load more (loaded / unique = halo factor)
+    unique, loaded = ctx.solver_bytes()
+    achieved = unique / avg_s / 1e9 if timed else None
+    effective = SWEEP_BYTES_PER_CELL * ksweeps * cells / avg_s / 1e9 if timed else None
     working_set = (3 * 8 + 1) * (n + 2) * (n + 2)
     kernels = {}
     for k in ("rb_tb", "pbuf_fix", "solve_begin", "fg", "rhs", "adap_uv", "restore",
@@ -641,6 +660,10 @@ def main():
         if kl:
             kernels[k] = {"launches": kl, "timed": kt,
                           "avg_us": round(kms * 1e3 / kt, 3) if kt else None}
+            if k in STAGE_BYTES_PER_CELL and kt:
+                # single-launch event pairs: they also time the event packets
+                kernels[k]["gbs_event_timed"] = round(
+                    STAGE_BYTES_PER_CELL[k] * (n + 2) ** 2 / (kms * 1e-3 / kt) / 1e9, 1)
     sweeps_per_step = sweeps / a.steps
     out = {
         "metric": METRIC,
@@ -676,7 +699,16 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS if achieved else None,
             "traffic": traffic,
-            "bytes_per_launch": bytes_per_launch,
+            "traffic_source": ("FETCH_SIZE and WRITE_SIZE per working dispatch from two "
+                               "rocprofv3 --pmc runs of `bench.py --profile --variant 9`, "
+                               "scaled by a 256 MiB copy (profiles/traffic.json); not "
+                               "measured in this run" if traffic else None),
+            "bytes_per_launch": unique,
+            "model": "25 B per cell per launch: p read, rhs read, flags read, p written",
+            "loaded_bytes_per_launch": loaded,
+            "halo_factor": loaded / unique,
+            "effective_sweep_gbs": effective,
+            "sweeps_per_launch": ksweeps,
             "avg_launch_us": avg_s * 1e6,
             "timed_launches": timed,
             "timing": "HIP events around queued batches of 8-64 launches",

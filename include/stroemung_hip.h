@@ -192,6 +192,13 @@ int stm_autotune_ticks(stm_ctx *ctx, int32_t *chosen, double *cell_sweeps_per_s)
 
 /* ---- measurement (no reference counterpart) */
 int stm_synchronize(stm_ctx *ctx);
+/* Bytes one blocked launch of the current variant moves over every channel:
+ * *unique = each cell once (p read, rhs read, flags read, p written: 25 B per
+ * cell, ring included), the least any launch can move; *loaded = what its
+ * tiles read and write, their regions overlapping by the halo (p, rhs and
+ * flags for every region cell inside the channel, p for every output cell).
+ * loaded / unique is the halo factor. */
+int stm_solver_bytes(stm_ctx *ctx, double *unique_bytes, double *loaded_bytes);
 /* Streaming device-to-device copy of `bytes` (read + write), `reps` times,
  * on the context's stream: the measured bandwidth ceiling in GB/s. */
 int stm_stream_copy(stm_ctx *ctx, uint64_t bytes, uint32_t reps, double *gbs);

@@ -3,7 +3,7 @@
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
  * it, as the checker.  The product (stroemung_amd/) never links or calls it.
  *
- * Built with -O2 -ffp-contract=off: Rust never fuses a*b+c, so neither may we.
+ * Built with -O3 -ffp-contract=off: Rust never fuses a*b+c, so neither may we.
  * Every expression below keeps the reference's association order; sq(x) is
  * x*x (what f64::powi(2) lowers to).
  *
@@ -756,6 +756,12 @@ double orc_masked_residual_sum(orc_sim *s, double *rhs_sum2)
  * checked every check_every sweeps and capped at max_iterations sweeps.
  * *norm2 = sum r^2 / fluid_cells.
  */
+static int g_rb_threads = 1;
+
+/* Threads for the red-black sweeps of orc_solve_rb (bit-identical to one:
+ * orc_rb_sweeps_mt); the residual sums stay sequential. */
+void orc_set_threads(int threads) { g_rb_threads = threads > 1 ? threads : 1; }
+
 int orc_solve_rb(orc_sim *s, uint32_t *iters, double *norm2)
 {
     int every = s->check_every > 0 ? s->check_every : 1;
@@ -766,7 +772,8 @@ int orc_solve_rb(orc_sim *s, uint32_t *iters, double *norm2)
     int it = 0;
     while (it < s->max_iterations) {
         int n = s->max_iterations - it < every ? s->max_iterations - it : every;
-        orc_rb_sweeps(s, n);
+        if (g_rb_threads > 1) orc_rb_sweeps_mt(s, n, g_rb_threads);
+        else orc_rb_sweeps(s, n);
         it += n;
         res2 = orc_masked_residual_sum(s, NULL);
         if (res2 <= tol2) break;

@@ -128,6 +128,7 @@ void orc_rb_sweeps(orc_sim *s, int nsweeps);
 void orc_rb_sweeps_mt(orc_sim *s, int nsweeps, int threads);
 double orc_masked_residual_sum(orc_sim *s, double *rhs_sum2);
 double orc_rhs_mean(const orc_sim *s);
+void orc_set_threads(int threads);
 int orc_solve_rb(orc_sim *s, uint32_t *iters, double *norm2);
 
 #ifdef __cplusplus

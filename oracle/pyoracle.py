@@ -118,6 +118,8 @@ def _declare(L):
     L.orc_rb_half_sweep.argtypes = [P, C.c_int]
     L.orc_rb_sweeps.argtypes = [P, C.c_int]
     L.orc_rb_sweeps_mt.argtypes = [P, C.c_int, C.c_int]
+    L.orc_set_threads.argtypes = [C.c_int]
+    L.orc_set_threads.restype = None
     L.orc_masked_residual_sum.argtypes = [P, _D]
     L.orc_masked_residual_sum.restype = C.c_double
     L.orc_rhs_mean.argtypes = [P]
@@ -128,6 +130,12 @@ def _declare(L):
 
 def _dp(a):
     return a.ctypes.data_as(_D)
+
+
+def set_threads(n=None):
+    """Threads for the red-black solves (n None: every core); the results
+    are the same bits for any count."""
+    lib().orc_set_threads(int(n or os.cpu_count() or 1))
 
 
 def view3(a):

@@ -1606,6 +1606,23 @@ int stm_stream_copy(stm_ctx* ctx, uint64_t bytes, uint32_t reps, double* gbs) {
     return check_launch(ctx);
 }
 
+int stm_solver_bytes(stm_ctx* ctx, double* unique_bytes, double* loaded_bytes) {
+    int rc = ready(ctx);
+    if (rc) return rc;
+    const TbVariant& v = TB_VARIANTS[ctx->variant];
+    double unique = 0.0, loaded = 0.0;
+    for (const auto& c : ctx->chans) unique += 25.0 * (double)c.nx * (double)c.ny;
+    for (const TbTile& t : ctx->h_tb) {
+        const HostChan& c = ctx->chans[t.chan];
+        const int64_t rx = std::min<int64_t>(t.rx0 + v.ex, c.nx) - std::max(t.rx0, 0);
+        const int64_t ry = std::min<int64_t>(t.ry0 + v.ey, c.ny) - std::max(t.ry0, 0);
+        loaded += 17.0 * (double)(rx * ry) + 8.0 * (double)t.xn * (double)t.yn;
+    }
+    if (unique_bytes) *unique_bytes = unique;
+    if (loaded_bytes) *loaded_bytes = loaded;
+    return STM_OK;
+}
+
 int stm_synchronize(stm_ctx* ctx) {
     if (!ctx) return fail(nullptr, STM_ERR_INVALID_ARG, "null context");
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));

@@ -18,10 +18,11 @@
 //
 // Schedule inside a launch: ghosts take the colour of their cell and read
 // only fluid cells of the other colour, and a cell of colour c reads only
-// cells of the other colour.  So the red ghosts (sources black) can be
+// cells of the other colour.  So the red ghosts (sources black) are
 // refreshed in the red phase and the black ghosts (sources red) in the black
 // phase, each from values that the phase does not change -- the same values
-// the start-of-sweep refresh sees.  Two barriers per sweep.
+// the start-of-sweep refresh sees -- and from the same neighbour reads as the
+// phase's updates.  Two barriers per sweep.
 //
 // Registers and LDS: the loaded region is EX x EY cells, EY = 2*HY.  Thread
 // (strip s, pair m) owns rows [s*R, s*R + R) of the y-pair (2m, 2m+1) -- one
@@ -46,7 +47,37 @@ namespace stm {
 
 namespace {
 
-enum : uint8_t { C_UPDATE = 1, C_OUT = 2, C_EDGE_SHIFT = 2 };   // edge kind in bits 2..5
+// For a ghost cell, the fluid neighbours its pressure copies
+// (src/grid/mod.rs:343-412): N, S, E, W; one bit for a straight edge, a
+// vertical and a horizontal one for a corner, whose ghost is their mean.
+enum : uint32_t { C_GN = 1, C_GS = 2, C_GE = 4, C_GW = 8 };
+
+__device__ __forceinline__ uint32_t ghost_bits(uint8_t edge) {
+    switch (edge) {
+    case E_N: return C_GN;
+    case E_NE: return C_GN | C_GE;
+    case E_E: return C_GE;
+    case E_SE: return C_GS | C_GE;
+    case E_S: return C_GS;
+    case E_SW: return C_GS | C_GW;
+    case E_W: return C_GW;
+    case E_NW: return C_GN | C_GW;
+    default: return 0;
+    }
+}
+
+// What a thread knows of its 2R own cells, cell i = 2r + q, packed so the
+// codes take a few registers: bit i of `upd` = updated by SOR, of `out` = in
+// the output tile; 4 ghost bits per cell.
+template <int R>
+struct Codes {
+    static constexpr int NG = (2 * R + 7) / 8;
+    uint32_t upd, out;
+    uint32_t g[NG];
+    __device__ __forceinline__ uint32_t ghost(int i) const {
+        return (g[i >> 3] >> (4 * (i & 7))) & 15u;
+    }
+};
 
 __device__ __forceinline__ double wave_sum(double v) {
     for (int o = 32; o > 0; o >>= 1) v = v + __shfl_xor(v, o, 64);
@@ -78,148 +109,152 @@ struct Blk {
     static constexpr int PS = EX * HY + 2 * HY;
 
     __device__ static __forceinline__ int col(int r, int q) { return RP0 ^ (r & 1) ^ q; }
-    // own slot of colour c in row r
-    __device__ static __forceinline__ int slot(int r, int c) { return c ^ RP0 ^ (r & 1); }
 
-    // the four neighbours (colour col(r,q) ^ 1) of own cell (r, q)
-    template <int r, int q>
-    __device__ static __forceinline__ void nbrs(const double (*P)[PS],
-                                                const double (&pv)[R][2], int lx0, int m,
-                                                double& pn, double& ps, double& pe, double& pw) {
-        constexpr int o = RP0 ^ (r & 1) ^ q ^ 1;
-        const int lx = lx0 + r;
-        const double x = P[o][OFF + lx * HY + m - 1 + 2 * q];
-        pn = q ? pv[r][0] : x;
-        ps = q ? x : pv[r][1];
-        if constexpr (r + 1 < R) pe = pv[r + 1][q]; else pe = P[o][OFF + (lx + 1) * HY + m];
-        if constexpr (r >= 1) pw = pv[r - 1][q]; else pw = P[o][OFF + (lx - 1) * HY + m];
-    }
-
-    // ghost pressure (src/grid/mod.rs:343-412) of own ghosts of colour g
-    template <int r, int g>
-    __device__ static __forceinline__ void refresh_row(double (*P)[PS], double (&pv)[R][2],
-                                                       const uint8_t (&cc)[R][2], int lx0, int m) {
-        constexpr int q = g ^ RP0 ^ (r & 1);
-        const int e = cc[r][q] >> C_EDGE_SHIFT;
-        if (!e) return;
-        double pn, ps, pe, pw;
-        nbrs<r, q>(P, pv, lx0, m, pn, ps, pe, pw);
-        double v;
-        switch (e) {
-        case E_N: v = pn; break;
-        case E_NE: v = (pn + pe) * 0.5; break;
-        case E_E: v = pe; break;
-        case E_SE: v = (ps + pe) * 0.5; break;
-        case E_S: v = ps; break;
-        case E_SW: v = (ps + pw) * 0.5; break;
-        case E_W: v = pw; break;
-        default: v = (pn + pw) * 0.5; break;   // E_NW
-        }
-        pv[r][q] = v;
-        P[g][OFF + (lx0 + r) * HY + m] = v;
-    }
-
-    template <int g, int r = 0>
-    __device__ static __forceinline__ void refresh(double (*P)[PS], double (&pv)[R][2],
-                                                   const uint8_t (&cc)[R][2], int lx0, int m) {
-        if constexpr (r < R) {
-            refresh_row<r, g>(P, pv, cc, lx0, m);
-            refresh<g, r + 1>(P, pv, cc, lx0, m);
-        }
-    }
-
-    // Red-black update of own fluid cells of colour c.  Every LDS read of the
-    // phase is of the other colour, which the phase does not change, so they
-    // are issued first as one batch (x[r]: N or S of row r; xe / xw: E of the
-    // last row and W of the first, across the strip ends) and the wave waits
-    // once.  Branch-free: every own cell of the colour is computed and the
-    // cells that are not updated (ghosts, solids, the region's border) keep
-    // their value; bit 2r+q of `upd` says which are updated.
+    // The neighbour values of colour c ^ 1 that own cells of colour c read:
+    // N / S from LDS (x[r]; one of N / S is the thread's own other cell),
+    // E / W from the thread's registers except across the strip ends (xe,
+    // xw).  The LDS reads are issued as one batch, so the wave waits once.
     template <int c>
-    __device__ static __forceinline__ void update(double (*P)[PS], double (&pv)[R][2],
-                                                  const double (&rr)[R][2], uint32_t upd,
-                                                  int lx0, int m, double omw, double mid,
-                                                  double rdx2, double rdy2) {
+    __device__ static __forceinline__ void reads(double (*P)[PS], int lx0, int m, double (&x)[R],
+                                                 double& xe, double& xw) {
         constexpr int o = c ^ 1;
-        double x[R];
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const int q = c ^ RP0 ^ (r & 1);
             x[r] = P[o][OFF + (lx0 + r) * HY + m - 1 + 2 * q];
         }
-        const double xe = P[o][OFF + (lx0 + R) * HY + m];
-        const double xw = P[o][OFF + (lx0 - 1) * HY + m];
+        xe = P[o][OFF + (lx0 + R) * HY + m];
+        xw = P[o][OFF + (lx0 - 1) * HY + m];
+    }
+
+    // Red-black update of own fluid cells of colour c (bit 2r+q of cc.upd);
+    // branch-free: every own cell of the colour is computed and the cells
+    // that are not updated (ghosts, solids, the region's border) keep their
+    // value.  Every value read has colour c ^ 1, which the phase does not
+    // change.
+    template <int c>
+    __device__ static __forceinline__ void update(double (*P)[PS], double (&pv)[R][2],
+                                                  const double (&rr)[R][2], const Codes<R>& cc,
+                                                  int lx0, int m, double omw, double mid,
+                                                  double rdx2, double rdy2) {
+        double x[R], xe, xw;
+        reads<c>(P, lx0, m, x, xe, xw);
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const int q = c ^ RP0 ^ (r & 1);
-            // cells (r, q ^ 1), (r +- 1, q) have colour c ^ 1: unchanged here
             const double pn = q ? pv[r][0] : x[r];
             const double ps = q ? x[r] : pv[r][1];
             const double pe = r + 1 < R ? pv[r + 1 < R ? r + 1 : r][q] : xe;
             const double pw = r >= 1 ? pv[r >= 1 ? r - 1 : r][q] : xw;
             const double tt = fma(pe + pw, rdx2, (ps + pn) * rdy2);
             const double nv = fma(omw, pv[r][q], mid * (tt - rr[r][q]));
-            pv[r][q] = ((upd >> (2 * r + q)) & 1u) ? nv : pv[r][q];
+            pv[r][q] = ((cc.upd >> (2 * r + q)) & 1u) ? nv : pv[r][q];
             P[c][OFF + (lx0 + r) * HY + m] = pv[r][q];
         }
     }
 
-    // sum of r^2 over own fluid output cells (ghosts fresh)
-    template <int r = 0, int q = 0>
-    __device__ static __forceinline__ double resid(double (*P)[PS], const double (&pv)[R][2],
-                                                   const double (&rr)[R][2],
-                                                   const uint8_t (&cc)[R][2], int lx0, int m,
-                                                   double rdx2, double rdy2, double acc) {
-        if constexpr (r < R) {
-            if ((cc[r][q] & (C_UPDATE | C_OUT)) == (C_UPDATE | C_OUT)) {
-                double pn, ps, pe, pw;
-                nbrs<r, q>(P, pv, lx0, m, pn, ps, pe, pw);
-                const double pc = pv[r][q];
-                const double a = (pe - pc) - (pc - pw);
-                const double b = (ps - pc) - (pc - pn);
-                const double res = fma(a, rdx2, b * rdy2) - rr[r][q];
-                acc = acc + res * res;
-            }
-            if constexpr (q == 0) return resid<r, 1>(P, pv, rr, cc, lx0, m, rdx2, rdy2, acc);
-            else return resid<r + 1, 0>(P, pv, rr, cc, lx0, m, rdx2, rdy2, acc);
-        } else {
-            return acc;
+    // Ghost cells of colour c take their fluid neighbours' pressure
+    // (src/grid/mod.rs:343-412: one neighbour, or (a + b) * 0.5 of a
+    // vertical and a horizontal one) -- the reference's start-of-sweep copy:
+    // a ghost of colour c reads only fluid cells of colour c ^ 1 and is read
+    // only by them, so refreshing it in colour c's phase gives the values
+    // that copy gives.  Branch-free over the thread's cells, batched reads;
+    // run only by waves that hold a ghost cell (the choice is wave-uniform).
+    template <int c>
+    __device__ static __forceinline__ void ghosts(double (*P)[PS], double (&pv)[R][2],
+                                                  const Codes<R>& cc, int lx0, int m) {
+        double x[R], xe, xw;
+        reads<c>(P, lx0, m, x, xe, xw);
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int q = c ^ RP0 ^ (r & 1);
+            const uint32_t g = cc.ghost(2 * r + q);
+            const double pn = q ? pv[r][0] : x[r];
+            const double ps = q ? x[r] : pv[r][1];
+            const double pe = r + 1 < R ? pv[r + 1 < R ? r + 1 : r][q] : xe;
+            const double pw = r >= 1 ? pv[r >= 1 ? r - 1 : r][q] : xw;
+            const double a = (g & C_GN) ? pn : ps;
+            const double b = (g & C_GE) ? pe : pw;
+            const bool vert = g & (C_GN | C_GS), horz = g & (C_GE | C_GW);
+            const double gv = vert ? (horz ? (a + b) * 0.5 : a) : b;
+            pv[r][q] = g ? gv : pv[r][q];
+            P[c][OFF + (lx0 + r) * HY + m] = pv[r][q];
         }
     }
 
-    __device__ static __forceinline__ double run(double (*P)[PS], double (&pv)[R][2],
-                                                 const double (&rr)[R][2],
-                                                 const uint8_t (&cc)[R][2], int lx0, int m,
-                                                 int nsw, bool check, const ChanDesc& d) {
-        const double omw = d.omw, mid = d.mid, rdx2 = d.rdx2, rdy2 = d.rdy2;
-        uint32_t upd = 0;
-        int ghost = 0;
+    // sum of r^2 over own fluid output cells (ghosts fresh); the LDS reads
+    // (both colours' N / S neighbours and the strip-end E / W ones) first
+    __device__ static __forceinline__ double resid(double (*P)[PS], const double (&pv)[R][2],
+                                                   const double (&rr)[R][2],
+                                                   const Codes<R>& cc, int lx0, int m,
+                                                   double rdx2, double rdy2) {
+        double xn[R], xs[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            xn[r] = P[col(r, 0) ^ 1][OFF + (lx0 + r) * HY + m - 1];   // N of (r, 0)
+            xs[r] = P[col(r, 1) ^ 1][OFF + (lx0 + r) * HY + m + 1];   // S of (r, 1)
+        }
+        double xe[2], xw[2];
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            xe[q] = P[col(R - 1, q) ^ 1][OFF + (lx0 + R) * HY + m];
+            xw[q] = P[col(0, q) ^ 1][OFF + (lx0 - 1) * HY + m];
+        }
+        double acc = 0.0;
 #pragma unroll
         for (int r = 0; r < R; r++)
 #pragma unroll
             for (int q = 0; q < 2; q++) {
-                P[col(r, q)][OFF + (lx0 + r) * HY + m] = pv[r][q];
-                if (cc[r][q] & C_UPDATE) upd |= 1u << (2 * r + q);
-                if (cc[r][q] >> C_EDGE_SHIFT) ghost = 1;
+                const double pn = q ? pv[r][0] : xn[r];
+                const double ps = q ? xs[r] : pv[r][1];
+                const double pe = r + 1 < R ? pv[r + 1 < R ? r + 1 : r][q] : xe[q];
+                const double pw = r >= 1 ? pv[r >= 1 ? r - 1 : r][q] : xw[q];
+                const double pc = pv[r][q];
+                const double a = (pe - pc) - (pc - pw);
+                const double b = (ps - pc) - (pc - pn);
+                const double res = fma(a, rdx2, b * rdy2) - rr[r][q];
+                const bool in = ((cc.upd & cc.out) >> (2 * r + q)) & 1u;
+                acc = acc + (in ? res * res : 0.0);
             }
-        // ghost refreshes only in waves that own a ghost cell (wave-uniform)
+        return acc;
+    }
+
+    __device__ static __forceinline__ double run(double (*P)[PS], double (&pv)[R][2],
+                                                 const double (&rr)[R][2],
+                                                 const Codes<R>& cc, int lx0, int m,
+                                                 int nsw, bool check, const ChanDesc& d) {
+        const double omw = d.omw, mid = d.mid, rdx2 = d.rdx2, rdy2 = d.rdy2;
+        int ghost = 0;
+#pragma unroll
+        for (int r = 0; r < R; r++)
+#pragma unroll
+            for (int q = 0; q < 2; q++) P[col(r, q)][OFF + (lx0 + r) * HY + m] = pv[r][q];
+#pragma unroll
+        for (int k = 0; k < Codes<R>::NG; k++) ghost |= cc.g[k] != 0;
+        // the ghost refresh only in waves that own a ghost cell (wave-uniform)
         ghost = __ballot(ghost) != 0;
         __syncthreads();
-        if (ghost) refresh<1>(P, pv, cc, lx0, m);   // black ghosts, for the first red half
+        if (ghost) ghosts<1>(P, pv, cc, lx0, m);   // black ghosts, for the first red half
         __syncthreads();
+        Codes<R> ck = cc;
         for (int sw = 0; sw < nsw; sw++) {
-            update<0>(P, pv, rr, upd, lx0, m, omw, mid, rdx2, rdy2);
-            if (ghost) refresh<0>(P, pv, cc, lx0, m);   // red ghosts, for the black half
+            // the lane masks the codes give are cheap to form: form them in each
+            // sweep rather than keep them all live in scalar registers
+            asm volatile("" : "+v"(ck.upd));
+#pragma unroll
+            for (int k = 0; k < Codes<R>::NG; k++) asm volatile("" : "+v"(ck.g[k]));
+            update<0>(P, pv, rr, ck, lx0, m, omw, mid, rdx2, rdy2);
+            if (ghost) ghosts<0>(P, pv, ck, lx0, m);   // red ghosts, for the black half
             __syncthreads();
-            update<1>(P, pv, rr, upd, lx0, m, omw, mid, rdx2, rdy2);
-            if (ghost && (check || sw + 1 < nsw)) refresh<1>(P, pv, cc, lx0, m);
+            update<1>(P, pv, rr, ck, lx0, m, omw, mid, rdx2, rdy2);
+            if (ghost && (check || sw + 1 < nsw)) ghosts<1>(P, pv, ck, lx0, m);
             __syncthreads();
         }
         double acc = 0.0;
         if (check) {
-            if (ghost) refresh<0>(P, pv, cc, lx0, m);   // red ghosts from the final black
+            if (ghost) ghosts<0>(P, pv, cc, lx0, m);   // red ghosts from the final black
             __syncthreads();
-            acc = resid(P, pv, rr, cc, lx0, m, rdx2, rdy2, 0.0);
+            acc = resid(P, pv, rr, cc, lx0, m, rdx2, rdy2);
         }
         return acc;
     }
@@ -267,7 +302,10 @@ k_rb_strip(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
     // the solve is for rhs - mean (the all-Neumann problem needs zero mean)
     const double mean = s.rhs_mean;
     double pv[R][2], rr[R][2];
-    uint8_t cc[R][2];
+    Codes<R> cc;
+    cc.upd = cc.out = 0;
+#pragma unroll
+    for (int k = 0; k < Codes<R>::NG; k++) cc.g[k] = 0;
 #pragma unroll
     for (int r = 0; r < R; r++) {
         const int lx = lx0 + r, gx = ex0 + lx;
@@ -278,8 +316,9 @@ k_rb_strip(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
         if (row_in && gy >= 0 && gy + 1 < ny) {
             pp = *reinterpret_cast<const double2*>(pin + g);
             rh = *reinterpret_cast<const double2*>(rhs + g);
-            f0 = flags[g];
-            f1 = flags[g + 1];
+            const uint16_t ff = *reinterpret_cast<const uint16_t*>(flags + g);   // g is even
+            f0 = (uint8_t)(ff & 0xff);
+            f1 = (uint8_t)(ff >> 8);
         } else if (row_in && gy >= 0 && gy < ny) {
             pp.x = pin[g];
             rh.x = rhs[g];
@@ -292,17 +331,15 @@ k_rb_strip(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
 #pragma unroll
         for (int q = 0; q < 2; q++) {
             const uint8_t fl = q ? f1 : f0;
-            const int ly = 2 * m + q, gyq = gy + q;
-            uint8_t code = 0;
+            const int ly = 2 * m + q, gyq = gy + q, i = 2 * r + q;
             if (fl != 0xff) {
                 const bool inner = lx >= 1 && lx <= EX - 2 && ly >= 1 && ly <= EY - 2;
                 if (inner && flag_fluid(fl) && gx >= 1 && gx <= nx - 2 && gyq >= 1 && gyq <= ny - 2)
-                    code |= C_UPDATE;
-                if (inner && !flag_fluid(fl)) code |= (uint8_t)(flag_edge(fl) << C_EDGE_SHIFT);
+                    cc.upd |= 1u << i;
+                if (inner && !flag_fluid(fl)) cc.g[i >> 3] |= ghost_bits(flag_edge(fl)) << (4 * (i & 7));
                 if (gx >= t.x0 && gx < t.x0 + t.xn && gyq >= t.y0 && gyq < t.y0 + t.yn)
-                    code |= C_OUT;
+                    cc.out |= 1u << i;
             }
-            cc[r][q] = code;
         }
     }
 
@@ -344,7 +381,7 @@ k_rb_strip(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
 #pragma unroll
     for (int r = 0; r < R; r++) {
         const int64_t g = base + (int64_t)(ex0 + lx0 + r) * pitch + gy;
-        const bool o0 = cc[r][0] & C_OUT, o1 = cc[r][1] & C_OUT;
+        const bool o0 = (cc.out >> (2 * r)) & 1u, o1 = (cc.out >> (2 * r + 1)) & 1u;
         if (o0 && o1) *reinterpret_cast<double2*>(pout + g) = make_double2(pv[r][0], pv[r][1]);
         else if (o0) pout[g] = pv[r][0];
         else if (o1) pout[g + 1] = pv[r][1];

@@ -156,6 +156,13 @@ class Context:
         self._check(lib().stm_solver_variant(self._ctx, variant, *[C.byref(o) for o in out]))
         return tuple(o.value for o in out)
 
+    def solver_bytes(self):
+        """(unique, loaded) bytes of one blocked launch of the current
+        variant: every cell once (25 B) and what the overlapping tiles move."""
+        u, l = C.c_double(), C.c_double()
+        self._check(lib().stm_solver_bytes(self._ctx, C.byref(u), C.byref(l)))
+        return u.value, l.value
+
     def stream_copy(self, nbytes, reps=20):
         """Measured device copy bandwidth, GB/s (read + write bytes)."""
         g = C.c_double()

@@ -170,7 +170,8 @@ int stm_solve_pressure(stm_ctx *ctx);         /* src/simulation.rs:239-285 */
 int stm_set_u_and_v(stm_ctx *ctx);            /* src/simulation.rs:287-322 */
 int stm_copy_pressure_to_boundaries(stm_ctx *ctx); /* src/grid/mod.rs:343-412 */
 /* Exactly n red-black sweeps (ghost refresh, red, black) on every channel,
- * with no residual test (no reference counterpart). */
+ * with no residual test, then a last ghost refresh so the ghosts hold the
+ * final values as after a solve (no reference counterpart). */
 int stm_rb_sweeps(stm_ctx *ctx, uint32_t n);
 
 /* ---- red-black solver configuration (no reference counterpart)

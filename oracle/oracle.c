@@ -711,9 +711,13 @@ void orc_rb_sweeps_mt(orc_sim *s, int nsweeps, int threads)
                 }
         }
     }
+    if (nsweeps > 0) orc_ghost_refresh(s, -1);
 }
 
-/* One sweep = ghost refresh of every boundary cell, red half, black half. */
+/* One sweep = ghost refresh of every boundary cell, red half, black half;
+ * after the last sweep the ghosts are refreshed once more, so they hold the
+ * final values (as the red-black solve leaves them: its residual test
+ * refreshes them). */
 void orc_rb_sweeps(orc_sim *s, int nsweeps)
 {
     for (int k = 0; k < nsweeps; k++) {
@@ -721,6 +725,7 @@ void orc_rb_sweeps(orc_sim *s, int nsweeps)
         orc_rb_half_sweep(s, 0);
         orc_rb_half_sweep(s, 1);
     }
+    if (nsweeps > 0) orc_ghost_refresh(s, -1);
 }
 
 /*

@@ -366,7 +366,37 @@ int count_tb(stm_ctx* ctx, int variant) {
     return n;
 }
 
-// Output tiles of the blocked solver for `variant`, channel by channel.
+// Morton index of (a, b): bits interleaved, a in the odd positions.
+uint64_t morton2(uint32_t a, uint32_t b) {
+    uint64_t m = 0;
+    for (int i = 0; i < 32; i++)
+        m |= (uint64_t)((a >> i) & 1u) << (2 * i + 1) | (uint64_t)((b >> i) & 1u) << (2 * i);
+    return m;
+}
+
+// Launch order of one channel's tiles (positions [p0, p0 + n) of the list).
+// Workgroups go to the chip's XCDs round-robin (position p to XCD p % 8), and
+// each XCD has its own L2: a tile's halo is served from that L2 when the
+// neighbour that shares it runs on the same XCD.  So each XCD gets one
+// compact block: the tiles in Z order (Morton index of their grid position),
+// cut into consecutive runs of the sizes the XCDs' position classes have,
+// each run laid on one class.  Every tile keeps its canonical `idx` (its
+// residual slot), so the order changes no result.
+void order_for_xcds(TbTile* t, int p0, int n, int nyt) {
+    constexpr int NXCD = 8;
+    std::vector<TbTile> z(t, t + n);
+    std::stable_sort(z.begin(), z.end(), [&](const TbTile& a, const TbTile& b) {
+        const int ia = a.idx - p0, ib = b.idx - p0;
+        return morton2(ia / nyt, ia % nyt) < morton2(ib / nyt, ib % nyt);
+    });
+    int k = 0;
+    for (int c = 0; c < NXCD; c++)
+        for (int p = p0 + c; p < p0 + n; p += NXCD) t[p - p0] = z[k++];
+}
+
+// Output tiles of the blocked solver for `variant`, channel by channel
+// (canonical idx: x-major over the channel's tile grid; launch order from
+// order_for_xcds).
 int plan_tb(stm_ctx* ctx, int variant) {
     invalidate_graphs(ctx);   // blocked launches capture d_tb and its length
     const TbVariant& v = TB_VARIANTS[variant];
@@ -375,11 +405,16 @@ int plan_tb(stm_ctx* ctx, int variant) {
     for (size_t k = 0; k < ctx->chans.size(); k++) {
         HostChan& c = ctx->chans[k];
         c.tb0 = (int32_t)tl.size();
-        for (auto xs : plan_axis((int)c.nx, v.ex, H, 1))
-            for (auto ys : plan_axis((int)c.ny, v.ey, H, 2))
+        const auto ax = plan_axis((int)c.nx, v.ex, H, 1), ay = plan_axis((int)c.ny, v.ey, H, 2);
+        for (auto xs : ax)
+            for (auto ys : ay)
                 tl.push_back(TbTile{(int32_t)k, xs.start, ys.start, xs.len, ys.len,
-                                    (int32_t)tl.size(), xs.origin, ys.origin});
+                                    (int32_t)tl.size(), xs.origin, ys.origin, (int32_t)c.nx,
+                                    (int32_t)c.ny, (int32_t)c.pitch, c.tb0, 0, 0,
+                                    (int64_t)c.off});
         c.ntb = (int32_t)tl.size() - c.tb0;
+        for (size_t i = (size_t)c.tb0; i < tl.size(); i++) tl[i].ntb = c.ntb;
+        order_for_xcds(tl.data() + c.tb0, c.tb0, c.ntb, (int)ay.size());
     }
     if ((int)tl.size() > ctx->tb_capacity) {
         if (ctx->d_tb) HIPCHK(ctx, hipFree(ctx->d_tb));
@@ -564,14 +599,21 @@ int nch(stm_ctx* ctx) { return (int)ctx->chans.size(); }
 // ------------------------------------------------------------- stages
 
 int stage_bc(stm_ctx* ctx) {
-    LAUNCH("bc_velocity", launch_bc_velocity(ctx->stream, nch(ctx), ctx->d_cd, ctx->d_be,
-                                             ctx->d_restore, ctx->u, ctx->v));
+    LAUNCH("bc_velocity", launch_bc_velocity(ctx->stream, ctx->nbtot, ctx->d_be, ctx->d_restore,
+                                             ctx->u, ctx->v));
     return STM_OK;
 }
 
 int stage_fg(stm_ctx* ctx) {
     LAUNCH("fg", launch_fg(ctx->stream, ctx->ntiles, ctx->d_cd, ctx->d_cs, ctx->d_tiles,
                            ctx->flags, ctx->u, ctx->v, ctx->f, ctx->g));
+    return STM_OK;
+}
+
+// COMP_FG and COMP_RHS in one pass
+int stage_fg_rhs(stm_ctx* ctx) {
+    LAUNCH("fg_rhs", launch_fg_rhs(ctx->stream, ctx->ntiles, ctx->d_cd, ctx->d_cs, ctx->d_tiles,
+                                   ctx->flags, ctx->u, ctx->v, ctx->f, ctx->g, ctx->rhs));
     return STM_OK;
 }
 
@@ -1007,11 +1049,16 @@ int stage_solve(stm_ctx* ctx) {
     return blocked_solve(ctx, 1);
 }
 
-int stage_adap(stm_ctx* ctx) {
-    LAUNCH("adap_uv", launch_adap_uv(ctx->stream, ctx->ntiles, ctx->d_cd, ctx->d_cs,
-                                     ctx->d_tiles, ctx->f, ctx->g, ctx->p, ctx->u, ctx->v));
+// ADAP_UV with the restore list (src/simulation.rs:287-322) and the ranges
+// that follow it: the restore list first (it sets every boundary cell), then
+// ADAP_UV of the fluid cells fused with the range reductions.
+int stage_adap(stm_ctx* ctx, int want_pressure) {
     LAUNCH("restore", launch_restore(ctx->stream, ctx->nbtot, ctx->d_be, ctx->d_restore,
                                      ctx->u, ctx->v));
+    LAUNCH("adap_ranges", launch_adap_ranges(ctx->stream, ctx->ntiles, nch(ctx), ctx->d_cd,
+                                             ctx->d_cs, ctx->d_tiles, ctx->flags, ctx->f, ctx->g,
+                                             ctx->p, ctx->u, ctx->v, ctx->d_part,
+                                             want_pressure));
     return STM_OK;
 }
 
@@ -1026,11 +1073,9 @@ int tick(stm_ctx* ctx) {
     int rc;
     LAUNCH("delt", launch_delt(ctx->stream, nch(ctx), ctx->d_cd, ctx->d_cs));
     if ((rc = stage_bc(ctx))) return rc;
-    if ((rc = stage_fg(ctx))) return rc;
-    if ((rc = stage_rhs(ctx))) return rc;
+    if ((rc = stage_fg_rhs(ctx))) return rc;
     if ((rc = stage_solve(ctx))) return rc;
-    if ((rc = stage_adap(ctx))) return rc;
-    if ((rc = stage_ranges(ctx, 2))) return rc;
+    if ((rc = stage_adap(ctx, 2))) return rc;
     LAUNCH("advance_time", launch_advance_time(ctx->stream, nch(ctx), ctx->d_cs));
     return check_launch(ctx);
 }
@@ -1322,8 +1367,7 @@ int stm_construct(stm_ctx* ctx) {
     launch_fill(ctx->stream, ctx->total, ctx->f, 0.0);
     launch_fill(ctx->stream, ctx->total, ctx->g, 0.0);
     launch_fill(ctx->stream, ctx->total, ctx->rhs, 0.0);
-    if ((rc = stage_fg(ctx))) return rc;
-    if ((rc = stage_rhs(ctx))) return rc;
+    if ((rc = stage_fg_rhs(ctx))) return rc;
     LAUNCH("initial_norm", launch_initial_norm(ctx->stream, nch(ctx), ctx->d_cd, ctx->d_cs,
                                                ctx->rhs, ctx->p, ctx->scratch));
     return read_states(ctx);
@@ -1392,7 +1436,7 @@ int stm_solve_pressure(stm_ctx* ctx) {
 
 int stm_set_u_and_v(stm_ctx* ctx) {
     int rc = ready(ctx);
-    if (rc || (rc = stage_adap(ctx)) || (rc = stage_ranges(ctx, 0))) return rc;
+    if (rc || (rc = stage_adap(ctx, 0))) return rc;
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     return check_launch(ctx);
 }

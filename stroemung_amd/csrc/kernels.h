@@ -7,19 +7,26 @@
 namespace stm {
 
 // kernels_step.hip
-void launch_bc_velocity(hipStream_t, int nchan, const ChanDesc*, const BEntry*, double2* restore,
-                        double* u, double* v);
+void launch_bc_velocity(hipStream_t, int nbtot, const BEntry*, double2* restore, double* u,
+                        double* v);
 void launch_fg(hipStream_t, int ntiles, const ChanDesc*, const ChanState*, const Tile*,
                const uint8_t* flags, const double* u, const double* v, double* f, double* g);
 void launch_rhs(hipStream_t, int ntiles, const ChanDesc*, const ChanState*, const Tile*,
                 const double* f, const double* g, double* rhs);
-void launch_adap_uv(hipStream_t, int ntiles, const ChanDesc*, const ChanState*, const Tile*,
-                    const double* f, const double* g, const double* p, double* u, double* v);
+// launch_fg then launch_rhs in one pass (the same bits)
+void launch_fg_rhs(hipStream_t, int ntiles, const ChanDesc*, const ChanState*, const Tile*,
+                   const uint8_t* flags, const double* u, const double* v, double* f, double* g,
+                   double* rhs);
 void launch_restore(hipStream_t, int nbtot, const BEntry*, const double2* restore, double* u,
                     double* v);
 void launch_ranges(hipStream_t, int ntiles, int nchan, const ChanDesc*, ChanState*, const Tile*,
                    const uint8_t* flags, const double* u, const double* v, const double* p,
                    double* part, int want_pressure);
+// ADAP_UV on the fluid cells fused with the ranges; after launch_restore
+void launch_adap_ranges(hipStream_t, int ntiles, int nchan, const ChanDesc*, ChanState*,
+                        const Tile*, const uint8_t* flags, const double* f, const double* g,
+                        const double* p, double* u, double* v, double* part,
+                        int want_pressure);
 void launch_delt(hipStream_t, int nchan, const ChanDesc*, ChanState*);
 void launch_advance_time(hipStream_t, int nchan, ChanState*);
 void launch_fill(hipStream_t, int64_t n, double* a, double val);

@@ -18,11 +18,18 @@
 //
 // Schedule inside a launch: ghosts take the colour of their cell and read
 // only fluid cells of the other colour, and a cell of colour c reads only
-// cells of the other colour.  So the red ghosts (sources black) are
-// refreshed in the red phase and the black ghosts (sources red) in the black
-// phase, each from values that the phase does not change -- the same values
-// the start-of-sweep refresh sees -- and from the same neighbour reads as the
-// phase's updates.  Two barriers per sweep.
+// cells of the other colour.  A straight ghost (one fluid neighbour) is read
+// only by that neighbour, which takes its own value in its place -- the value
+// the start-of-sweep refresh would copy.  A corner ghost (two fluid
+// neighbours) of colour c is refreshed in the phase of colour c, from values
+// that the phase does not change.  Every ghost is refreshed after the last
+// sweep.  Two barriers per sweep.  The update has a few forms (with or
+// without the substitutions, see Blk::update), chosen per wave.
+//
+// Launch order: workgroups go to the XCDs round-robin; the host orders each
+// channel's tiles so that every XCD gets one compact block of them (shared
+// halos then hit in that XCD's L2).  The tile carries its channel's geometry,
+// so the loads issue after a single scalar load.
 //
 // Registers and LDS: the loaded region is EX x EY cells, EY = 2*HY.  Thread
 // (strip s, pair m) owns rows [s*R, s*R + R) of the y-pair (2m, 2m+1) -- one
@@ -40,6 +47,8 @@
 // each other's halos).
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "kernels.h"
 #include "stm_internal.h"
 
@@ -52,32 +61,51 @@ namespace {
 // vertical and a horizontal one for a corner, whose ghost is their mean.
 enum : uint32_t { C_GN = 1, C_GS = 2, C_GE = 4, C_GW = 8 };
 
-__device__ __forceinline__ uint32_t ghost_bits(uint8_t edge) {
-    switch (edge) {
-    case E_N: return C_GN;
-    case E_NE: return C_GN | C_GE;
-    case E_E: return C_GE;
-    case E_SE: return C_GS | C_GE;
-    case E_S: return C_GS;
-    case E_SW: return C_GS | C_GW;
-    case E_W: return C_GW;
-    case E_NW: return C_GN | C_GW;
-    default: return 0;
-    }
+// ghost_bits(e), 4 bits per edge kind E_N .. E_NW (0 for E_NONE and above
+// E_NW), so the codes are formed without branches
+constexpr uint64_t GHOST_TAB =
+    (uint64_t)C_GN << (4 * E_N) | (uint64_t)(C_GN | C_GE) << (4 * E_NE) |
+    (uint64_t)C_GE << (4 * E_E) | (uint64_t)(C_GS | C_GE) << (4 * E_SE) |
+    (uint64_t)C_GS << (4 * E_S) | (uint64_t)(C_GS | C_GW) << (4 * E_SW) |
+    (uint64_t)C_GW << (4 * E_W) | (uint64_t)(C_GN | C_GW) << (4 * E_NW);
+// the corner kinds (two fluid neighbours), one bit per edge kind
+constexpr uint32_t CORNER_SET = 1u << E_NE | 1u << E_SE | 1u << E_SW | 1u << E_NW;
+
+__device__ __forceinline__ uint32_t ghost_bits(uint32_t edge) {
+    return (uint32_t)(GHOST_TAB >> (4 * (edge & 15u))) & 15u;
 }
+
+// The forms a wave runs (wave-uniform bits): it holds a ghost cell, a corner
+// ghost, a fluid cell that is not updated (SEL); its vertical substitutions
+// (2 bits at W_VSHIFT: none, per lane, per cell); per-cell E / W ones (HX).
+enum : int { W_GHOST = 1, W_DIAG = 2, W_SEL = 4, W_VSHIFT = 3, W_HX = 32 };
 
 // What a thread knows of its 2R own cells, cell i = 2r + q, packed so the
 // codes take a few registers: bit i of `upd` = updated by SOR, of `out` = in
-// the output tile; 4 ghost bits per cell.
+// the output tile, of `diag` = a corner ghost (two fluid neighbours); 4
+// ghost bits per cell (C_GN ...); and for an updated cell, bits 2i, 2i + 1
+// of `vsub` (N, S) and `hsub` (E, W): that neighbour is a straight ghost
+// whose one fluid neighbour is this cell.
 template <int R>
 struct Codes {
+    static_assert(2 * R <= 32, "one bit per cell in a word");
     static constexpr int NG = (2 * R + 7) / 8;
-    uint32_t upd, out;
+    // two bits per cell: 4R bits
+    using Sub = std::conditional_t<(4 * R > 32), uint64_t, uint32_t>;
+    uint32_t upd, out, diag;
+    Sub vsub, hsub;
     uint32_t g[NG];
     __device__ __forceinline__ uint32_t ghost(int i) const {
         return (g[i >> 3] >> (4 * (i & 7))) & 15u;
     }
 };
+
+// A straight ghost of kind `kind` (its one fluid neighbour on that side)?
+// Not fluid and that edge kind; the 0xff of a cell outside the channel has
+// the fluid bit, so it is none.
+__device__ __forceinline__ uint32_t straight_ghost(uint32_t fl, uint32_t kind) {
+    return (fl & (F_FLUID | 15u)) == kind ? 1u : 0u;
+}
 
 __device__ __forceinline__ double wave_sum(double v) {
     for (int o = 32; o > 0; o >>= 1) v = v + __shfl_xor(v, o, 64);
@@ -110,6 +138,7 @@ struct Blk {
 
     __device__ static __forceinline__ int col(int r, int q) { return RP0 ^ (r & 1) ^ q; }
 
+
     // The neighbour values of colour c ^ 1 that own cells of colour c read:
     // N / S from LDS (x[r]; one of N / S is the thread's own other cell),
     // E / W from the thread's registers except across the strip ends (xe,
@@ -127,28 +156,72 @@ struct Blk {
         xw = P[o][OFF + (lx0 - 1) * HY + m];
     }
 
-    // Red-black update of own fluid cells of colour c (bit 2r+q of cc.upd);
-    // branch-free: every own cell of the colour is computed and the cells
-    // that are not updated (ghosts, solids, the region's border) keep their
-    // value.  Every value read has colour c ^ 1, which the phase does not
-    // change.
-    template <int c>
+    // Ghost value (src/grid/mod.rs:343-412): one fluid neighbour, or
+    // (a + b) * 0.5 of a vertical and a horizontal one; ghost bits g != 0.
+    __device__ static __forceinline__ double ghost_value(uint32_t g, double pn, double ps,
+                                                         double pe, double pw) {
+        const double a = (g & C_GN) ? pn : ps;
+        const double b = (g & C_GE) ? pe : pw;
+        const bool vert = g & (C_GN | C_GS), horz = g & (C_GE | C_GW);
+        return vert ? (horz ? (a + b) * 0.5 : a) : b;
+    }
+
+    // Red-black update of the own cells of colour c.  Every value read has
+    // colour c ^ 1, which the phase does not change.
+    //
+    // What the cells that are not updated hold during the sweeps: a
+    // straight ghost (src/grid/mod.rs:343-412: one fluid neighbour) copies
+    // its one fluid neighbour at the start of each sweep, and that neighbour
+    // is its only reader, so the reader takes its own value in its place --
+    // the same operand -- and the ghost's stored value is not read; a
+    // corner ghost is refreshed after the update of its colour, before its
+    // readers run; every ghost is refreshed after the last sweep; a cell
+    // with no fluid neighbour (solid) is read by no update, and an output
+    // cell of that kind is taken from the input at write-back; cells on the
+    // region's border or outside the channel are never output, and what
+    // they hold reaches the output no sooner than their staleness would.
+    // So the update may store into all of them (no select) unless the wave
+    // holds a fluid cell that is not updated (a fluid cell on the channel's
+    // frame: SEL, the select on cc.upd).
+    //
+    // The substitutions: VM 1 = the reader's straight ghost is its own other
+    // cell of the pair, the same way for every updated cell of that slot in
+    // the lane (the channel's walls along x): one lane flag per slot (vown
+    // bit 0: q = 1 reads N, bit 1: q = 0 reads S); VM 2 = per-cell bits
+    // (cc.vsub); HX = per-cell E / W bits (cc.hsub).  The form is chosen
+    // per wave.
+    template <int c, bool SEL, int VM, bool HX>
     __device__ static __forceinline__ void update(double (*P)[PS], double (&pv)[R][2],
                                                   const double (&rr)[R][2], const Codes<R>& cc,
-                                                  int lx0, int m, double omw, double mid,
-                                                  double rdx2, double rdy2) {
+                                                  uint32_t vown, int lx0, int m, double omw,
+                                                  double mid, double rdx2, double rdy2) {
         double x[R], xe, xw;
         reads<c>(P, lx0, m, x, xe, xw);
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            const int q = c ^ RP0 ^ (r & 1);
-            const double pn = q ? pv[r][0] : x[r];
-            const double ps = q ? x[r] : pv[r][1];
-            const double pe = r + 1 < R ? pv[r + 1 < R ? r + 1 : r][q] : xe;
-            const double pw = r >= 1 ? pv[r >= 1 ? r - 1 : r][q] : xw;
+            const int q = c ^ RP0 ^ (r & 1), i = 2 * r + q;
+            const double pc = pv[r][q];
+            double pn = q ? pv[r][0] : x[r];
+            double ps = q ? x[r] : pv[r][1];
+            double pe = r + 1 < R ? pv[r + 1 < R ? r + 1 : r][q] : xe;
+            double pw = r >= 1 ? pv[r >= 1 ? r - 1 : r][q] : xw;
+            if constexpr (VM == 1) {
+                if (q) pn = (vown & 1u) ? pc : pn;
+                else ps = (vown & 2u) ? pc : ps;
+            }
+            if constexpr (VM == 2) {
+                const uint32_t b = (uint32_t)(cc.vsub >> (2 * i));
+                pn = (b & 1u) ? pc : pn;
+                ps = (b & 2u) ? pc : ps;
+            }
+            if constexpr (HX) {
+                const uint32_t b = (uint32_t)(cc.hsub >> (2 * i));
+                pe = (b & 1u) ? pc : pe;
+                pw = (b & 2u) ? pc : pw;
+            }
             const double tt = fma(pe + pw, rdx2, (ps + pn) * rdy2);
-            const double nv = fma(omw, pv[r][q], mid * (tt - rr[r][q]));
-            pv[r][q] = ((cc.upd >> (2 * r + q)) & 1u) ? nv : pv[r][q];
+            const double nv = fma(omw, pc, mid * (tt - rr[r][q]));
+            pv[r][q] = SEL ? (((cc.upd >> i) & 1u) ? nv : pc) : nv;
             P[c][OFF + (lx0 + r) * HY + m] = pv[r][q];
         }
     }
@@ -158,9 +231,11 @@ struct Blk {
     // vertical and a horizontal one) -- the reference's start-of-sweep copy:
     // a ghost of colour c reads only fluid cells of colour c ^ 1 and is read
     // only by them, so refreshing it in colour c's phase gives the values
-    // that copy gives.  Branch-free over the thread's cells, batched reads;
-    // run only by waves that hold a ghost cell (the choice is wave-uniform).
-    template <int c>
+    // that copy gives.  Inside the sweeps only the corner ghosts (DIAG: two
+    // readers) need it; after the last sweep every ghost takes the final
+    // values.  Branch-free over the thread's cells, batched reads; run only
+    // by waves that hold such a cell (the choice is wave-uniform).
+    template <int c, bool DIAG>
     __device__ static __forceinline__ void ghosts(double (*P)[PS], double (&pv)[R][2],
                                                   const Codes<R>& cc, int lx0, int m) {
         double x[R], xe, xw;
@@ -173,11 +248,8 @@ struct Blk {
             const double ps = q ? x[r] : pv[r][1];
             const double pe = r + 1 < R ? pv[r + 1 < R ? r + 1 : r][q] : xe;
             const double pw = r >= 1 ? pv[r >= 1 ? r - 1 : r][q] : xw;
-            const double a = (g & C_GN) ? pn : ps;
-            const double b = (g & C_GE) ? pe : pw;
-            const bool vert = g & (C_GN | C_GS), horz = g & (C_GE | C_GW);
-            const double gv = vert ? (horz ? (a + b) * 0.5 : a) : b;
-            pv[r][q] = g ? gv : pv[r][q];
+            const bool on = DIAG ? ((cc.diag >> (2 * r + q)) & 1u) != 0 : g != 0;
+            pv[r][q] = on ? ghost_value(g, pn, ps, pe, pw) : pv[r][q];
             P[c][OFF + (lx0 + r) * HY + m] = pv[r][q];
         }
     }
@@ -219,40 +291,93 @@ struct Blk {
         return acc;
     }
 
-    __device__ static __forceinline__ double run(double (*P)[PS], double (&pv)[R][2],
-                                                 const double (&rr)[R][2],
-                                                 const Codes<R>& cc, int lx0, int m,
-                                                 int nsw, bool check, const ChanDesc& d) {
-        const double omw = d.omw, mid = d.mid, rdx2 = d.rdx2, rdy2 = d.rdy2;
-        int ghost = 0;
+    // The loaded cells and their flags (FL, two per (row, pair)) into LDS;
+    // returns W_GHOST / W_DIAG: the wave holds a ghost / a corner ghost
+    // (wave-uniform).  Ends with a barrier.
+    template <int NF>
+    __device__ static __forceinline__ int init(double (*P)[PS], uint16_t* FL,
+                                               const double (&pv)[R][2],
+                                               const uint32_t (&fo)[NF], const Codes<R>& cc,
+                                               int lx0, int m) {
+        int any = 0;
 #pragma unroll
-        for (int r = 0; r < R; r++)
+        for (int r = 0; r < R; r++) {
 #pragma unroll
             for (int q = 0; q < 2; q++) P[col(r, q)][OFF + (lx0 + r) * HY + m] = pv[r][q];
+            FL[(lx0 + r) * HY + m] = (uint16_t)(fo[r >> 1] >> (16 * (r & 1)));
+        }
 #pragma unroll
-        for (int k = 0; k < Codes<R>::NG; k++) ghost |= cc.g[k] != 0;
-        // the ghost refresh only in waves that own a ghost cell (wave-uniform)
-        ghost = __ballot(ghost) != 0;
+        for (int k = 0; k < Codes<R>::NG; k++) any |= cc.g[k] != 0;
+        int w = __ballot(any) != 0 ? W_GHOST : 0;
+        w |= __ballot(cc.diag != 0) != 0 ? W_DIAG : 0;
         __syncthreads();
-        if (ghost) ghosts<1>(P, pv, cc, lx0, m);   // black ghosts, for the first red half
-        __syncthreads();
-        Codes<R> ck = cc;
+        return w;
+    }
+
+    // Before the sweeps (a barrier follows): the black corner ghosts, for
+    // the first red half.
+    __device__ static __forceinline__ void start(double (*P)[PS], double (&pv)[R][2],
+                                                 const Codes<R>& cc, int lx0, int m, int wm) {
+        if (wm & W_DIAG) ghosts<1, true>(P, pv, cc, lx0, m);
+    }
+
+    // The sweeps, from a barrier after start(); the final refresh of every
+    // ghost, and the residual when checked.
+    // The sweeps in one form of the update (each form has its own loop, so
+    // no form's registers meet another's inside it; the forms without corner
+    // ghosts have no branch in the loop).
+    template <bool SEL, int VM, bool HX, bool DIAG>
+    __device__ static __forceinline__ void sweeps(double (*P)[PS], double (&pv)[R][2],
+                                                  const double (&rr)[R][2], Codes<R>& cc,
+                                                  uint32_t vown, int lx0, int m, int nsw,
+                                                  double omw, double mid, double rdx2,
+                                                  double rdy2) {
         for (int sw = 0; sw < nsw; sw++) {
             // the lane masks the codes give are cheap to form: form them in each
-            // sweep rather than keep them all live in scalar registers
-            asm volatile("" : "+v"(ck.upd));
+            // sweep rather than keep them all live in scalar registers (the
+            // empty asm "changes" the codes, so no mask is hoisted; the codes
+            // themselves stay one copy)
+            asm volatile("" : "+v"(cc.upd), "+v"(cc.diag), "+v"(cc.vsub), "+v"(cc.hsub));
 #pragma unroll
-            for (int k = 0; k < Codes<R>::NG; k++) asm volatile("" : "+v"(ck.g[k]));
-            update<0>(P, pv, rr, ck, lx0, m, omw, mid, rdx2, rdy2);
-            if (ghost) ghosts<0>(P, pv, ck, lx0, m);   // red ghosts, for the black half
+            for (int k = 0; k < Codes<R>::NG; k++) asm volatile("" : "+v"(cc.g[k]));
+            // red cells, and the red corner ghosts for the black half
+            update<0, SEL, VM, HX>(P, pv, rr, cc, vown, lx0, m, omw, mid, rdx2, rdy2);
+            if (DIAG) ghosts<0, true>(P, pv, cc, lx0, m);
             __syncthreads();
-            update<1>(P, pv, rr, ck, lx0, m, omw, mid, rdx2, rdy2);
-            if (ghost && (check || sw + 1 < nsw)) ghosts<1>(P, pv, ck, lx0, m);
+            // black cells, and the black corner ghosts for the next red half
+            update<1, SEL, VM, HX>(P, pv, rr, cc, vown, lx0, m, omw, mid, rdx2, rdy2);
+            if (DIAG && sw + 1 < nsw) ghosts<1, true>(P, pv, cc, lx0, m);
             __syncthreads();
+        }
+    }
+
+    // The sweeps, from a barrier after start(), in the wave's form; the final
+    // refresh of every ghost, and the residual when checked.
+    __device__ static __forceinline__ double run(double (*P)[PS], double (&pv)[R][2],
+                                                 const double (&rr)[R][2],
+                                                 Codes<R>& cc, uint32_t vown, int lx0, int m,
+                                                 int nsw, bool check, int wm,
+                                                 const ChanDesc& d) {
+        const double omw = d.omw, mid = d.mid, rdx2 = d.rdx2, rdy2 = d.rdy2;
+        const bool ghost = wm & W_GHOST;
+        const int vm = (wm >> W_VSHIFT) & 3;
+        const bool hx = wm & W_HX;
+#define STM_SW(S, V, X, D) sweeps<S, V, X, D>(P, pv, rr, cc, vown, lx0, m, nsw, omw, mid, rdx2, rdy2)
+        // a wave with a corner ghost or a fluid cell that is not updated runs
+        // the general form: the select, per-cell substitutions, the corner
+        // refresh
+        if (wm & (W_SEL | W_DIAG)) STM_SW(true, 2, true, true);
+        else if (vm == 0) { if (hx) STM_SW(false, 0, true, false); else STM_SW(false, 0, false, false); }
+        else if (vm == 1) { if (hx) STM_SW(false, 1, true, false); else STM_SW(false, 1, false, false); }
+        else { if (hx) STM_SW(false, 2, true, false); else STM_SW(false, 2, false, false); }
+#undef STM_SW
+        // every ghost from the final values, for the output and the residual
+        if (ghost) {
+            ghosts<0, false>(P, pv, cc, lx0, m);
+            ghosts<1, false>(P, pv, cc, lx0, m);
         }
         double acc = 0.0;
         if (check) {
-            if (ghost) ghosts<0>(P, pv, cc, lx0, m);   // red ghosts from the final black
             __syncthreads();
             acc = resid(P, pv, rr, cc, lx0, m, rdx2, rdy2);
         }
@@ -279,85 +404,199 @@ k_rb_strip(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
     static_assert(EX > 2 * H && EY > 2 * H, "tile smaller than its halo");
 
     __shared__ double P[2][EX * HY + 2 * HY];   // plus guard slots, see Blk::OFF
+    __shared__ uint16_t FL[EX * HY];            // flags of (row, pair), for the neighbours
     __shared__ double wsum[NT / 64];
 
+    // The tile carries its channel's geometry: the region's loads issue after
+    // one scalar load, together with the channel's state.
     const TbTile t = tiles[blockIdx.x];
     const int c = t.chan;
-    const ChanDesc& d = cd[c];
     ChanState& s = cs[c];
-    if (s.stop_launch >= 0) return;              // uniform: the channel is done
+    // read here, tested only after the loads: no load waits for it
+    const int stopped = s.stop_launch;
+    const double mean = s.rhs_mean;
     const int tid = threadIdx.x;
     const bool check = mode & TB_CHECK;
     const double* __restrict__ pin = (L & 1) ? pb : pa;
     double* __restrict__ pout = (L & 1) ? pa : pb;
 
     const int ex0 = t.rx0, ey0 = t.ry0;          // region origin; ey0 is even
-    const int nx = d.nx, ny = d.ny;
-    const int64_t pitch = d.pitch, base = d.off;
+    const int nx = t.nx, ny = t.ny;
+    const int64_t pitch = t.pitch, base = t.off;
     const int strip = tid / HY, m = tid - (tid / HY) * HY;
     const int lx0 = strip * R;
     const int gy = ey0 + 2 * m;                  // even: (gy, gy+1) is one aligned pair
 
-    // ---- load (issued before the stop decision so the latencies overlap);
-    // the solve is for rhs - mean (the all-Neumann problem needs zero mean)
-    const double mean = s.rhs_mean;
+    // The previous launch's residual partials of this channel: the first
+    // share is loaded with the region below, unconditionally (clamped index).
+    const bool sum_parts = L > 0 && check;
+    const double part0 = part_prev[t.tb0 + min(tid, t.ntb - 1)];
+
+    // ---- load; the solve is for rhs - mean (the all-Neumann problem needs
+    // zero mean).  Every address is clamped into the channel (rows of pitch
+    // >= ny + 1 for odd ny, so an aligned pair load at the last even y is in
+    // bounds) and values outside the channel are masked afterwards: the
+    // loads carry no branches and issue back to back.
     double pv[R][2], rr[R][2];
+    // own cells' flags, 0xff outside the channel, packed: row r in bits
+    // 16r .. 16r + 15 of fo (q = 0 in the low byte)
+    uint32_t fo[(2 * R + 3) / 4] = {};
+    const int gyc = min(max(gy, 0), (ny - 1) & ~1);   // even
+    const bool y0in = gy >= 0 && gy < ny, y1in = gy >= 0 && gy + 1 < ny;
+    double2 pp[R], rh[R];
+    uint32_t ff[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int64_t row = base + (int64_t)min(max(ex0 + lx0 + r, 0), nx - 1) * pitch;
+        pp[r] = *reinterpret_cast<const double2*>(pin + row + gyc);
+        rh[r] = *reinterpret_cast<const double2*>(rhs + row + gyc);
+        ff[r] = *reinterpret_cast<const uint16_t*>(flags + row + gyc);
+    }
+    // every load is issued before any of them is used: the wave waits once
+    // for the whole region, row by row, instead of once per batch the
+    // scheduler would otherwise form
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int gx = ex0 + lx0 + r;
+        const bool row_in = gx >= 0 && gx < nx;
+        const bool in0 = row_in && y0in, in1 = row_in && y1in;
+        pv[r][0] = in0 ? pp[r].x : 0.0;
+        pv[r][1] = in1 ? pp[r].y : 0.0;
+        rr[r][0] = (in0 ? rh[r].x : 0.0) - mean;
+        rr[r][1] = (in1 ? rh[r].y : 0.0) - mean;
+        fo[r >> 1] |= ((in0 ? ff[r] & 0xffu : 0xffu) | (in1 ? ff[r] & 0xff00u : 0xff00u)) << (16 * (r & 1));
+        // formed here, not after the stop test: otherwise the raw rhs loads
+        // stay live (and spill) through the code section
+        asm volatile("" : "+v"(rr[r][0]), "+v"(rr[r][1]), "+v"(pv[r][0]), "+v"(pv[r][1]));
+    }
+#pragma unroll
+    for (int k = 0; k < (2 * R + 3) / 4; k++) asm volatile("" : "+v"(fo[k]));
+    // this thread's first share of the partials, taken with the region
+    double pacc = sum_parts && tid < t.ntb ? part0 : 0.0;
+    asm volatile("" : "+v"(pacc));
+    // a channel that has stopped: nothing to do (uniform; the loads above
+    // are in bounds and their results unused)
+    if (stopped >= 0) return;
+    const ChanDesc& d = cd[c];
+    auto fown = [&](int r, int q) -> uint32_t { return (fo[r >> 1] >> (16 * (r & 1) + 8 * q)) & 0xffu; };
+
+    // ---- the codes of the own cells, branch-free (row by row, as the flags
+    // arrive): updated = an interior fluid cell inside the region's border;
+    // a ghost = a non-fluid cell inside the border with an edge kind; the
+    // output tile; 0xff (outside the channel) is none of them
     Codes<R> cc;
-    cc.upd = cc.out = 0;
+    cc.upd = cc.out = cc.diag = cc.vsub = cc.hsub = 0;
+    uint32_t keep = 0;   // fluid cells inside the border that are not updated
 #pragma unroll
     for (int k = 0; k < Codes<R>::NG; k++) cc.g[k] = 0;
 #pragma unroll
     for (int r = 0; r < R; r++) {
         const int lx = lx0 + r, gx = ex0 + lx;
-        const bool row_in = gx >= 0 && gx < nx;
-        const int64_t g = base + gx * pitch + gy;
-        double2 pp = make_double2(0.0, 0.0), rh = make_double2(0.0, 0.0);
-        uint8_t f0 = 0xff, f1 = 0xff;             // 0xff: outside the channel
-        if (row_in && gy >= 0 && gy + 1 < ny) {
-            pp = *reinterpret_cast<const double2*>(pin + g);
-            rh = *reinterpret_cast<const double2*>(rhs + g);
-            const uint16_t ff = *reinterpret_cast<const uint16_t*>(flags + g);   // g is even
-            f0 = (uint8_t)(ff & 0xff);
-            f1 = (uint8_t)(ff >> 8);
-        } else if (row_in && gy >= 0 && gy < ny) {
-            pp.x = pin[g];
-            rh.x = rhs[g];
-            f0 = flags[g];
-        }
-        pv[r][0] = pp.x;
-        pv[r][1] = pp.y;
-        rr[r][0] = rh.x - mean;
-        rr[r][1] = rh.y - mean;
+        const bool xin = (lx >= 1) & (lx <= EX - 2);
+        const bool xupd = (gx >= 1) & (gx <= nx - 2);
+        const bool xout = (gx >= t.x0) & (gx < t.x0 + t.xn);
 #pragma unroll
         for (int q = 0; q < 2; q++) {
-            const uint8_t fl = q ? f1 : f0;
+            const uint32_t fl = fown(r, q), e = fl & 15u;
             const int ly = 2 * m + q, gyq = gy + q, i = 2 * r + q;
-            if (fl != 0xff) {
-                const bool inner = lx >= 1 && lx <= EX - 2 && ly >= 1 && ly <= EY - 2;
-                if (inner && flag_fluid(fl) && gx >= 1 && gx <= nx - 2 && gyq >= 1 && gyq <= ny - 2)
-                    cc.upd |= 1u << i;
-                if (inner && !flag_fluid(fl)) cc.g[i >> 3] |= ghost_bits(flag_edge(fl)) << (4 * (i & 7));
-                if (gx >= t.x0 && gx < t.x0 + t.xn && gyq >= t.y0 && gyq < t.y0 + t.yn)
-                    cc.out |= 1u << i;
-            }
+            const bool valid = fl != 0xffu, fluid = (fl & F_FLUID) != 0;
+            const bool inner = valid & xin & (ly >= 1) & (ly <= EY - 2);
+            const bool yupd = (gyq >= 1) & (gyq <= ny - 2);
+            const bool yout = (gyq >= t.y0) & (gyq < t.y0 + t.yn);
+            const bool gh = inner & !fluid;
+            cc.upd |= (uint32_t)(inner & fluid & xupd & yupd) << i;
+            keep |= (uint32_t)(inner & fluid & !(xupd & yupd)) << i;
+            cc.g[i >> 3] |= (gh ? ghost_bits(e) : 0u) << (4 * (i & 7));
+            cc.diag |= (gh ? (CORNER_SET >> e) & 1u : 0u) << i;
+            cc.out |= (uint32_t)(valid & xout & yout) << i;
         }
     }
+
+    int wm = (ex0 & 1) ? Blk<K, R, HY, NS, 1>::init(P, FL, pv, fo, cc, lx0, m)
+                       : Blk<K, R, HY, NS, 0>::init(P, FL, pv, fo, cc, lx0, m);
+    uint32_t vown = 0;
+
+    // ---- the updated cells' neighbours that are straight ghosts of them,
+    // from the flags every thread has put in LDS (N / S across the pair
+    // boundary, E / W across the strip ends; the rest are the thread's own);
+    // the LDS reads are one batch and the bits are masked, not branched
+    {
+        auto fls = [&](int lx, int mm) -> uint32_t {   // a pair's flags, 0xffff outside the region
+            const bool in = lx >= 0 && lx < EX && mm >= 0 && mm < HY;
+            const int k = min(max(lx, 0), EX - 1) * HY + min(max(mm, 0), HY - 1);
+            const uint32_t f = FL[k];
+            return in ? f : 0xffffu;
+        };
+        uint32_t fN[R], fS[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            fN[r] = fls(lx0 + r, m - 1) >> 8;      // the S cell of the pair above
+            fS[r] = fls(lx0 + r, m + 1) & 0xffu;   // the N cell of the pair below
+        }
+        const uint32_t fE = fls(lx0 + R, m), fW = fls(lx0 - 1, m);
+        using Sub = typename Codes<R>::Sub;
+        // cell masks of the vertical substitutions: from the own other cell
+        // (q = 1 reads N: own_n; q = 0 reads S: own_s) or across the pair
+        // boundary (cross)
+        uint32_t own_n = 0, own_s = 0, cross = 0;
+#pragma unroll
+        for (int r = 0; r < R; r++)
+#pragma unroll
+            for (int q = 0; q < 2; q++) {
+                const int i = 2 * r + q;
+                const uint32_t u = (cc.upd >> i) & 1u;
+                const uint32_t fn = q ? fown(r, 0) : fN[r];
+                const uint32_t fs = q ? fS[r] : fown(r, 1);
+                const uint32_t fe_ = r + 1 < R ? fown(r + 1 < R ? r + 1 : r, q) : (fE >> (8 * q)) & 0xffu;
+                const uint32_t fw_ = r >= 1 ? fown(r >= 1 ? r - 1 : r, q) : (fW >> (8 * q)) & 0xffu;
+                const uint32_t sn = u & straight_ghost(fn, E_S), ss = u & straight_ghost(fs, E_N);
+                const uint32_t se = u & straight_ghost(fe_, E_W), sw = u & straight_ghost(fw_, E_E);
+                cc.vsub |= (Sub)(sn | ss << 1) << (2 * i);
+                cc.hsub |= (Sub)(se | sw << 1) << (2 * i);
+                if (q) { own_n |= sn << i; cross |= ss << i; }
+                else { own_s |= ss << i; cross |= sn << i; }
+            }
+        // per lane when every updated cell of the slot has it (or none does)
+        constexpr uint32_t Q1 = 0xaaaaaaaau & ((2 * R < 32) ? (1u << 2 * R) - 1 : ~0u);
+        const uint32_t upd1 = cc.upd & Q1, upd0 = cc.upd & ~Q1;
+        const bool lane_ok = cross == 0 && (own_n == 0 || own_n == upd1) &&
+                             (own_s == 0 || own_s == upd0);
+        vown = (own_n != 0 ? 1u : 0u) | (own_s != 0 ? 2u : 0u);
+        const int vm = __ballot(cc.vsub != 0) == 0 ? 0 : (__ballot(!lane_ok) != 0 ? 2 : 1);
+        wm |= vm << W_VSHIFT;
+        wm |= __ballot(cc.hsub != 0) != 0 ? W_HX : 0;
+        wm |= __ballot(keep != 0) != 0 ? W_SEL : 0;
+    }
+
+    // ---- every thread adds its share of the partials in a fixed order,
+    // waves add theirs, and after the barrier every thread adds the wave sums
+    // in wave order: the same sum, bit for bit, in every tile of the channel
+    {
+        if (sum_parts)
+            for (int i = t.tb0 + tid + NT; i < t.tb0 + t.ntb; i += NT) pacc = pacc + part_prev[i];
+        const double w = wave_sum(pacc);
+        if ((tid & 63) == 0) wsum[tid >> 6] = w;
+    }
+    // the black corner ghosts for the first red half (before the same barrier)
+    if (ex0 & 1) Blk<K, R, HY, NS, 1>::start(P, pv, cc, lx0, m, wm);
+    else Blk<K, R, HY, NS, 0>::start(P, pv, cc, lx0, m, wm);
+    __syncthreads();
 
     // ---- stop decision on the previous launch (identical in every tile)
     const int itermax = (s.budget > 0 && s.budget < d.itermax) ? s.budget : d.itermax;
     const int64_t sweeps = (int64_t)L * K;
     const int done = sweeps < itermax ? (int)sweeps : itermax;
     if (L > 0) {
-        double acc = 0.0;
-        if (check)
-            for (int i = d.tb0 + tid; i < d.tb0 + d.ntb; i += NT) acc = acc + part_prev[i];
-        const double res2 = block_sum<NT>(acc, wsum);
+        double res2 = 0.0;
+#pragma unroll
+        for (int w = 0; w < NT / 64; w++) res2 = res2 + wsum[w];
         const bool conv = check && res2 <= s.tol2;
         // a residual that is not a number never meets the test: stop there
         // (the tick then reports it) instead of sweeping to itermax
         const bool nan = check && res2 != res2;
         if (conv || nan || done >= itermax) {
-            if (t.idx == d.tb0 && tid == 0) {
+            if (t.idx == t.tb0 && tid == 0) {
                 s.stop_launch = L - 1;
                 s.sor_iters = (uint32_t)done;
                 s.pbuf = L & 1;
@@ -374,10 +613,28 @@ k_rb_strip(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
     }
     const int nsw = itermax - done < K ? itermax - done : K;
 
-    const double acc = (ex0 & 1) ? Blk<K, R, HY, NS, 1>::run(P, pv, rr, cc, lx0, m, nsw, check, d)
-                                 : Blk<K, R, HY, NS, 0>::run(P, pv, rr, cc, lx0, m, nsw, check, d);
+    const double acc =
+        (ex0 & 1) ? Blk<K, R, HY, NS, 1>::run(P, pv, rr, cc, vown, lx0, m, nsw, check, wm, d)
+                  : Blk<K, R, HY, NS, 0>::run(P, pv, rr, cc, vown, lx0, m, nsw, check, wm, d);
 
-    // ---- write the output tile
+    // ---- write the output tile.  Output cells that are neither updated nor
+    // ghosts keep their value in every sweep: they are taken from the input
+    // (the sweeps may have stored into them, see Blk::update).
+    {
+        uint32_t rest = 0;
+#pragma unroll
+        for (int i = 0; i < 2 * R; i++)
+            rest |= (cc.ghost(i) == 0 ? 1u : 0u) << i;
+        rest &= cc.out & ~cc.upd;
+        if (__ballot(rest != 0) != 0) {
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const int64_t g = base + (int64_t)(ex0 + lx0 + r) * pitch + gy;
+                if ((rest >> (2 * r)) & 1u) pv[r][0] = pin[g];
+                if ((rest >> (2 * r + 1)) & 1u) pv[r][1] = pin[g + 1];
+            }
+        }
+    }
 #pragma unroll
     for (int r = 0; r < R; r++) {
         const int64_t g = base + (int64_t)(ex0 + lx0 + r) * pitch + gy;

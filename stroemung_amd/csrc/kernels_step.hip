@@ -32,14 +32,15 @@ __device__ __forceinline__ double reflect(bool moving, double b, double fluid) {
 // cell at (x-1, y+1).  So: phase 1 computes every boundary cell's own values
 // and its restore values from the pre-pass fields (substituting that writer's
 // constant for v[W]); phase 2, after a barrier, stores the fluid-face
-// constants.  One workgroup per channel.
+// constants.  Each phase is a kernel over every boundary cell of every
+// channel (one thread per cell); the kernel boundary is the barrier.
 __global__ void __launch_bounds__(256)
-k_bc_velocity(const ChanDesc* __restrict__ cd, const BEntry* __restrict__ be,
-              double2* __restrict__ restore, double* __restrict__ u, double* __restrict__ v)
+k_bc_cells(int nbtot, const BEntry* __restrict__ be, double2* __restrict__ restore,
+           double* __restrict__ u, double* __restrict__ v)
 {
-    const ChanDesc d = cd[blockIdx.x];
-    for (int k = threadIdx.x; k < d.nb; k += blockDim.x) {
-        const BEntry e = be[d.boff + k];
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nbtot) {
+        const BEntry e = be[k];
         const int64_t i = e.gidx, P = e.pitch;
         const int64_t N = i - 1, S = i + 1, E = i + P, W = i - P;
         const double ub = u[i], vb = v[i];
@@ -86,16 +87,21 @@ k_bc_velocity(const ChanDesc* __restrict__ cd, const BEntry* __restrict__ be,
         }
         u[i] = nu;
         v[i] = nv;
-        restore[d.boff + k] = make_double2(ru, rv);
+        restore[k] = make_double2(ru, rv);
     }
-    __syncthreads();
-    for (int k = threadIdx.x; k < d.nb; k += blockDim.x) {
-        const BEntry e = be[d.boff + k];
-        const uint8_t ed = e.edge;
-        if (ed == E_NONE || e.type == T_OUTFLOW) continue;
-        if (ed == E_N || ed == E_NE || ed == E_NW) v[e.gidx - 1] = e.bv;
-        if (ed == E_SW || ed == E_W || ed == E_NW) u[e.gidx - e.pitch] = e.bu;
-    }
+}
+
+__global__ void __launch_bounds__(256)
+k_bc_faces(int nbtot, const BEntry* __restrict__ be, double* __restrict__ u,
+           double* __restrict__ v)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nbtot) return;
+    const BEntry e = be[k];
+    const uint8_t ed = e.edge;
+    if (ed == E_NONE || e.type == T_OUTFLOW) return;
+    if (ed == E_N || ed == E_NE || ed == E_NW) v[e.gidx - 1] = e.bv;
+    if (ed == E_SW || ed == E_W || ed == E_NW) u[e.gidx - e.pitch] = e.bu;
 }
 
 // ---------------------------------------------------------------- COMP_FG
@@ -159,10 +165,35 @@ __device__ __forceinline__ void fg_cell(
     G = vc + (delt * (((lv / re) - duvdx) - dv2dy));
 }
 
-// src/simulation.rs:122-202.  F, G on interior cells; then f = u, g = v on
-// boundary cells, and on fluid cells whose east (south) neighbour is a
-// boundary, f = u (g = v) -- the West-type (North-type) restore of
-// src/simulation.rs:167-201.  Fluid cells on the outer ring keep their f, g.
+// src/simulation.rs:122-202 at cell (x, y) of a channel: F, G on interior
+// cells; then f = u, g = v on boundary cells, and on fluid cells whose east
+// (south) neighbour is a boundary, f = u (g = v) -- the West-type
+// (North-type) restore of src/simulation.rs:167-201.  wf / wg: f / g is
+// written; fluid cells on the outer ring keep their f, g.
+__device__ __forceinline__ void fg_at(const ChanDesc& d, double delt,
+                                      const uint8_t* __restrict__ flags,
+                                      const double* __restrict__ u, const double* __restrict__ v,
+                                      int x, int y, int64_t i, double& F, double& G, bool& wf,
+                                      bool& wg) {
+    const int64_t P = d.pitch;
+    if (!flag_fluid(flags[i])) {
+        F = u[i];
+        G = v[i];
+        wf = wg = true;
+        return;
+    }
+    F = G = 0.0;
+    wf = wg = false;
+    if (x >= 1 && x <= d.nx - 2 && y >= 1 && y <= d.ny - 2) {
+        fg_cell(u[i], u[i - 1], u[i + 1], u[i + P], u[i - P], u[i - P + 1],
+                v[i], v[i - 1], v[i + 1], v[i + P], v[i - P], v[i + P - 1],
+                d.delx, d.dely, delt, d.gamma, d.reynolds, F, G);
+        wf = wg = true;
+    }
+    if (x + 1 < d.nx && !flag_fluid(flags[i + P])) { F = u[i]; wf = true; }
+    if (y + 1 < d.ny && !flag_fluid(flags[i + 1])) { G = v[i]; wg = true; }
+}
+
 __global__ void __launch_bounds__(TILE_THREADS)
 k_fg(const ChanDesc* __restrict__ cd, const ChanState* __restrict__ cs,
      const Tile* __restrict__ tiles, const uint8_t* __restrict__ flags,
@@ -173,25 +204,58 @@ k_fg(const ChanDesc* __restrict__ cd, const ChanState* __restrict__ cs,
     const ChanDesc& d = cd[t.chan];
     const int x = t.x0 + threadIdx.y, y = t.y0 + threadIdx.x;
     if (x >= d.nx || y >= d.ny) return;
-    const int64_t P = d.pitch, i = d.off + x * P + y;
-    const uint8_t fl = flags[i];
-    if (!flag_fluid(fl)) {
-        f[i] = u[i];
-        g[i] = v[i];
-        return;
-    }
-    double F = 0.0, G = 0.0;
-    bool wf = false, wg = false;
-    if (x >= 1 && x <= d.nx - 2 && y >= 1 && y <= d.ny - 2) {
-        fg_cell(u[i], u[i - 1], u[i + 1], u[i + P], u[i - P], u[i - P + 1],
-                v[i], v[i - 1], v[i + 1], v[i + P], v[i - P], v[i + P - 1],
-                d.delx, d.dely, cs[t.chan].delt, d.gamma, d.reynolds, F, G);
-        wf = wg = true;
-    }
-    if (x + 1 < d.nx && !flag_fluid(flags[i + P])) { F = u[i]; wf = true; }
-    if (y + 1 < d.ny && !flag_fluid(flags[i + 1])) { G = v[i]; wg = true; }
+    const int64_t i = d.off + x * (int64_t)d.pitch + y;
+    double F, G;
+    bool wf, wg;
+    fg_at(d, cs[t.chan].delt, flags, u, v, x, y, i, F, G, wf, wg);
     if (wf) f[i] = F;
     if (wg) g[i] = G;
+}
+
+// COMP_FG and COMP_RHS (src/simulation.rs:204-214) in one pass over a tile
+// of TILE_X rows of TILE_Y cells: a fifth row of threads computes f of the
+// row before the tile, and the first lane of each row g of the cell before
+// it, so the rhs of every tile cell takes its f(x-1, y) and g(x, y-1) from
+// LDS -- the values k_fg would have stored -- and f, g are not read back.
+// The same expressions in the same order: bit-identical to k_fg + k_rhs.
+__global__ void __launch_bounds__(TILE_THREADS + TILE_Y)
+k_fg_rhs(const ChanDesc* __restrict__ cd, const ChanState* __restrict__ cs,
+         const Tile* __restrict__ tiles, const uint8_t* __restrict__ flags,
+         const double* __restrict__ u, const double* __restrict__ v,
+         double* __restrict__ f, double* __restrict__ g, double* __restrict__ rhs)
+{
+    __shared__ double sf[TILE_X + 1][TILE_Y];
+    __shared__ double sg[TILE_X + 1][TILE_Y + 1];
+    const Tile t = tiles[blockIdx.x];
+    const ChanDesc& d = cd[t.chan];
+    const double delt = cs[t.chan].delt;
+    const int r = threadIdx.y, l = threadIdx.x;       // row r - 1 of the tile
+    const int x = t.x0 - 1 + r, y = t.y0 + l;
+    const int64_t P = d.pitch;
+    if (x >= 0 && x < d.nx && y < d.ny) {
+        const int64_t i = d.off + x * P + y;
+        double F, G;
+        bool wf, wg;
+        fg_at(d, delt, flags, u, v, x, y, i, F, G, wf, wg);
+        sf[r][l] = wf ? F : f[i];
+        sg[r][l + 1] = wg ? G : g[i];
+        if (r > 0) {
+            if (wf) f[i] = F;
+            if (wg) g[i] = G;
+        }
+        if (r > 0 && l == 0 && y >= 1) {             // g of the cell before the tile's column
+            double F2, G2;
+            bool wf2, wg2;
+            fg_at(d, delt, flags, u, v, x, y - 1, i - 1, F2, G2, wf2, wg2);
+            sg[r][0] = wg2 ? G2 : g[i - 1];
+        }
+    }
+    __syncthreads();
+    if (r > 0 && x >= 1 && x < d.nx && y >= 1 && y < d.ny) {
+        const int64_t i = d.off + x * P + y;
+        rhs[i] = (((sf[r][l] - sf[r - 1][l]) / d.delx) + ((sg[r][l + 1] - sg[r][l]) / d.dely))
+                 / delt;
+    }
 }
 
 // Deterministic block sum: fixed LDS tree over TILE_THREADS values.
@@ -226,24 +290,6 @@ k_rhs(const ChanDesc* __restrict__ cd, const ChanState* __restrict__ cs,
 
 // ---------------------------------------------------------------- ADAP_UV
 
-// src/simulation.rs:287-311 on x in [0, nx-2], y in [0, ny-2].
-__global__ void __launch_bounds__(TILE_THREADS)
-k_adap_uv(const ChanDesc* __restrict__ cd, const ChanState* __restrict__ cs,
-          const Tile* __restrict__ tiles, const double* __restrict__ f,
-          const double* __restrict__ g, const double* __restrict__ p,
-          double* __restrict__ u, double* __restrict__ v)
-{
-    const Tile t = tiles[blockIdx.x];
-    const ChanDesc& d = cd[t.chan];
-    const int x = t.x0 + threadIdx.y, y = t.y0 + threadIdx.x;
-    if (x > d.nx - 2 || y > d.ny - 2) return;
-    const int64_t P = d.pitch, i = d.off + x * P + y;
-    const double delt = cs[t.chan].delt;
-    const double pc = p[i];
-    u[i] = f[i] - ((delt / d.delx) * (p[i + P] - pc));
-    v[i] = g[i] - ((delt / d.dely) * (p[i + 1] - pc));
-}
-
 // src/simulation.rs:313-320: the net effect of the restore list, per cell.
 __global__ void __launch_bounds__(256)
 k_restore(int nbtot, const BEntry* __restrict__ be, const double2* __restrict__ restore,
@@ -260,13 +306,26 @@ k_restore(int nbtot, const BEntry* __restrict__ be, const double2* __restrict__ 
 // ---------------------------------------------------------------- reductions
 
 // Per-tile partials over fluid cells: max|u|, max|v| over fluid faces, the
-// min / max of u^2+v^2 (src/grid/mod.rs:253-268) and of p (237-251).
+// min / max of u^2+v^2 (src/grid/mod.rs:253-268) and of p (237-251).  min and
+// max ignore a NaN operand (fmin / fmax, as Real::min / max and the oracle's
+// comparisons do) and are order-free, so the fold order is free too: a
+// shuffle tree inside each wave, then the four waves' results.
+__device__ __forceinline__ double wave_fmax(double v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ double wave_fmin(double v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
 __global__ void __launch_bounds__(TILE_THREADS)
 k_ranges(const ChanDesc* __restrict__ cd, const Tile* __restrict__ tiles,
          const uint8_t* __restrict__ flags, const double* __restrict__ u,
          const double* __restrict__ v, const double* __restrict__ p, double* __restrict__ part)
 {
-    __shared__ double sh[6][TILE_THREADS];
+    constexpr int NW = TILE_THREADS / 64;
+    __shared__ double sh[6][NW];
     const Tile t = tiles[blockIdx.x];
     const ChanDesc& d = cd[t.chan];
     const int x = t.x0 + threadIdx.y, y = t.y0 + threadIdx.x;
@@ -276,10 +335,10 @@ k_ranges(const ChanDesc* __restrict__ cd, const Tile* __restrict__ tiles,
         const int64_t P = d.pitch, i = d.off + x * P + y;
         if (flag_fluid(flags[i])) {
             const double uc = u[i], vc = v[i];
-            um = fabs(uc);
-            vm = fabs(vc);
-            if (x > 0 && !flag_fluid(flags[i - P])) { const double a = fabs(u[i - P]); um = a > um ? a : um; }
-            if (y > 0 && !flag_fluid(flags[i - 1])) { const double a = fabs(v[i - 1]); vm = a > vm ? a : vm; }
+            um = fmax(0.0, fabs(uc));
+            vm = fmax(0.0, fabs(vc));
+            if (x > 0 && !flag_fluid(flags[i - P])) um = fmax(um, fabs(u[i - P]));
+            if (y > 0 && !flag_fluid(flags[i - 1])) vm = fmax(vm, fabs(v[i - 1]));
             const double sp = sq(uc) + sq(vc);
             smin = fmin(__DBL_MAX__, sp);
             smax = fmax(0.0, sp);
@@ -287,23 +346,92 @@ k_ranges(const ChanDesc* __restrict__ cd, const Tile* __restrict__ tiles,
             pmax = fmax(0.0, p[i]);
         }
     }
-    sh[0][tid] = um; sh[1][tid] = vm; sh[2][tid] = smin;
-    sh[3][tid] = smax; sh[4][tid] = pmin; sh[5][tid] = pmax;
-    __syncthreads();
-    for (int s = TILE_THREADS / 2; s > 0; s >>= 1) {
-        if (tid < s) {
-            double a, b;
-            a = sh[0][tid]; b = sh[0][tid + s]; sh[0][tid] = b > a ? b : a;
-            a = sh[1][tid]; b = sh[1][tid + s]; sh[1][tid] = b > a ? b : a;
-            sh[2][tid] = fmin(sh[2][tid], sh[2][tid + s]);
-            sh[3][tid] = fmax(sh[3][tid], sh[3][tid + s]);
-            sh[4][tid] = fmin(sh[4][tid], sh[4][tid + s]);
-            sh[5][tid] = fmax(sh[5][tid], sh[5][tid + s]);
-        }
-        __syncthreads();
+    um = wave_fmax(um);
+    vm = wave_fmax(vm);
+    smin = wave_fmin(smin);
+    smax = wave_fmax(smax);
+    pmin = wave_fmin(pmin);
+    pmax = wave_fmax(pmax);
+    if ((tid & 63) == 0) {
+        const int w = tid >> 6;
+        sh[0][w] = um; sh[1][w] = vm; sh[2][w] = smin;
+        sh[3][w] = smax; sh[4][w] = pmin; sh[5][w] = pmax;
     }
-    if (tid == 0)
-        for (int k = 0; k < 6; k++) part[blockIdx.x * NPART + k] = sh[k][0];
+    __syncthreads();
+    if (tid < 6) {
+        const bool lo = tid == 2 || tid == 4;
+        double r = sh[tid][0];
+#pragma unroll
+        for (int w = 1; w < NW; w++) r = lo ? fmin(r, sh[tid][w]) : fmax(r, sh[tid][w]);
+        part[blockIdx.x * NPART + tid] = r;
+    }
+}
+
+// ADAP_UV (src/simulation.rs:287-311) fused with the range partials of
+// k_ranges.  Runs after the restore list has been applied: every boundary
+// cell is in that list, so the list's values are the boundary cells' final
+// u, v whatever ADAP_UV computes for them, and this kernel writes only the
+// fluid cells of [0, nx-2] x [0, ny-2] (same expression, same bits); the
+// boundary faces the max |u|, |v| reads are then final too.
+__global__ void __launch_bounds__(TILE_THREADS)
+k_adap_ranges(const ChanDesc* __restrict__ cd, const ChanState* __restrict__ cs,
+              const Tile* __restrict__ tiles, const uint8_t* __restrict__ flags,
+              const double* __restrict__ f, const double* __restrict__ g,
+              const double* __restrict__ p, double* __restrict__ u, double* __restrict__ v,
+              double* __restrict__ part)
+{
+    constexpr int NW = TILE_THREADS / 64;
+    __shared__ double sh[6][NW];
+    const Tile t = tiles[blockIdx.x];
+    const ChanDesc& d = cd[t.chan];
+    const int x = t.x0 + threadIdx.y, y = t.y0 + threadIdx.x;
+    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+    double um = 0.0, vm = 0.0, smin = __DBL_MAX__, smax = 0.0, pmin = __DBL_MAX__, pmax = 0.0;
+    if (x < d.nx && y < d.ny) {
+        const int64_t P = d.pitch, i = d.off + x * P + y;
+        if (flag_fluid(flags[i])) {
+            const double pc = p[i];
+            double uc, vc;
+            if (x <= d.nx - 2 && y <= d.ny - 2) {
+                const double delt = cs[t.chan].delt;
+                uc = f[i] - ((delt / d.delx) * (p[i + P] - pc));
+                vc = g[i] - ((delt / d.dely) * (p[i + 1] - pc));
+                u[i] = uc;
+                v[i] = vc;
+            } else {
+                uc = u[i];
+                vc = v[i];
+            }
+            um = fmax(0.0, fabs(uc));
+            vm = fmax(0.0, fabs(vc));
+            if (x > 0 && !flag_fluid(flags[i - P])) um = fmax(um, fabs(u[i - P]));
+            if (y > 0 && !flag_fluid(flags[i - 1])) vm = fmax(vm, fabs(v[i - 1]));
+            const double sp = sq(uc) + sq(vc);
+            smin = fmin(__DBL_MAX__, sp);
+            smax = fmax(0.0, sp);
+            pmin = fmin(__DBL_MAX__, pc);
+            pmax = fmax(0.0, pc);
+        }
+    }
+    um = wave_fmax(um);
+    vm = wave_fmax(vm);
+    smin = wave_fmin(smin);
+    smax = wave_fmax(smax);
+    pmin = wave_fmin(pmin);
+    pmax = wave_fmax(pmax);
+    if ((tid & 63) == 0) {
+        const int w = tid >> 6;
+        sh[0][w] = um; sh[1][w] = vm; sh[2][w] = smin;
+        sh[3][w] = smax; sh[4][w] = pmin; sh[5][w] = pmax;
+    }
+    __syncthreads();
+    if (tid < 6) {
+        const bool lo = tid == 2 || tid == 4;
+        double r = sh[tid][0];
+#pragma unroll
+        for (int w = 1; w < NW; w++) r = lo ? fmin(r, sh[tid][w]) : fmax(r, sh[tid][w]);
+        part[blockIdx.x * NPART + tid] = r;
+    }
 }
 
 // One workgroup per channel: fold the range partials of its tiles (min and
@@ -320,8 +448,8 @@ k_ranges_reduce(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
     double um = 0.0, vm = 0.0, smin = __DBL_MAX__, smax = 0.0, pmin = __DBL_MAX__, pmax = 0.0;
     for (int k = d.tile0 + tid; k < d.tile0 + d.ntiles; k += 256) {
         const double* q = part + (int64_t)k * NPART;
-        um = q[0] > um ? q[0] : um;
-        vm = q[1] > vm ? q[1] : vm;
+        um = fmax(um, q[0]);
+        vm = fmax(vm, q[1]);
         smin = fmin(smin, q[2]);
         smax = fmax(smax, q[3]);
         pmin = fmin(pmin, q[4]);
@@ -332,9 +460,8 @@ k_ranges_reduce(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
     __syncthreads();
     for (int st = 128; st > 0; st >>= 1) {
         if (tid < st) {
-            double a, b;
-            a = sh[0][tid]; b = sh[0][tid + st]; sh[0][tid] = b > a ? b : a;
-            a = sh[1][tid]; b = sh[1][tid + st]; sh[1][tid] = b > a ? b : a;
+            sh[0][tid] = fmax(sh[0][tid], sh[0][tid + st]);
+            sh[1][tid] = fmax(sh[1][tid], sh[1][tid + st]);
             sh[2][tid] = fmin(sh[2][tid], sh[2][tid + st]);
             sh[3][tid] = fmax(sh[3][tid], sh[3][tid + st]);
             sh[4][tid] = fmin(sh[4][tid], sh[4][tid + st]);
@@ -419,23 +546,25 @@ k_fill(int64_t n, double* __restrict__ a, double val)
 
 #define TB dim3(TILE_Y, TILE_X)
 
-void launch_bc_velocity(hipStream_t st, int nchan, const ChanDesc* cd, const BEntry* be,
-                        double2* restore, double* u, double* v) {
-    k_bc_velocity<<<nchan, 256, 0, st>>>(cd, be, restore, u, v);
+void launch_bc_velocity(hipStream_t st, int nbtot, const BEntry* be, double2* restore, double* u,
+                        double* v) {
+    if (nbtot <= 0) return;
+    k_bc_cells<<<(nbtot + 255) / 256, 256, 0, st>>>(nbtot, be, restore, u, v);
+    k_bc_faces<<<(nbtot + 255) / 256, 256, 0, st>>>(nbtot, be, u, v);
 }
 void launch_fg(hipStream_t st, int ntiles, const ChanDesc* cd, const ChanState* cs,
                const Tile* tiles, const uint8_t* flags, const double* u, const double* v,
                double* f, double* g) {
     k_fg<<<ntiles, TB, 0, st>>>(cd, cs, tiles, flags, u, v, f, g);
 }
+void launch_fg_rhs(hipStream_t st, int ntiles, const ChanDesc* cd, const ChanState* cs,
+                   const Tile* tiles, const uint8_t* flags, const double* u, const double* v,
+                   double* f, double* g, double* rhs) {
+    k_fg_rhs<<<ntiles, dim3(TILE_Y, TILE_X + 1), 0, st>>>(cd, cs, tiles, flags, u, v, f, g, rhs);
+}
 void launch_rhs(hipStream_t st, int ntiles, const ChanDesc* cd, const ChanState* cs,
                 const Tile* tiles, const double* f, const double* g, double* rhs) {
     k_rhs<<<ntiles, TB, 0, st>>>(cd, cs, tiles, f, g, rhs);
-}
-void launch_adap_uv(hipStream_t st, int ntiles, const ChanDesc* cd, const ChanState* cs,
-                    const Tile* tiles, const double* f, const double* g, const double* p,
-                    double* u, double* v) {
-    k_adap_uv<<<ntiles, TB, 0, st>>>(cd, cs, tiles, f, g, p, u, v);
 }
 void launch_restore(hipStream_t st, int nbtot, const BEntry* be, const double2* restore,
                     double* u, double* v) {
@@ -445,6 +574,13 @@ void launch_ranges(hipStream_t st, int ntiles, int nchan, const ChanDesc* cd, Ch
                    const Tile* tiles, const uint8_t* flags, const double* u, const double* v,
                    const double* p, double* part, int want_pressure) {
     k_ranges<<<ntiles, TB, 0, st>>>(cd, tiles, flags, u, v, p, part);
+    k_ranges_reduce<<<nchan, 256, 0, st>>>(cd, cs, part, want_pressure);
+}
+void launch_adap_ranges(hipStream_t st, int ntiles, int nchan, const ChanDesc* cd, ChanState* cs,
+                        const Tile* tiles, const uint8_t* flags, const double* f,
+                        const double* g, const double* p, double* u, double* v, double* part,
+                        int want_pressure) {
+    k_adap_ranges<<<ntiles, TB, 0, st>>>(cd, cs, tiles, flags, f, g, p, u, v, part);
     k_ranges_reduce<<<nchan, 256, 0, st>>>(cd, cs, part, want_pressure);
 }
 void launch_delt(hipStream_t st, int nchan, const ChanDesc* cd, ChanState* cs) {

@@ -103,7 +103,12 @@ struct TbTile {
     int32_t chan, x0, y0, xn, yn;   // output cells [x0, x0 + xn) x [y0, y0 + yn)
     int32_t idx;                 // position in the full tile list (residual partial slot)
     int32_t rx0, ry0;            // origin of the loaded region (ry0 even)
+    // the channel's geometry and tile slice (copies of its ChanDesc fields),
+    // so a launch addresses its region from the tile alone
+    int32_t nx, ny, pitch, tb0, ntb, pad;
+    int64_t off;
 };
+static_assert(sizeof(TbTile) == 64, "one 64-byte scalar load per tile");
 
 constexpr int TILE_X = 4;        // x-rows per tile
 constexpr int TILE_Y = 64;       // y-cells per tile (one wavefront)

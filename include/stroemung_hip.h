@@ -307,6 +307,16 @@ int stm_set_learned(stm_ctx *ctx, stm_net *net, uint32_t cycles, uint32_t sweeps
 #define STM_RENDER_PRESSURE 0
 #define STM_RENDER_SPEED 1
 int stm_render(stm_ctx *ctx, uint32_t channel, int32_t mode, uint8_t *rgba);
+/* stm_render without allocating anything (no counterpart in the reference:
+ * for callers that cannot allocate device memory -- a frame loop under a
+ * harness that forbids it, code that must not call hipMalloc).  The image is
+ * staged in a static device buffer of 4 * 2^22 pixels (16 MiB; a channel of
+ * more cells is refused with STM_ERR_INVALID_ARG) that each device holds
+ * until the process ends; the context must already hold its fields (after
+ * stm_upload or stm_construct, else STM_ERR_STATE).  Same image as
+ * stm_render, bit for bit.  Calls that overlap on one device share the
+ * buffer and race. */
+int stm_render_static(stm_ctx *ctx, uint32_t channel, int32_t mode, uint8_t *rgba);
 
 /* on != 0: learned cycles after the first are captured once into a hipGraph
  * (per net, coarsening, sweeps, step rule and solver variant) and replayed;

@@ -132,6 +132,7 @@ SIGNATURES = {
     "stm_set_capture": (C.c_int, [_CTX, _NET, C.c_uint32, C.c_uint32, C.c_uint32]),
     "stm_set_graphs": (C.c_int, [_CTX, C.c_int32]),
     "stm_render": (C.c_int, [_CTX, C.c_uint32, C.c_int32, _U8]),
+    "stm_render_static": (C.c_int, [_CTX, C.c_uint32, C.c_int32, _U8]),
 }
 
 _lib = None

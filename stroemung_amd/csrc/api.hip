@@ -1145,6 +1145,12 @@ int stm_create(int device, stm_ctx** out) {
         delete ctx;
         return fail(nullptr, STM_ERR_HIP, "stream creation: %s", hipGetErrorString(e));
     }
+    // the static image buffer's address on this device (once per device)
+    if ((e = render_static_init(device)) != hipSuccess) {
+        hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return fail(nullptr, STM_ERR_HIP, "static image buffer: %s", hipGetErrorString(e));
+    }
     *out = ctx;
     return STM_OK;
 }
@@ -1737,6 +1743,37 @@ int stm_render(stm_ctx* ctx, uint32_t ch, int32_t mode, uint8_t* rgba) {
     LAUNCH("render", launch_render(ctx->stream, ctx->d_cd, ctx->d_cs, (int)ch, npix, ctx->flags,
                                    ctx->u, ctx->v, ctx->p, mode, ctx->d_rgba));
     HIPCHK(ctx, hipMemcpyAsync(rgba, ctx->d_rgba, sizeof(uint32_t) * npix, hipMemcpyDeviceToHost,
+                               ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return check_launch(ctx);
+}
+
+// stm_render without the library's image buffer: the kernel writes into the
+// device's static buffer (looked up in stm_create).  No allocation of any
+// kind: the context must already hold its fields, and no profiling record
+// is kept.
+int stm_render_static(stm_ctx* ctx, uint32_t ch, int32_t mode, uint8_t* rgba) {
+    if (!ctx) return fail(nullptr, STM_ERR_INVALID_ARG, "null context");
+    if (!ctx->finalized)
+        return fail(ctx, STM_ERR_STATE, "upload or construct before rendering without allocation");
+    int rc = check_chan(ctx, ch);
+    if (rc) return rc;
+    if (!rgba) return fail(ctx, STM_ERR_INVALID_ARG, "null image");
+    if (mode != STM_RENDER_PRESSURE && mode != STM_RENDER_SPEED)
+        return fail(ctx, STM_ERR_INVALID_ARG, "unknown render mode %d", mode);
+    const HostChan& c = ctx->chans[ch];
+    const int64_t npix = (int64_t)c.nx * c.ny;
+    if (npix > RENDER_STATIC_PIXELS)
+        return fail(ctx, STM_ERR_INVALID_ARG,
+                    "channel %u has %lld cells, more than the static image buffer's %lld",
+                    ch, (long long)npix, (long long)RENDER_STATIC_PIXELS);
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    uint32_t* buf = render_static_buffer(ctx->device);
+    if (!buf) return fail(ctx, STM_ERR_STATE, "static image buffer not set up on device %d",
+                          ctx->device);
+    launch_render(ctx->stream, ctx->d_cd, ctx->d_cs, (int)ch, npix, ctx->flags, ctx->u, ctx->v,
+                  ctx->p, mode, buf);
+    HIPCHK(ctx, hipMemcpyAsync(rgba, buf, sizeof(uint32_t) * npix, hipMemcpyDeviceToHost,
                                ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     return check_launch(ctx);

@@ -69,6 +69,14 @@ void launch_pbuf_fix(hipStream_t, int ntiles, int nchan, const ChanDesc*, ChanSt
 void launch_render(hipStream_t, const ChanDesc*, const ChanState*, int ch, int64_t npix,
                    const uint8_t* flags, const double* u, const double* v, const double* p,
                    int mode, uint32_t* rgba);
+// The static image buffer of stm_render_static: RENDER_STATIC_PIXELS pixels of
+// 4 B (16 MiB) per device, devices 0 .. RENDER_STATIC_DEVICES - 1.  init looks
+// its device address up (the device must be current; once per device);
+// buffer returns it, or null before init.
+constexpr int64_t RENDER_STATIC_PIXELS = (int64_t)4 << 20;
+constexpr int RENDER_STATIC_DEVICES = 64;
+hipError_t render_static_init(int device);
+uint32_t* render_static_buffer(int device);
 
 // kernels_learn.hip: the FP64 side of the learned correction and of capture
 void launch_learn_residual(hipStream_t, int ntiles, const ChanDesc*, const ChanState*,

@@ -7,6 +7,8 @@
 // products are not fused and the image is bit-identical to the reference's.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "kernels.h"
 #include "stm_internal.h"
 
@@ -70,6 +72,37 @@ k_render(const ChanDesc* __restrict__ cd, const ChanState* __restrict__ cs, int 
         hsl_to_rgb((float)t, 1.0f, 0.5f, r, g, b);
     }
     rgba[k] = to_u8(r) | (to_u8(g) << 8) | (to_u8(b) << 16) | (255u << 24);
+}
+
+// ---------------------------------------------------------------- static image buffer
+//
+// stm_render_static renders into this array instead of a buffer the library
+// allocates.  Every device the library runs on holds its own copy until the
+// process ends; host code reaches it only through the device address looked
+// up once per device (the array's name on the host is a stand-in, not a
+// device address).  Calls that overlap on one device share it and race.
+static __device__ __align__(256) uint32_t scratch[RENDER_STATIC_PIXELS];
+
+namespace {
+std::mutex g_scratch_mu;
+uint32_t* g_scratch_ptr[RENDER_STATIC_DEVICES] = {};
+}  // namespace
+
+hipError_t render_static_init(int device) {
+    if (device < 0 || device >= RENDER_STATIC_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return hipSuccess;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));   // device is current
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = static_cast<uint32_t*>(ptr);
+    return hipSuccess;
+}
+
+uint32_t* render_static_buffer(int device) {
+    if (device < 0 || device >= RENDER_STATIC_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return g_scratch_ptr[device];
 }
 
 void launch_render(hipStream_t st, const ChanDesc* cd, const ChanState* cs, int ch, int64_t npix,

@@ -211,12 +211,14 @@ class Context:
         self._check(lib().stm_set_learned(self._ctx, net._net if net else None, cycles, sweeps,
                                           rule, coarsen))
 
-    def render(self, ch, mode="pressure"):
-        """The reference's GUI image of channel ch: (ny, nx, 4) uint8 RGBA."""
+    def render(self, ch, mode="pressure", static=False):
+        """The reference's GUI image of channel ch: (ny, nx, 4) uint8 RGBA.
+        static=True stages it in the device's static buffer (no allocation)."""
         nx, ny = self.shapes[ch]
         out = np.empty((ny, nx, 4), np.uint8)
-        self._check(lib().stm_render(self._ctx, ch, {"pressure": 0, "speed": 1}[mode],
-                                     out.ctypes.data_as(C.POINTER(C.c_uint8))))
+        fn = lib().stm_render_static if static else lib().stm_render
+        self._check(fn(self._ctx, ch, {"pressure": 0, "speed": 1}[mode],
+                       out.ctypes.data_as(C.POINTER(C.c_uint8))))
         return out
 
     def set_graphs(self, on=True):

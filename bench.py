@@ -197,11 +197,23 @@ def make_context(a):
     return ctx, ch
 
 
+PROFILE_TICK_SWEEPS = 96
+
+
 def profile_run(a):
-    """One tick, the tuned variant, 64 blocked launches of fixed sweeps and
-    one copy of COPY_PROBE_BYTES (the byte-counter reference)."""
+    from stroemung_amd.simulation import make_params
+    """One full tick; then 8 ticks whose solves stop at PROFILE_TICK_SWEEPS
+    sweeps (every stage kernel 8 times, on the benchmark's grid and fields);
+    the tuned variant's 64 blocked launches of fixed sweeps; one copy of
+    COPY_PROBE_BYTES (the byte-counter reference)."""
     ctx, ch = make_context(a)
     ctx.tick()
+    n = a.n
+    ctx.set_params(ch, make_params(1.0 / n, 1.0 / n, 0.0, 0.9, 1000.0, a.eps,
+                                   PROFILE_TICK_SWEEPS, 1.8, tau=0.5, solver="rb",
+                                   check_every=a.check_every))
+    for _ in range(8):
+        ctx.tick()
     if a.variant >= 0:
         ctx.solver_variant(a.variant)
     else:
@@ -210,8 +222,9 @@ def profile_run(a):
     ctx.stage("rb_sweeps", 64 * k)
     gbs = ctx.stream_copy(COPY_PROBE_BYTES, 1)
     ctx.synchronize()
-    print("profile run: variant %d (K=%d, tile %dx%d), %d sweeps; copy probe %d bytes, %.0f GB/s"
-          % (variant, k, tx, ty, 64 * k, COPY_PROBE_BYTES, gbs), flush=True)
+    print("profile run: 1 + 8 ticks (%d sweeps each after the first); variant %d (K=%d, "
+          "tile %dx%d), %d sweeps; copy probe %d bytes, %.0f GB/s"
+          % (PROFILE_TICK_SWEEPS, variant, k, tx, ty, 64 * k, COPY_PROBE_BYTES, gbs), flush=True)
 
 
 def sweeps_run(a):

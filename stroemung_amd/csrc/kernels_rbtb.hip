@@ -68,8 +68,6 @@ constexpr uint64_t GHOST_TAB =
     (uint64_t)C_GE << (4 * E_E) | (uint64_t)(C_GS | C_GE) << (4 * E_SE) |
     (uint64_t)C_GS << (4 * E_S) | (uint64_t)(C_GS | C_GW) << (4 * E_SW) |
     (uint64_t)C_GW << (4 * E_W) | (uint64_t)(C_GN | C_GW) << (4 * E_NW);
-// the corner kinds (two fluid neighbours), one bit per edge kind
-constexpr uint32_t CORNER_SET = 1u << E_NE | 1u << E_SE | 1u << E_SW | 1u << E_NW;
 
 __device__ __forceinline__ uint32_t ghost_bits(uint32_t edge) {
     return (uint32_t)(GHOST_TAB >> (4 * (edge & 15u))) & 15u;
@@ -82,45 +80,84 @@ enum : int { W_GHOST = 1, W_DIAG = 2, W_SEL = 4, W_VSHIFT = 3, W_HX = 32 };
 
 // What a thread knows of its 2R own cells, cell i = 2r + q, packed so the
 // codes take a few registers: bit i of `upd` = updated by SOR, of `out` = in
-// the output tile, of `diag` = a corner ghost (two fluid neighbours); 4
-// ghost bits per cell (C_GN ...); and for an updated cell, bits 2i, 2i + 1
-// of `vsub` (N, S) and `hsub` (E, W): that neighbour is a straight ghost
-// whose one fluid neighbour is this cell.
+// the output tile, of `diag` = a corner ghost (two fluid neighbours); the
+// edge kind of each ghost cell (4 bits per cell, 0 for any other cell); and
+// for an updated cell, bit i of `vsub` / `hsub` (N / E) and bit SH + i (S /
+// W): that neighbour is a straight ghost whose one fluid neighbour is this
+// cell.
 template <int R>
 struct Codes {
     static_assert(2 * R <= 32, "one bit per cell in a word");
     static constexpr int NG = (2 * R + 7) / 8;
-    // two bits per cell: 4R bits
-    using Sub = std::conditional_t<(4 * R > 32), uint64_t, uint32_t>;
+    static constexpr int SH = 2 * R <= 16 ? 16 : 32;
+    using Sub = std::conditional_t<(SH + 2 * R > 32), uint64_t, uint32_t>;
     uint32_t upd, out, diag;
     Sub vsub, hsub;
     uint32_t g[NG];
-    __device__ __forceinline__ uint32_t ghost(int i) const {
+    __device__ __forceinline__ uint32_t edge(int i) const {
         return (g[i >> 3] >> (4 * (i & 7))) & 15u;
     }
 };
 
-// A straight ghost of kind `kind` (its one fluid neighbour on that side)?
-// Not fluid and that edge kind; the 0xff of a cell outside the channel has
-// the fluid bit, so it is none.
-__device__ __forceinline__ uint32_t straight_ghost(uint32_t fl, uint32_t kind) {
-    return (fl & (F_FLUID | 15u)) == kind ? 1u : 0u;
+// SWAR helpers on words of four cells' flag bytes (cell 4k + j in byte j of
+// word k).  zero7: bit 7 of each byte set where the byte is 0 (bytes below
+// 0x80); ones: bit 7 of each byte to the full byte; nib7: bits 7, 15, 23, 31
+// to bits 0 .. 3.
+__device__ __forceinline__ uint32_t zero7(uint32_t t) {
+    const uint32_t nz = (((t & 0x7f7f7f7fu) + 0x7f7f7f7fu) | t) & 0x80808080u;
+    return nz ^ 0x80808080u;
+}
+__device__ __forceinline__ uint32_t ones(uint32_t b7) { return (b7 << 1) - (b7 >> 7); }
+__device__ __forceinline__ uint32_t nib7(uint32_t m) {
+    const uint32_t x = (m >> 7) & 0x01010101u;
+    return (x | (x >> 7) | (x >> 14) | (x >> 21)) & 15u;
 }
 
+// Straight ghosts of kind `kind` (one fluid neighbour, on that side) among
+// four flag bytes: bit 7 of each matching byte.  Not fluid and that edge
+// kind; the 0xff of a cell outside the channel has the fluid bit, so it is
+// none.
+__device__ __forceinline__ uint32_t straight7(uint32_t w, uint32_t kind) {
+    return zero7((w & 0x1f1f1f1fu) ^ (kind * 0x01010101u));
+}
+
+// One DPP move of a double (both halves); lanes of rows outside ROWS keep
+// `old`'s value.
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ double dpp_move(double v, double old) {
+    const int lo = __builtin_amdgcn_update_dpp(__double2loint(old), __double2loint(v), CTRL,
+                                               ROWS, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(__double2hiint(old), __double2hiint(v), CTRL,
+                                               ROWS, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
+// The sum of v over the wave, in VALU (DPP) steps rather than LDS round
+// trips, in a fixed order: pairs, quads, the quads of a row, then rows
+// (r0 + r1) and (r2 + r3), then (r2 + r3) + (r0 + r1).  Uniform result.
 __device__ __forceinline__ double wave_sum(double v) {
-    for (int o = 32; o > 0; o >>= 1) v = v + __shfl_xor(v, o, 64);
-    return v;
+    v = v + dpp_move<0xb1>(v, 0.0);          // quad_perm [1, 0, 3, 2]
+    v = v + dpp_move<0x4e>(v, 0.0);          // quad_perm [2, 3, 0, 1]
+    v = v + dpp_move<0x124>(v, 0.0);         // row_ror 4
+    v = v + dpp_move<0x128>(v, 0.0);         // row_ror 8
+    v = v + dpp_move<0x142, 0xa>(v, 0.0);    // row_bcast 15 into rows 1, 3
+    v = v + dpp_move<0x143, 0xc>(v, 0.0);    // row_bcast 31 into rows 2, 3
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+    return __hiloint2double(hi, lo);
 }
 
+// The workgroup's sum of v, in wave order, for thread 0 (the last use of
+// wsum in the kernel: no barrier after it).
 template <int NT>
-__device__ __forceinline__ double block_sum(double v, double* wsum) {
+__device__ __forceinline__ double block_sum_to_first(double v, double* wsum) {
     v = wave_sum(v);
     if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
     __syncthreads();
     double s = 0.0;
+    if (threadIdx.x == 0)
 #pragma unroll
-    for (int w = 0; w < NT / 64; w++) s = s + wsum[w];
-    __syncthreads();
+        for (int w = 0; w < NT / 64; w++) s = s + wsum[w];
     return s;
 }
 
@@ -210,14 +247,12 @@ struct Blk {
                 else ps = (vown & 2u) ? pc : ps;
             }
             if constexpr (VM == 2) {
-                const uint32_t b = (uint32_t)(cc.vsub >> (2 * i));
-                pn = (b & 1u) ? pc : pn;
-                ps = (b & 2u) ? pc : ps;
+                pn = ((cc.vsub >> i) & 1u) ? pc : pn;
+                ps = ((cc.vsub >> (Codes<R>::SH + i)) & 1u) ? pc : ps;
             }
             if constexpr (HX) {
-                const uint32_t b = (uint32_t)(cc.hsub >> (2 * i));
-                pe = (b & 1u) ? pc : pe;
-                pw = (b & 2u) ? pc : pw;
+                pe = ((cc.hsub >> i) & 1u) ? pc : pe;
+                pw = ((cc.hsub >> (Codes<R>::SH + i)) & 1u) ? pc : pw;
             }
             const double tt = fma(pe + pw, rdx2, (ps + pn) * rdy2);
             const double nv = fma(omw, pc, mid * (tt - rr[r][q]));
@@ -243,7 +278,7 @@ struct Blk {
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const int q = c ^ RP0 ^ (r & 1);
-            const uint32_t g = cc.ghost(2 * r + q);
+            const uint32_t g = ghost_bits(cc.edge(2 * r + q));
             const double pn = q ? pv[r][0] : x[r];
             const double ps = q ? x[r] : pv[r][1];
             const double pe = r + 1 < R ? pv[r + 1 < R ? r + 1 : r][q] : xe;
@@ -479,38 +514,62 @@ k_rb_strip(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
     // are in bounds and their results unused)
     if (stopped >= 0) return;
     const ChanDesc& d = cd[c];
-    auto fown = [&](int r, int q) -> uint32_t { return (fo[r >> 1] >> (16 * (r & 1) + 8 * q)) & 0xffu; };
 
-    // ---- the codes of the own cells, branch-free (row by row, as the flags
-    // arrive): updated = an interior fluid cell inside the region's border;
-    // a ghost = a non-fluid cell inside the border with an edge kind; the
-    // output tile; 0xff (outside the channel) is none of them
+    // ---- every thread adds its share of the partials in a fixed order,
+    // waves add theirs, and after the barrier below every thread adds the wave sums
+    // in wave order: the same sum, bit for bit, in every tile of the channel
+    {
+        if (sum_parts)
+            for (int i = t.tb0 + tid + NT; i < t.tb0 + t.ntb; i += NT) pacc = pacc + part_prev[i];
+        const double w = wave_sum(pacc);
+        if ((tid & 63) == 0) wsum[tid >> 6] = w;
+    }
+    // ---- the codes of the own cells, four cells at a time (the flag bytes
+    // of word k of fo are cells 4k .. 4k + 3; byte masks, then cell bits):
+    // updated = an interior fluid cell inside the region's border; a ghost =
+    // a non-fluid cell inside the border (its edge kind kept); the output
+    // tile; 0xff (outside the channel) is none of them
+    constexpr int NW = (2 * R + 3) / 4;
+    static_assert(2 * NW == R, "two rows per word");
+    auto rows = [](bool a, bool b) -> uint32_t {   // byte mask of rows 2k, 2k + 1
+        return (a ? 0x0000ffffu : 0u) | (b ? 0xffff0000u : 0u);
+    };
+    auto slots = [](bool a, bool b) -> uint32_t {  // byte mask of slots q = 0, 1
+        return (a ? 0x00ff00ffu : 0u) | (b ? 0xff00ff00u : 0u);
+    };
+    const uint32_t yin = slots(2 * m >= 1 && 2 * m <= EY - 2, 2 * m + 1 <= EY - 2);
+    const uint32_t yupd = slots(gy >= 1 && gy <= ny - 2, gy + 1 >= 1 && gy + 1 <= ny - 2);
+    const uint32_t yout = slots(gy >= t.y0 && gy < t.y0 + t.yn,
+                                gy + 1 >= t.y0 && gy + 1 < t.y0 + t.yn);
+    auto xin = [&](int r) { const int lx = lx0 + r; return lx >= 1 && lx <= EX - 2; };
+    auto xupd = [&](int r) { const int gx = ex0 + lx0 + r; return gx >= 1 && gx <= nx - 2; };
+    auto xout = [&](int r) { const int gx = ex0 + lx0 + r; return gx >= t.x0 && gx < t.x0 + t.xn; };
     Codes<R> cc;
-    cc.upd = cc.out = cc.diag = cc.vsub = cc.hsub = 0;
+    cc.upd = cc.out = cc.diag = 0;
+    cc.vsub = cc.hsub = 0;
     uint32_t keep = 0;   // fluid cells inside the border that are not updated
+    uint32_t upd7[NW];   // bit 7 of each updated cell's byte
 #pragma unroll
     for (int k = 0; k < Codes<R>::NG; k++) cc.g[k] = 0;
 #pragma unroll
-    for (int r = 0; r < R; r++) {
-        const int lx = lx0 + r, gx = ex0 + lx;
-        const bool xin = (lx >= 1) & (lx <= EX - 2);
-        const bool xupd = (gx >= 1) & (gx <= nx - 2);
-        const bool xout = (gx >= t.x0) & (gx < t.x0 + t.xn);
-#pragma unroll
-        for (int q = 0; q < 2; q++) {
-            const uint32_t fl = fown(r, q), e = fl & 15u;
-            const int ly = 2 * m + q, gyq = gy + q, i = 2 * r + q;
-            const bool valid = fl != 0xffu, fluid = (fl & F_FLUID) != 0;
-            const bool inner = valid & xin & (ly >= 1) & (ly <= EY - 2);
-            const bool yupd = (gyq >= 1) & (gyq <= ny - 2);
-            const bool yout = (gyq >= t.y0) & (gyq < t.y0 + t.yn);
-            const bool gh = inner & !fluid;
-            cc.upd |= (uint32_t)(inner & fluid & xupd & yupd) << i;
-            keep |= (uint32_t)(inner & fluid & !(xupd & yupd)) << i;
-            cc.g[i >> 3] |= (gh ? ghost_bits(e) : 0u) << (4 * (i & 7));
-            cc.diag |= (gh ? (CORNER_SET >> e) & 1u : 0u) << i;
-            cc.out |= (uint32_t)(valid & xout & yout) << i;
-        }
+    for (int k = 0; k < NW; k++) {
+        const uint32_t w = fo[k];
+        const uint32_t valid = ones(zero7(~w) ^ 0x80808080u);
+        const uint32_t fluid = ones((w << 3) & 0x80808080u);
+        const uint32_t rin = rows(xin(2 * k), xin(2 * k + 1)) & yin & valid;
+        const uint32_t ru = rows(xupd(2 * k), xupd(2 * k + 1)) & yupd;
+        const uint32_t u = rin & fluid & ru;
+        const uint32_t gh = rin & ~fluid;
+        const uint32_t o = valid & rows(xout(2 * k), xout(2 * k + 1)) & yout;
+        const uint32_t e4 = w & 0x0f0f0f0fu & gh;                       // ghosts' edge kinds
+        const uint32_t corner = (zero7(e4) ^ 0x80808080u) & ~(e4 << 7);   // kind 2, 4, 6 or 8
+        upd7[k] = u & 0x80808080u;
+        cc.upd |= nib7(u) << (4 * k);
+        cc.out |= nib7(o) << (4 * k);
+        cc.diag |= nib7(corner) << (4 * k);
+        keep |= nib7(rin & fluid & ~ru) << (4 * k);
+        const uint32_t y = e4 | (e4 >> 4);
+        cc.g[k >> 1] |= ((y & 0xffu) | ((y >> 8) & 0xff00u)) << (16 * (k & 1));
     }
 
     int wm = (ex0 & 1) ? Blk<K, R, HY, NS, 1>::init(P, FL, pv, fo, cc, lx0, m)
@@ -519,8 +578,10 @@ k_rb_strip(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
 
     // ---- the updated cells' neighbours that are straight ghosts of them,
     // from the flags every thread has put in LDS (N / S across the pair
-    // boundary, E / W across the strip ends; the rest are the thread's own);
-    // the LDS reads are one batch and the bits are masked, not branched
+    // boundary, E / W across the strip ends; the rest are the thread's own),
+    // four cells at a time: each direction's neighbour flags laid out as the
+    // cells' own (byte j of word k), matched against the edge kind that
+    // makes this cell their fluid neighbour
     {
         auto fls = [&](int lx, int mm) -> uint32_t {   // a pair's flags, 0xffff outside the region
             const bool in = lx >= 0 && lx < EX && mm >= 0 && mm < HY;
@@ -535,49 +596,38 @@ k_rb_strip(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
             fS[r] = fls(lx0 + r, m + 1) & 0xffu;   // the N cell of the pair below
         }
         const uint32_t fE = fls(lx0 + R, m), fW = fls(lx0 - 1, m);
+        uint32_t nsub = 0, ssub = 0, esub = 0, wsub = 0;
+#pragma unroll
+        for (int k = 0; k < NW; k++) {
+            const uint32_t w = fo[k];
+            const uint32_t wn = ((w << 8) & 0xff00ff00u) | fN[2 * k] | (fN[2 * k + 1] << 16);
+            const uint32_t ws = ((w >> 8) & 0x00ff00ffu) | (fS[2 * k] << 8) | (fS[2 * k + 1] << 24);
+            const uint32_t we = (w >> 16) | ((k + 1 < NW ? fo[k + 1 < NW ? k + 1 : k] & 0xffffu : fE) << 16);
+            const uint32_t ww = (w << 16) | (k > 0 ? fo[k > 0 ? k - 1 : k] >> 16 : fW);
+            nsub |= nib7(straight7(wn, E_S) & upd7[k]) << (4 * k);
+            ssub |= nib7(straight7(ws, E_N) & upd7[k]) << (4 * k);
+            esub |= nib7(straight7(we, E_W) & upd7[k]) << (4 * k);
+            wsub |= nib7(straight7(ww, E_E) & upd7[k]) << (4 * k);
+        }
         using Sub = typename Codes<R>::Sub;
-        // cell masks of the vertical substitutions: from the own other cell
-        // (q = 1 reads N: own_n; q = 0 reads S: own_s) or across the pair
-        // boundary (cross)
-        uint32_t own_n = 0, own_s = 0, cross = 0;
-#pragma unroll
-        for (int r = 0; r < R; r++)
-#pragma unroll
-            for (int q = 0; q < 2; q++) {
-                const int i = 2 * r + q;
-                const uint32_t u = (cc.upd >> i) & 1u;
-                const uint32_t fn = q ? fown(r, 0) : fN[r];
-                const uint32_t fs = q ? fS[r] : fown(r, 1);
-                const uint32_t fe_ = r + 1 < R ? fown(r + 1 < R ? r + 1 : r, q) : (fE >> (8 * q)) & 0xffu;
-                const uint32_t fw_ = r >= 1 ? fown(r >= 1 ? r - 1 : r, q) : (fW >> (8 * q)) & 0xffu;
-                const uint32_t sn = u & straight_ghost(fn, E_S), ss = u & straight_ghost(fs, E_N);
-                const uint32_t se = u & straight_ghost(fe_, E_W), sw = u & straight_ghost(fw_, E_E);
-                cc.vsub |= (Sub)(sn | ss << 1) << (2 * i);
-                cc.hsub |= (Sub)(se | sw << 1) << (2 * i);
-                if (q) { own_n |= sn << i; cross |= ss << i; }
-                else { own_s |= ss << i; cross |= sn << i; }
-            }
+        cc.vsub = (Sub)nsub | (Sub)ssub << Codes<R>::SH;
+        cc.hsub = (Sub)esub | (Sub)wsub << Codes<R>::SH;
+        // the vertical substitutions from the own other cell (q = 1 reads N:
+        // own_n; q = 0 reads S: own_s) or across the pair boundary (cross);
         // per lane when every updated cell of the slot has it (or none does)
         constexpr uint32_t Q1 = 0xaaaaaaaau & ((2 * R < 32) ? (1u << 2 * R) - 1 : ~0u);
+        const uint32_t own_n = nsub & Q1, own_s = ssub & ~Q1;
+        const uint32_t cross = (nsub & ~Q1) | (ssub & Q1);
         const uint32_t upd1 = cc.upd & Q1, upd0 = cc.upd & ~Q1;
         const bool lane_ok = cross == 0 && (own_n == 0 || own_n == upd1) &&
                              (own_s == 0 || own_s == upd0);
         vown = (own_n != 0 ? 1u : 0u) | (own_s != 0 ? 2u : 0u);
-        const int vm = __ballot(cc.vsub != 0) == 0 ? 0 : (__ballot(!lane_ok) != 0 ? 2 : 1);
+        const int vm = __ballot((nsub | ssub) != 0) == 0 ? 0 : (__ballot(!lane_ok) != 0 ? 2 : 1);
         wm |= vm << W_VSHIFT;
-        wm |= __ballot(cc.hsub != 0) != 0 ? W_HX : 0;
+        wm |= __ballot((esub | wsub) != 0) != 0 ? W_HX : 0;
         wm |= __ballot(keep != 0) != 0 ? W_SEL : 0;
     }
 
-    // ---- every thread adds its share of the partials in a fixed order,
-    // waves add theirs, and after the barrier every thread adds the wave sums
-    // in wave order: the same sum, bit for bit, in every tile of the channel
-    {
-        if (sum_parts)
-            for (int i = t.tb0 + tid + NT; i < t.tb0 + t.ntb; i += NT) pacc = pacc + part_prev[i];
-        const double w = wave_sum(pacc);
-        if ((tid & 63) == 0) wsum[tid >> 6] = w;
-    }
     // the black corner ghosts for the first red half (before the same barrier)
     if (ex0 & 1) Blk<K, R, HY, NS, 1>::start(P, pv, cc, lx0, m, wm);
     else Blk<K, R, HY, NS, 0>::start(P, pv, cc, lx0, m, wm);
@@ -624,7 +674,7 @@ k_rb_strip(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
         uint32_t rest = 0;
 #pragma unroll
         for (int i = 0; i < 2 * R; i++)
-            rest |= (cc.ghost(i) == 0 ? 1u : 0u) << i;
+            rest |= (cc.edge(i) == 0 ? 1u : 0u) << i;
         rest &= cc.out & ~cc.upd;
         if (__ballot(rest != 0) != 0) {
 #pragma unroll
@@ -643,7 +693,7 @@ k_rb_strip(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
         else if (o0) pout[g] = pv[r][0];
         else if (o1) pout[g + 1] = pv[r][1];
     }
-    const double b = block_sum<NT>(acc, wsum);
+    const double b = block_sum_to_first<NT>(acc, wsum);
     if (tid == 0) part_cur[t.idx] = b;
 }
 

@@ -26,6 +26,10 @@ cavities of 256 x 256 (the network trained first, on the GPU, from pairs the
 solver produces on other layouts), both to the same residual; and
 "learned_same_grid": that network's learned solve on the benchmark's own grid,
 the same ticks timed as for `value`, each solve to the same residual bound.
+Two more summaries ride along: "channels", the 256 uneven channels advanced
+together (two ticks), and "learned_channels", plain against learned on the 64
+channels of 512 x 256 at eps 1e-7 (a shorter training than
+--workload learned_channels).  --no-channels / --no-learned leave them out.
 """
 import argparse
 import json
@@ -80,6 +84,8 @@ def parse():
                          "independent channels of uneven size, seed 0; learned_*: plain vs "
                          "learned solve")
     ap.add_argument("--channels", type=int, default=256)
+    ap.add_argument("--no-channels", action="store_true",
+                    help="leave the 256-channel summary out of the default line")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned comparison of the default run")
     return ap.parse_args()
@@ -467,6 +473,76 @@ def learned_channels(ticks, log=None):
     return res
 
 
+def channels_summary(a):
+    """The 256 uneven channels (as --workload channels) in brief for the
+    default line: one warm-up tick, the whole-tick tuner, two timed ticks."""
+    from stroemung_amd import workloads
+    from stroemung_amd.simulation import Context
+
+    ws = workloads.random_channels(count=256, seed=0, eps=1e-5, check_every=a.check_every)
+    ctx = Context(0)
+    try:
+        for ct, vel, prm in ws:
+            ctx.add_grid(prm, ct, vel)
+        ctx.construct()
+        cells = [(ct.shape[0] - 2) * (ct.shape[1] - 2) for ct, _, _ in ws]
+        ctx.tick()
+        ctx.autotune_ticks()          # as --workload channels: the whole-tick tuner
+        variant, k, tx, ty = ctx.solver_variant()
+        ctx.synchronize()
+        its_all, bound = [], True
+        t0 = time.perf_counter()
+        for _ in range(2):
+            its, _ = ctx.tick()
+            its_all += its
+        ctx.synchronize()
+        el = time.perf_counter() - t0
+        for c in range(len(ws)):
+            st = ctx.state(c)
+            bound = bound and st["residual_squared"] <= st["tolerance_squared"] \
+                and not st["exhausted"]
+    finally:
+        ctx.close()
+    return {"workload": "256 channels, imax 64-512 x jmax 32-256 log-uniform, 1-4 random "
+                        "obstacles, Re 50-1000, eps 1e-5, seed 0 (as --workload channels): "
+                        "1 warm-up tick, the whole-tick tuner, 2 timed ticks; variant %d (K=%d)"
+                        % (variant, k),
+            "cell_updates_per_s": sum(cells) * 2 / el, "ms_per_tick": el * 1e3 / 2,
+            "interior_cells": sum(cells),
+            "sweeps_per_channel_tick": {"mean": sum(its_all) / len(its_all),
+                                        "max": max(its_all), "min": min(its_all)},
+            "residual_bound_met": bound}
+
+
+def learned_channels_summary(log=None):
+    """The 64 channels 512 x 256 at eps 1e-7 (as --workload learned_channels)
+    in brief for the default line: a shorter training (half the capture ticks
+    and Adam steps, one DAgger round) and 4 plain warm-up ticks, then the
+    plain and learned solves over the same 3 ticks, each solve to the same
+    residual bound."""
+    from stroemung_amd import learned, workloads
+    log = log or (lambda *x: None)
+    train = workloads.fixed_channels(64, seed=1001, eps=1e-7)
+    params, info = learned.train_net(train, coarsen=8, capture_ticks=20, points=4,
+                                     capture_sweeps=32, steps=1500, batch=32, dagger_rounds=1,
+                                     dagger_ticks=12, dagger_steps=800, cycles=64, sweeps=32,
+                                     step="momentum", log=log)
+    test = workloads.fixed_channels(64, seed=1, eps=1e-7)
+    res = learned.compare(test, params, coarsen=8, cycles=64, sweeps=32, warm=4, ticks=3,
+                          step="momentum", log=log)
+    pl, lr = res["plain"], res["learned"]
+    return {"workload": "64 channels 512x256, 1-4 random obstacles, Re 100-1000, eps 1e-7 "
+                        "(as --workload learned_channels, shorter training): trained on seed "
+                        "1001, measured on seed 1 over 3 ticks after 4 plain ticks",
+            "plain_cell_updates_per_s": pl["cell_updates_per_s"],
+            "learned_cell_updates_per_s": lr["cell_updates_per_s"],
+            "speedup": res["speedup"],
+            "plain_sweeps_per_solve": pl["sweeps_per_solve"],
+            "learned_sweeps_per_solve": lr["sweeps_per_solve"],
+            "residual_bound_met": pl["residual_bound_met"] and lr["residual_bound_met"],
+            "training_wall_s": info.get("wall_s")}
+
+
 def learned_uneven(ticks, log=None):
     """The 256 uneven channels of the batched workload (64x32 to 512x256, seed
     0, eps 1e-5): network trained on the 256 channels of seed 1001, one frame
@@ -741,6 +817,11 @@ def main():
         out["cpu_baseline"]["parallel_red_black"] = cpu_parallel(n, sweeps_per_step,
                                                                  4 * a.cpu_sweeps)
     ctx.close()
+    if not a.no_channels:
+        try:
+            out["channels"] = channels_summary(a)
+        except Exception as e:   # the plain measurement stands on its own
+            out["channels"] = {"error": "%s: %s" % (type(e).__name__, e)}
     if not a.no_learned:
         log = lambda *x: print("learned:", *x, file=sys.stderr, flush=True)  # noqa: E731
         try:
@@ -750,6 +831,10 @@ def main():
             out["learned"] = learned_cavities(10, log, trained)
         except Exception as e:   # the plain measurement stands on its own
             out["learned"] = {"error": "%s: %s" % (type(e).__name__, e)}
+        try:
+            out["learned_channels"] = learned_channels_summary(log)
+        except Exception as e:
+            out["learned_channels"] = {"error": "%s: %s" % (type(e).__name__, e)}
     print(json.dumps(out), flush=True)
 
 

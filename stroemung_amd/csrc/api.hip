@@ -77,7 +77,8 @@ struct stm_ctx {
     Tile* d_tiles = nullptr;
     double* d_part = nullptr;
     int32_t* d_active = nullptr;
-    int32_t* h_active = nullptr;   // pinned
+    int32_t* h_active = nullptr;   // pinned, two slots of nch flags (the blocked solve's polls)
+    hipEvent_t flag_evs[2] = {};   // one per slot: its copy has landed
     double* d_tbpart = nullptr;    // blocked solver: residual partials, two launches
     TbTile* d_tb = nullptr;
     int ntb = 0, tb_capacity = 0, part_capacity = 0;
@@ -498,7 +499,9 @@ int finalize(stm_ctx* ctx) {
     ctx->part_capacity = (int)maxtiles;
     HIPCHK(ctx, hipMalloc(&ctx->d_active, sizeof(int32_t) * nch));
     HIPCHK(ctx, hipMalloc(&ctx->d_tbpart, sizeof(double) * 2 * maxtiles));
-    HIPCHK(ctx, hipHostMalloc(&ctx->h_active, sizeof(int32_t) * nch, hipHostMallocDefault));
+    HIPCHK(ctx, hipHostMalloc(&ctx->h_active, sizeof(int32_t) * 2 * nch, hipHostMallocDefault));
+    for (hipEvent_t& e : ctx->flag_evs)
+        HIPCHK(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
     HIPCHK(ctx, hipHostMalloc(&ctx->h_cs, sizeof(ChanState) * nch, hipHostMallocDefault));
     HIPCHK(ctx, hipMemcpyAsync(ctx->d_tiles, tl.data(), sizeof(Tile) * tl.size(),
                                hipMemcpyHostToDevice, ctx->stream));
@@ -623,28 +626,17 @@ int stage_rhs(stm_ctx* ctx) {
     return STM_OK;
 }
 
-// Reads the per-channel "still running" flags; *any = some channel runs.
-int poll_active(stm_ctx* ctx, bool* any) {
-    HIPCHK(ctx, hipMemcpyAsync(ctx->h_active, ctx->d_active, sizeof(int32_t) * nch(ctx),
-                               hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
-    int rc = check_launch(ctx);
-    if (rc) return rc;
-    *any = false;
-    for (int k = 0; k < nch(ctx); k++) *any = *any || ctx->h_active[k];
-    return STM_OK;
-}
-
-// After a poll: when the channels still running hold under 7/8 of the tiles
-// launched so far, later launches cover only their tiles.  A channel that
-// stopped has already taken its stop decision (that is what the poll read),
-// and every tile of a running channel is kept, so each launch still writes
-// the partials its successors read.  The pinned staging buffer is rewritten
-// only after the next poll, when this copy has completed.
-int compact_tiles(stm_ctx* ctx) {
+// After a poll (`active`: the flags it read): when the channels still running
+// hold under 7/8 of the tiles launched so far, later launches cover only
+// their tiles.  A channel that stopped has already taken its stop decision
+// (that is what the poll read), and every tile of a running channel is kept,
+// so each launch still writes the partials its successors read.  The pinned
+// staging buffer is rewritten only after a later poll, when this copy (queued
+// before that poll's flags) has completed.
+int compact_tiles(stm_ctx* ctx, const int32_t* active) {
     int n = 0;
     for (int k = 0; k < nch(ctx); k++)
-        if (ctx->h_active[k]) n += ctx->chans[k].ntb;
+        if (active[k]) n += ctx->chans[k].ntb;
     if (n == 0 || 8 * n >= 7 * ctx->ntb_cur) return STM_OK;
     if (ctx->tbc_capacity < ctx->ntb) {
         if (ctx->d_tbc) HIPCHK(ctx, hipFree(ctx->d_tbc));
@@ -657,7 +649,7 @@ int compact_tiles(stm_ctx* ctx) {
     }
     int m = 0;
     for (int k = 0; k < nch(ctx); k++) {
-        if (!ctx->h_active[k]) continue;
+        if (!active[k]) continue;
         const HostChan& c = ctx->chans[k];
         for (int i = c.tb0; i < c.tb0 + c.ntb; i++) ctx->h_tbc[m++] = ctx->h_tb[i];
     }
@@ -679,11 +671,15 @@ void launch_blocked(stm_ctx* ctx, int variant, int L, int mode) {
 
 // Blocked red-black launches until no channel is running.  Launch L also
 // takes the stop decision on launch L - 1, so a solve ends with one launch
-// that only decides.  Launches are queued in growing batches between polls
-// of the per-channel flags; a launch finding its channel stopped returns at
-// once.  check = 0 runs a fixed number of sweeps without the residual.
+// that only decides.  Launches are queued in growing batches; after each
+// batch the per-channel flags are copied to a pinned slot (slots alternate,
+// each with an event), and the host then reads the previous batch's slot: the
+// queue holds a batch while the host decides, and at most one batch after the
+// stop runs for nothing (a launch finding its channel stopped returns after
+// its loads).  check = 0 runs a fixed number of sweeps without the residual.
 int blocked_solve(stm_ctx* ctx, int check) {
-    int batch = 2, L = 0;
+    int batch = 2, L = 0, prev = -1, slot = 0;
+    const int n = nch(ctx);
     ctx->tb_cur = ctx->d_tb;
     ctx->ntb_cur = ctx->ntb;
     for (;;) {
@@ -718,11 +714,21 @@ int blocked_solve(stm_ctx* ctx, int check) {
             ctx->pending.push_back(PendingTiming{&ctx->stats["rb_tb_batch"], ea, eb,
                                                  (uint32_t)batch});
         }
-        bool any = false;
-        int rc = poll_active(ctx, &any);
-        if (rc) return rc;
-        if (!any) break;
-        if ((rc = compact_tiles(ctx))) return rc;
+        HIPCHK(ctx, hipMemcpyAsync(ctx->h_active + (size_t)slot * n, ctx->d_active,
+                                   sizeof(int32_t) * n, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(ctx, hipEventRecord(ctx->flag_evs[slot], ctx->stream));
+        if (prev >= 0) {
+            HIPCHK(ctx, hipEventSynchronize(ctx->flag_evs[prev]));
+            int rc = check_launch(ctx);
+            if (rc) return rc;
+            const int32_t* active = ctx->h_active + (size_t)prev * n;
+            bool any = false;
+            for (int k = 0; k < n; k++) any = any || active[k];
+            if (!any) break;
+            if ((rc = compact_tiles(ctx, active))) return rc;
+        }
+        prev = slot;
+        slot ^= 1;
         if (batch < 64) batch *= 2;
     }
     ctx->tb_cur = ctx->d_tb;
@@ -1174,6 +1180,8 @@ void stm_destroy(stm_ctx* ctx) {
     for (void* b : bufs)
         if (b) hipFree(b);
     if (ctx->h_active) hipHostFree(ctx->h_active);
+    for (hipEvent_t e : ctx->flag_evs)
+        if (e) hipEventDestroy(e);
     if (ctx->h_cs) hipHostFree(ctx->h_cs);
     if (ctx->h_lc) hipHostFree(ctx->h_lc);
     if (ctx->h_tbc) hipHostFree(ctx->h_tbc);

@@ -75,8 +75,10 @@ __device__ __forceinline__ uint32_t ghost_bits(uint32_t edge) {
 
 // The forms a wave runs (wave-uniform bits): it holds a ghost cell, a corner
 // ghost, a fluid cell that is not updated (SEL); its vertical substitutions
-// (2 bits at W_VSHIFT: none, per lane, per cell); per-cell E / W ones (HX).
-enum : int { W_GHOST = 1, W_DIAG = 2, W_SEL = 4, W_VSHIFT = 3, W_HX = 32 };
+// (2 bits at W_VSHIFT: none, per lane, per cell); per-cell E / W ones (HX);
+// every ghost of the wave is straight and its fluid neighbour is a cell of
+// the same thread (OWN).
+enum : int { W_GHOST = 1, W_DIAG = 2, W_SEL = 4, W_VSHIFT = 3, W_HX = 32, W_OWN = 64 };
 
 // What a thread knows of its 2R own cells, cell i = 2r + q, packed so the
 // codes take a few registers: bit i of `upd` = updated by SOR, of `out` = in
@@ -349,6 +351,28 @@ struct Blk {
         return w;
     }
 
+    // The final refresh of a wave whose ghosts are all straight with their
+    // fluid neighbour in the thread (W_OWN): each takes that neighbour's
+    // value from the registers -- the other cell of its pair (bit i of cpv),
+    // the next row (bit i of cph) or the previous row (bit SH + i of cph).
+    // Only that neighbour reads such a ghost, from the registers too, so
+    // nothing goes to LDS.  The sources are fluid cells, which no copy
+    // changes.
+    __device__ static __forceinline__ void own_refresh(double (*pv)[2], uint32_t cpv,
+                                                       typename Codes<R>::Sub cph) {
+#pragma unroll
+        for (int r = 0; r < R; r++)
+#pragma unroll
+            for (int q = 0; q < 2; q++) {
+                const int i = 2 * r + q;
+                double v = pv[r][q];
+                v = ((cpv >> i) & 1u) ? pv[r][q ^ 1] : v;
+                if (r + 1 < R) v = ((cph >> i) & 1u) ? pv[r + 1 < R ? r + 1 : r][q] : v;
+                if (r >= 1) v = ((cph >> (Codes<R>::SH + i)) & 1u) ? pv[r >= 1 ? r - 1 : r][q] : v;
+                pv[r][q] = v;
+            }
+    }
+
     // Before the sweeps (a barrier follows): the black corner ghosts, for
     // the first red half.
     __device__ static __forceinline__ void start(double (*P)[PS], double (&pv)[R][2],
@@ -390,7 +414,8 @@ struct Blk {
     // refresh of every ghost, and the residual when checked.
     __device__ static __forceinline__ double run(double (*P)[PS], double (&pv)[R][2],
                                                  const double (&rr)[R][2],
-                                                 Codes<R>& cc, uint32_t vown, int lx0, int m,
+                                                 Codes<R>& cc, uint32_t vown, uint32_t cpv,
+                                                 typename Codes<R>::Sub cph, int lx0, int m,
                                                  int nsw, bool check, int wm,
                                                  const ChanDesc& d) {
         const double omw = d.omw, mid = d.mid, rdx2 = d.rdx2, rdy2 = d.rdy2;
@@ -408,8 +433,12 @@ struct Blk {
 #undef STM_SW
         // every ghost from the final values, for the output and the residual
         if (ghost) {
-            ghosts<0, false>(P, pv, cc, lx0, m);
-            ghosts<1, false>(P, pv, cc, lx0, m);
+            if (wm & W_OWN) {
+                own_refresh(pv, cpv, cph);
+            } else {
+                ghosts<0, false>(P, pv, cc, lx0, m);
+                ghosts<1, false>(P, pv, cc, lx0, m);
+            }
         }
         double acc = 0.0;
         if (check) {
@@ -548,6 +577,7 @@ k_rb_strip(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
     cc.upd = cc.out = cc.diag = 0;
     cc.vsub = cc.hsub = 0;
     uint32_t keep = 0;   // fluid cells inside the border that are not updated
+    uint32_t ghm = 0;    // ghost cells
     uint32_t upd7[NW];   // bit 7 of each updated cell's byte
 #pragma unroll
     for (int k = 0; k < Codes<R>::NG; k++) cc.g[k] = 0;
@@ -568,13 +598,15 @@ k_rb_strip(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
         cc.out |= nib7(o) << (4 * k);
         cc.diag |= nib7(corner) << (4 * k);
         keep |= nib7(rin & fluid & ~ru) << (4 * k);
+        ghm |= nib7(zero7(e4) ^ 0x80808080u) << (4 * k);
         const uint32_t y = e4 | (e4 >> 4);
         cc.g[k >> 1] |= ((y & 0xffu) | ((y >> 8) & 0xff00u)) << (16 * (k & 1));
     }
 
     int wm = (ex0 & 1) ? Blk<K, R, HY, NS, 1>::init(P, FL, pv, fo, cc, lx0, m)
                        : Blk<K, R, HY, NS, 0>::init(P, FL, pv, fo, cc, lx0, m);
-    uint32_t vown = 0;
+    uint32_t vown = 0, cpv = 0;
+    typename Codes<R>::Sub cph = 0;
 
     // ---- the updated cells' neighbours that are straight ghosts of them,
     // from the flags every thread has put in LDS (N / S across the pair
@@ -622,6 +654,13 @@ k_rb_strip(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
         const bool lane_ok = cross == 0 && (own_n == 0 || own_n == upd1) &&
                              (own_s == 0 || own_s == upd0);
         vown = (own_n != 0 ? 1u : 0u) | (own_s != 0 ? 2u : 0u);
+        // the ghosts whose one fluid neighbour is a cell of this thread (and
+        // updated): the other cell of the pair, the next or previous row
+        constexpr uint32_t CELLS = (2 * R < 32) ? (1u << 2 * R) - 1 : ~0u;
+        cpv = (own_n >> 1) | (own_s << 1);
+        const uint32_t ce = (wsub >> 2), cw = (esub << 2) & CELLS;
+        cph = (Sub)ce | (Sub)cw << Codes<R>::SH;
+        wm |= __ballot(ghm != (cpv | ce | cw)) == 0 ? W_OWN : 0;
         const int vm = __ballot((nsub | ssub) != 0) == 0 ? 0 : (__ballot(!lane_ok) != 0 ? 2 : 1);
         wm |= vm << W_VSHIFT;
         wm |= __ballot((esub | wsub) != 0) != 0 ? W_HX : 0;
@@ -664,8 +703,8 @@ k_rb_strip(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
     const int nsw = itermax - done < K ? itermax - done : K;
 
     const double acc =
-        (ex0 & 1) ? Blk<K, R, HY, NS, 1>::run(P, pv, rr, cc, vown, lx0, m, nsw, check, wm, d)
-                  : Blk<K, R, HY, NS, 0>::run(P, pv, rr, cc, vown, lx0, m, nsw, check, wm, d);
+        (ex0 & 1) ? Blk<K, R, HY, NS, 1>::run(P, pv, rr, cc, vown, cpv, cph, lx0, m, nsw, check, wm, d)
+                  : Blk<K, R, HY, NS, 0>::run(P, pv, rr, cc, vown, cpv, cph, lx0, m, nsw, check, wm, d);
 
     // ---- write the output tile.  Output cells that are neither updated nor
     // ghosts keep their value in every sweep: they are taken from the input

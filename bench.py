@@ -49,8 +49,10 @@ HBM_PEAK_GBS = 8000.0
 # the roofline's achieved rate
 SWEEP_BYTES_PER_CELL = 33.0
 # stage kernels: algorithmic bytes per cell (f64 8 B, flags 1 B, each value once)
-# (adap_ranges: f, g, p read, u, v written, flags; the ranges add no traffic)
-STAGE_BYTES_PER_CELL = {"fg": 33.0, "rhs": 24.0, "adap_ranges": 41.0, "ranges": 25.0}
+# (fg_rhs: u, v, flags read, f, g, rhs written; adap_ranges: f, g, p, flags read,
+# u, v written -- the ranges add no traffic)
+STAGE_BYTES_PER_CELL = {"fg": 33.0, "rhs": 24.0, "fg_rhs": 41.0, "adap_ranges": 41.0,
+                        "ranges": 25.0}
 COPY_PROBE_BYTES = 256 << 20
 # template arguments (R, HY, NS) of each blocked-solver variant, after K
 VARIANT_SHAPES = {0: "4, 32, 16", 1: "4, 32, 16", 2: "4, 64, 16", 3: "4, 32, 16",
@@ -744,8 +746,8 @@ def main():
     effective = SWEEP_BYTES_PER_CELL * ksweeps * cells / avg_s / 1e9 if timed else None
     working_set = (3 * 8 + 1) * (n + 2) * (n + 2)
     kernels = {}
-    for k in ("rb_tb", "pbuf_fix", "solve_begin", "rhs_stats", "fg", "rhs", "adap_ranges",
-              "restore", "bc_velocity", "delt"):
+    for k in ("rb_tb", "pbuf_fix", "solve_begin", "rhs_stats", "fg_rhs", "fg", "rhs",
+              "adap_ranges", "restore", "bc_velocity", "delt"):
         kms, kt, kl = ctx.kernel_stats(k)
         if kl:
             kernels[k] = {"launches": kl, "timed": kt,

@@ -676,8 +676,10 @@ void launch_blocked(stm_ctx* ctx, int variant, int L, int mode) {
 // each with an event), and the host then reads the previous batch's slot: the
 // queue holds a batch while the host decides, and at most one batch after the
 // stop runs for nothing (a launch finding its channel stopped returns after
-// its loads).  check = 0 runs a fixed number of sweeps without the residual.
-int blocked_solve(stm_ctx* ctx, int check) {
+// its loads).  check = 0 runs a fixed number of sweeps without the residual;
+// fixed > 0: exactly that many launches, with no polls (every channel's
+// budget ends at the same launch).
+int blocked_solve(stm_ctx* ctx, int check, int fixed = 0) {
     int batch = 2, L = 0, prev = -1, slot = 0;
     const int n = nch(ctx);
     ctx->tb_cur = ctx->d_tb;
@@ -694,6 +696,7 @@ int blocked_solve(stm_ctx* ctx, int check) {
             HIPCHK(ctx, hipEventRecord(ea, ctx->stream));
             ctx->span_open = true;
         }
+        if (fixed > 0) batch = std::min(batch, fixed - L);
         for (int b = 0; b < batch; b++, L++) {
             const int rc = launch(ctx, "rb_tb", [&] {
                 launch_blocked(ctx, ctx->variant, L, check ? TB_CHECK : 0);
@@ -713,6 +716,11 @@ int blocked_solve(stm_ctx* ctx, int check) {
             HIPCHK(ctx, hipEventRecord(eb, ctx->stream));
             ctx->pending.push_back(PendingTiming{&ctx->stats["rb_tb_batch"], ea, eb,
                                                  (uint32_t)batch});
+        }
+        if (fixed > 0) {
+            if (L >= fixed) break;
+            if (batch < 64) batch *= 2;
+            continue;
         }
         HIPCHK(ctx, hipMemcpyAsync(ctx->h_active + (size_t)slot * n, ctx->d_active,
                                    sizeof(int32_t) * n, hipMemcpyDeviceToHost, ctx->stream));
@@ -1504,7 +1512,9 @@ int stm_rb_sweeps(stm_ctx* ctx, uint32_t n) {
     std::vector<ChanState> saved;
     if ((rc = stage_rhs_stats(ctx))) return rc;     // the projection uses the current rhs
     if ((rc = arm_channels(ctx, (int)n, false, &saved))) return rc;
-    if ((rc = blocked_solve(ctx, 0))) return rc;
+    // every channel's budget is n: ceil(n / K) launches sweep, one more decides
+    const int k = TB_VARIANTS[ctx->variant].k;
+    if ((rc = blocked_solve(ctx, 0, ((int)n + k - 1) / k + 1))) return rc;
     // restore the scalar state, keep the new p
     for (auto& s : saved) s.pbuf = 0;
     HIPCHK(ctx, hipMemcpyAsync(ctx->d_cs, saved.data(), sizeof(ChanState) * saved.size(),

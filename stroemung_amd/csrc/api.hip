@@ -79,6 +79,8 @@ struct stm_ctx {
     int32_t* d_active = nullptr;
     int32_t* h_active = nullptr;   // pinned, two slots of nch flags (the blocked solve's polls)
     hipEvent_t flag_evs[2] = {};   // one per slot: its copy has landed
+    hipEvent_t tbc_ev = nullptr;   // the last copy out of h_tbc has completed
+    bool tbc_pending = false;      // tbc_ev recorded and not yet waited for
     double* d_tbpart = nullptr;    // blocked solver: residual partials, two launches
     TbTile* d_tb = nullptr;
     int ntb = 0, tb_capacity = 0, part_capacity = 0;
@@ -502,6 +504,7 @@ int finalize(stm_ctx* ctx) {
     HIPCHK(ctx, hipHostMalloc(&ctx->h_active, sizeof(int32_t) * 2 * nch, hipHostMallocDefault));
     for (hipEvent_t& e : ctx->flag_evs)
         HIPCHK(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    HIPCHK(ctx, hipEventCreateWithFlags(&ctx->tbc_ev, hipEventDisableTiming));
     HIPCHK(ctx, hipHostMalloc(&ctx->h_cs, sizeof(ChanState) * nch, hipHostMallocDefault));
     HIPCHK(ctx, hipMemcpyAsync(ctx->d_tiles, tl.data(), sizeof(Tile) * tl.size(),
                                hipMemcpyHostToDevice, ctx->stream));
@@ -631,13 +634,18 @@ int stage_rhs(stm_ctx* ctx) {
 // their tiles.  A channel that stopped has already taken its stop decision
 // (that is what the poll read), and every tile of a running channel is kept,
 // so each launch still writes the partials its successors read.  The pinned
-// staging buffer is rewritten only after a later poll, when this copy (queued
-// before that poll's flags) has completed.
+// staging buffer is rewritten only after the copy out of it has completed
+// (the host reads a batch's flags while the next batch is queued, so a poll
+// does not imply that the copy queued after the previous one has run).
 int compact_tiles(stm_ctx* ctx, const int32_t* active) {
     int n = 0;
     for (int k = 0; k < nch(ctx); k++)
         if (active[k]) n += ctx->chans[k].ntb;
     if (n == 0 || 8 * n >= 7 * ctx->ntb_cur) return STM_OK;
+    if (ctx->tbc_pending) {
+        HIPCHK(ctx, hipEventSynchronize(ctx->tbc_ev));
+        ctx->tbc_pending = false;
+    }
     if (ctx->tbc_capacity < ctx->ntb) {
         if (ctx->d_tbc) HIPCHK(ctx, hipFree(ctx->d_tbc));
         if (ctx->h_tbc) HIPCHK(ctx, hipHostFree(ctx->h_tbc));
@@ -655,6 +663,8 @@ int compact_tiles(stm_ctx* ctx, const int32_t* active) {
     }
     HIPCHK(ctx, hipMemcpyAsync(ctx->d_tbc, ctx->h_tbc, sizeof(TbTile) * m, hipMemcpyHostToDevice,
                                ctx->stream));
+    HIPCHK(ctx, hipEventRecord(ctx->tbc_ev, ctx->stream));
+    ctx->tbc_pending = true;
     ctx->tb_cur = ctx->d_tbc;
     ctx->ntb_cur = m;
     return STM_OK;
@@ -1190,6 +1200,7 @@ void stm_destroy(stm_ctx* ctx) {
     if (ctx->h_active) hipHostFree(ctx->h_active);
     for (hipEvent_t e : ctx->flag_evs)
         if (e) hipEventDestroy(e);
+    if (ctx->tbc_ev) hipEventDestroy(ctx->tbc_ev);
     if (ctx->h_cs) hipHostFree(ctx->h_cs);
     if (ctx->h_lc) hipHostFree(ctx->h_lc);
     if (ctx->h_tbc) hipHostFree(ctx->h_tbc);

@@ -116,10 +116,23 @@ k_conv3x3(const bf16* __restrict__ in, int cin, const bf16* __restrict__ w,
 }
 
 // LDS-tiled variant.  A workgroup owns a TH x TW pixel tile of one image
-// (TH * TW = 64 or 128) and 32 * WN output channels; for every 32-channel
-// slice of the input it stages the (TH + 2) x (TW + 2) patch once in LDS and
-// serves all nine taps from it, so each input value is read from memory once
-// per slice instead of nine times.  Waves: (4 / WN) in M x WN in N.
+// (TH * TW = 64 or 128) and NW = 32 * WN output channels; for every
+// 32-channel slice of the input it stages in LDS the (TH + 2) x (TW + 2)
+// patch and the slice's weights of its NW channels, and serves all nine taps
+// and all four waves from them: each input value is read from memory once per
+// slice and channel block, each weight once per workgroup.  Waves: (4 / WN) in
+// M x WN in N; a wave owns MT M tiles of 16 pixels (RM = 16 / TW image rows
+// each) on consecutive rows, and 32 output channels (two N tiles).  The
+// MFMAs take the weights as their first operand, so each result holds
+// channels as rows and pixels as columns, and a lane ends with four
+// consecutive channels of one pixel to store.
+//
+// The MFMA operand of tap (dy, dx) for M tile i is the patch block that
+// starts RM * i + dy rows below the wave's first row, shifted by dx: the
+// wave walks the distinct starts s once per dx and feeds each fragment to
+// every (i, dy) with RM * i + dy = s, so a fragment is read from LDS once for
+// up to three taps.  Each output's taps are added in the order dx-major,
+// dy ascending (0, 3, 6, 1, 4, 7, 2, 5, 8), slice after slice.
 //
 // The input is described by ConvIn: channels [0, ca) come from `a` read
 // directly, as the 2x2 max of an image twice the size (the U-Net's pooling)
@@ -128,13 +141,13 @@ k_conv3x3(const bf16* __restrict__ in, int cin, const bf16* __restrict__ w,
 // concatenation).  Pooling, upsampling and concatenation thus happen while
 // staging, without materialising their outputs.
 //
-// With PIPE (chosen for workgroups with at least three input slices) the
-// patch is double-buffered: the global loads of slice s + 1 are issued into
-// registers before the MFMAs of slice s, and stored to the other buffer after
-// them, so one barrier per slice suffices and the load latency hides behind
-// the matrix work; that costs registers, so layers with fewer slices (the
-// large shallow ones, where occupancy hides latency instead) stage each slice
-// and then multiply.
+// With PIPE (chosen for workgroups with at least three input slices, in
+// launches of at most two workgroups per CU) patch and weights are
+// double-buffered in LDS and the loads run two slices ahead in registers, so
+// one barrier per slice suffices and the load latency hides behind two
+// slices of matrix work.  The other launches (few slices, or enough
+// workgroups that occupancy hides the latency) stage each slice and then
+// multiply, in half the LDS.
 template <int TW, int TH, int WN, bool POOL, bool PIPE>
 __global__ void __launch_bounds__(256)
 k_conv3x3_tile(ConvIn src, const bf16* __restrict__ w, const float* __restrict__ bias,
@@ -143,12 +156,21 @@ k_conv3x3_tile(ConvIn src, const bf16* __restrict__ w, const float* __restrict__
 {
     constexpr int PH = TH + 2, PW = TW + 2;
     constexpr int CS = 40;                       // padded channel stride of a staged pixel
+    constexpr int CSW = 40;                      // ... of a staged (channel, tap) weight row
+    constexpr int NW = 32 * WN;                  // output channels of the workgroup
     constexpr int WM = 4 / WN;                   // waves along M
+    constexpr int RM = 16 / TW;                  // image rows per M tile
     constexpr int MT = TH * TW / WM / 16;        // M tiles per wave
+    constexpr int NS = RM * (MT - 1) + 3;        // distinct fragment starts per dx
     constexpr int NPV = PH * PW * 4;             // 16-byte vectors of one staged slice
     constexpr int NV = (NPV + 255) / 256;        // of them per thread
+    constexpr int NWVEC = NW * 9 * 4;            // 16-byte vectors of one slice's weights
+    constexpr int NWV = (NWVEC + 255) / 256;
+    constexpr int NB = PIPE ? 2 : 1;
+    static_assert(TW == 8 || TW == 16, "tile width");
     static_assert(MT >= 1 && TH * TW % (WM * 16) == 0, "tile shape");
-    __shared__ __attribute__((aligned(16))) bf16 patch[2][PH * PW * CS];
+    __shared__ __attribute__((aligned(16))) bf16 patch[NB][PH * PW * CS];
+    __shared__ __attribute__((aligned(16))) bf16 wl[NB][NW * 9 * CSW];
 
     const int cin = src.ca + src.cb;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -160,16 +182,14 @@ k_conv3x3_tile(ConvIn src, const bf16* __restrict__ w, const float* __restrict__
     const int th0 = (t % tiles_h) * TH;
     const int b = t / tiles_h;
     if (src.live && b >= *src.live) return;      // uniform over the block, before any barrier
-    const int n_base = blockIdx.y * (32 * WN) + wn * 32;
+    const int nb0 = blockIdx.y * NW;
+    const int n_base = nb0 + wn * 32;
     const int row = lane & 15, kq = (lane >> 4) * 8;
 
-    // the LDS offset of each M tile's pixel for this lane (tap (0, 0) at +1, +1)
-    int aoff[MT];
-#pragma unroll
-    for (int i = 0; i < MT; i++) {
-        const int m = (wm * MT + i) * 16 + row;
-        aoff[i] = ((m / TW) * PW + (m % TW)) * CS + kq;
-    }
+    // this lane's LDS offsets: its pixel of the wave's first M tile at tap
+    // (0, 0), and its weight row (channel n_base + row) at tap 0
+    const int aoff = ((wm * MT * RM + row / TW) * PW + row % TW) * CS + kq;
+    const int boff = (wn * 32 + row) * 9 * CSW + kq;
     f32x4 acc[MT][2];
 #pragma unroll
     for (int i = 0; i < MT; i++)
@@ -181,21 +201,23 @@ k_conv3x3_tile(ConvIn src, const bf16* __restrict__ w, const float* __restrict__
 
     // Staging: the loads of a slice are issued unconditionally (addresses
     // clamped into the image, out-of-image vectors zeroed when stored), with
-    // no branch between them, so the compiler can count them exactly and the
-    // MFMAs wait only for their weights; a pooled input (POOL, src.mode ==
-    // IN_POOL, never with a skip part) keeps its four window values in
-    // registers and takes their max when storing.
+    // no branch between them, so the compiler can count them exactly; a
+    // pooled input (POOL, src.mode == IN_POOL, never with a skip part) keeps
+    // its four window values in registers and takes their max when storing.
     constexpr bool pool = POOL;
     constexpr int NR = POOL ? 4 : 1;
-    bf16x8 raw[NV][NR];
-    uint32_t valid = 0;
-    auto fetch = [&](int c0) {
+    struct Stage {               // one slice's loads in flight
+        bf16x8 raw[NV][NR];
+        bf16x8 w[NWV];
+        uint32_t valid;
+    };
+    auto fetch = [&](Stage& g, int c0) {
         const bool first = c0 < src.ca;
         const bf16* base = first ? src.a : src.b;
         const int cs = first ? src.ca : src.cb, cc = first ? c0 : c0 - src.ca;
         const int up = (first && src.mode == IN_UP) ? 1 : 0, dn = pool ? 1 : 0;
         const int Hs = (H << dn) >> up, Ws = (W << dn) >> up;     // source image
-        valid = 0;
+        g.valid = 0;
 #pragma unroll
         for (int k = 0; k < NV; k++) {
             const int v = threadIdx.x + k * 256;
@@ -203,136 +225,200 @@ k_conv3x3_tile(ConvIn src, const bf16* __restrict__ w, const float* __restrict__
             const int pix = vv >> 2, part = vv & 3;
             const int ph = pix / PW, pw = pix - ph * PW;
             const int gh = th0 + ph - 1, gw = tw0 + pw - 1;
-            if (v < NPV && gh >= 0 && gh < H && gw >= 0 && gw < W) valid |= 1u << k;
+            if (v < NPV && gh >= 0 && gh < H && gw >= 0 && gw < W) g.valid |= 1u << k;
             const int ghc = min(max(gh, 0), H - 1), gwc = min(max(gw, 0), W - 1);
             const int64_t r0 = (((int64_t)b * Hs + ((ghc << dn) >> up)) * Ws + ((gwc << dn) >> up))
                                * cs + cc + part * 8;
-            raw[k][0] = load8(base + r0);
+            g.raw[k][0] = load8(base + r0);
             if constexpr (POOL) {
                 const int64_t rs = (int64_t)Ws * cs;
-                raw[k][1] = load8(base + r0 + cs);
-                raw[k][2] = load8(base + r0 + rs);
-                raw[k][3] = load8(base + r0 + rs + cs);
+                g.raw[k][1] = load8(base + r0 + cs);
+                g.raw[k][2] = load8(base + r0 + rs);
+                g.raw[k][3] = load8(base + r0 + rs + cs);
             }
         }
+        // the weights w[nb0 + n][tap][c0 .. c0 + 31] of the slice: vector
+        // v = (n * 9 + tap) * 4 + part
+#pragma unroll
+        for (int k = 0; k < NWV; k++) {
+            const int v = threadIdx.x + k * 256;
+            const int vv = v < NWVEC ? v : 0;
+            g.w[k] = load8(w + (int64_t)(nb0 + (vv >> 2) / 9) * 9 * cin +
+                           (int64_t)((vv >> 2) % 9) * cin + c0 + (vv & 3) * 8);
+        }
     };
-    auto put = [&](int buf) {
+    auto put = [&](const Stage& g, int buf) {
 #pragma unroll
         for (int k = 0; k < NV; k++) {
             const int v = threadIdx.x + k * 256;
             if (v >= NPV) continue;
-            bf16x8 x = raw[k][0];
+            bf16x8 x = g.raw[k][0];
             if constexpr (POOL) {
 #pragma unroll
                 for (int e = 0; e < 8; e++)
-                    x[e] = (bf16)fmaxf(fmaxf((float)raw[k][0][e], (float)raw[k][1][e]),
-                                       fmaxf((float)raw[k][2][e], (float)raw[k][3][e]));
+                    x[e] = (bf16)fmaxf(fmaxf((float)g.raw[k][0][e], (float)g.raw[k][1][e]),
+                                       fmaxf((float)g.raw[k][2][e], (float)g.raw[k][3][e]));
             }
-            if (!((valid >> k) & 1u)) x = zero8();
+            if (!((g.valid >> k) & 1u)) x = zero8();
             *reinterpret_cast<bf16x8*>(&patch[buf][(v >> 2) * CS + (v & 3) * 8]) = x;
         }
-    };
-
-    // this slice's weights are loaded before the next slice's inputs: the
-    // vector-memory counter completes in issue order, so the MFMAs can wait
-    // for their weights while the prefetch is still in flight
-    bf16x8 bw[9][2];
-    auto weights = [&](int c0) {
 #pragma unroll
-        for (int tap = 0; tap < 9; tap++)
-#pragma unroll
-            for (int u = 0; u < 2; u++)
-                bw[tap][u] = load8(w + ((int64_t)(n_base + u * 16 + row) * 9 + tap) * cin + c0 + kq);
+        for (int k = 0; k < NWV; k++) {
+            const int v = threadIdx.x + k * 256;
+            if (v >= NWVEC) continue;
+            *reinterpret_cast<bf16x8*>(&wl[buf][(v >> 2) * CSW + (v & 3) * 8]) = g.w[k];
+        }
     };
     auto mfmas = [&](int buf) {
+        const bf16* pa = &patch[buf][aoff];
+        const bf16* pb = &wl[buf][boff];
 #pragma unroll
-        for (int tap = 0; tap < 9; tap++) {
-            const int dy = tap / 3, dx = tap % 3;    // patch offsets 0..2
-            const int shift = (dy * PW + dx) * CS;
+        for (int dx = 0; dx < 3; dx++) {
+            bf16x8 bq[3][2];
 #pragma unroll
-            for (int i = 0; i < MT; i++) {
-                const bf16x8 a = *reinterpret_cast<const bf16x8*>(&patch[buf][aoff[i] + shift]);
+            for (int dy = 0; dy < 3; dy++)
 #pragma unroll
                 for (int u = 0; u < 2; u++)
-                    acc[i][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw[tap][u], acc[i][u], 0, 0, 0);
+                    bq[dy][u] = *reinterpret_cast<const bf16x8*>(pb + (u * 16 * 9 + dy * 3 + dx) * CSW);
+#pragma unroll
+            for (int s = 0; s < NS; s++) {
+                const bf16x8 a = *reinterpret_cast<const bf16x8*>(pa + (s * PW + dx) * CS);
+#pragma unroll
+                for (int dy = 0; dy < 3; dy++) {
+                    const int d = s - dy;
+                    if (d < 0 || d % RM != 0 || d / RM >= MT) continue;
+#pragma unroll
+                    for (int u = 0; u < 2; u++)   // channels as rows, pixels as columns
+                        acc[d / RM][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                            bq[dy][u], a, acc[d / RM][u], 0, 0, 0);
+                }
             }
         }
     };
 
     if constexpr (PIPE) {
-        if (sl0 < sl1) {
-            fetch(sl0 * 32);
-            put(0);
-        }
+        // Two slices in flight: slice s is multiplied out of one LDS buffer
+        // while slice s + 1 (loaded during the previous slice) is stored to
+        // the other and slice s + 2 is being loaded, so each load has two
+        // slices' time to arrive.  Slice s of the split sits in buffer
+        // (s - sl0) & 1 and in stage g0 / g1 by the same parity.  The
+        // workgroup has at least three slices (the launcher's condition);
+        // the loads in the main loop are unconditional, so the compiler's
+        // wait counts stay exact.
+        Stage g0, g1;
+        fetch(g0, sl0 * 32);
+        fetch(g1, min(sl0 + 1, sl1 - 1) * 32);
+        put(g0, 0);
         __syncthreads();
-        // every slice but the last prefetches its successor; no load in the
-        // loop is conditional, so the compiler's wait counts stay exact
-        for (int s = sl0; s + 1 < sl1; s++) {
-            const int buf = (s - sl0) & 1, c0 = s * 32;
-            weights(c0);
-            __builtin_amdgcn_sched_barrier(0);   // keep the issue order: weights, then inputs
-            fetch(c0 + 32);
+        int s = sl0;
+        for (; s + 3 < sl1; s += 2) {
+            fetch(g0, (s + 2) * 32);
+            __builtin_amdgcn_sched_barrier(0);   // the loads issue before the MFMAs
+            mfmas(0);
+            put(g1, 1);          // buffer 1 was last read before the previous barrier
+            __syncthreads();
+            fetch(g1, (s + 3) * 32);
             __builtin_amdgcn_sched_barrier(0);
-            mfmas(buf);
-            put(buf ^ 1);       // the other buffer was last read before the previous barrier
+            mfmas(1);
+            put(g0, 0);
             __syncthreads();
         }
-        if (sl0 < sl1) {
-            weights((sl1 - 1) * 32);
-            mfmas((sl1 - 1 - sl0) & 1);
+        // the last one to three slices: s in buffer 0, s + 1 in g1
+        if (s + 2 < sl1) fetch(g0, (s + 2) * 32);
+        mfmas(0);
+        if (s + 1 < sl1) {
+            put(g1, 1);
+            __syncthreads();
+            mfmas(1);
+            if (s + 2 < sl1) {
+                put(g0, 0);
+                __syncthreads();
+                mfmas(0);
+            }
         }
     } else {
-        // few slices per workgroup: stage, then multiply, in fewer registers
+        Stage g;
         for (int s = sl0; s < sl1; s++) {
             if (s > sl0) __syncthreads();        // the previous slice's MFMAs are done
-            fetch(s * 32);
-            put(0);
+            fetch(g, s * 32);
+            put(g, 0);
             __syncthreads();
-            weights(s * 32);
             mfmas(0);
         }
     }
-    // epilogue: lane holds column n = l % 16, rows 4 * (l / 16) + j of each M tile
+    // epilogue: the result tile is (channels x pixels), so the lane holds
+    // pixel row of each M tile and its four consecutive channels
+    // n_base + 16 u + 4 (l / 16) + 0..3: one 8-byte store of bf16 (16 bytes
+    // of fp32) per tile and channel half
+    const int q4 = (lane >> 4) * 4;
 #pragma unroll
     for (int u = 0; u < 2; u++) {
-        const int n = n_base + u * 16 + row;
-        const float bvs = (flags & 1) ? bias[n] : 0.f;
+        const int n = n_base + u * 16 + q4;
+        float4 bv = {0.f, 0.f, 0.f, 0.f};
+        if ((flags & 1) && nz == 1) bv = *reinterpret_cast<const float4*>(bias + n);
 #pragma unroll
-        for (int i = 0; i < MT; i++)
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const int m = (wm * MT + i) * 16 + (lane >> 4) * 4 + j;
-                const int gh = th0 + m / TW, gw = tw0 + m % TW;
-                if (gh >= H || gw >= W) continue;
-                const int64_t o = (((int64_t)b * H + gh) * W + gw) * cout + n;
-                if (nz > 1) {   // split z's partial sum in its own slice; k_conv_finish adds them
-                    src.acc32[(int64_t)z * ((int64_t)nb * H * W * cout) + o] = acc[i][u][j];
-                    continue;
-                }
-                float v = acc[i][u][j] + bvs;
-                if (flags & 2) v = v > 0.f ? v : 0.f;
-                if (out) out[o] = (bf16)v;
-                if (out32) out32[o] = v;
+        for (int i = 0; i < MT; i++) {
+            const int m = (wm * MT + i) * 16 + row;
+            const int gh = th0 + m / TW, gw = tw0 + m % TW;
+            if (gh >= H || gw >= W) continue;
+            const int64_t o = (((int64_t)b * H + gh) * W + gw) * cout + n;
+            const f32x4 r = acc[i][u];
+            if (nz > 1) {   // split z's partial sum in its own slice; k_conv_finish adds them
+                *reinterpret_cast<float4*>(src.acc32 + (int64_t)z * ((int64_t)nb * H * W * cout) + o) =
+                    float4{r[0], r[1], r[2], r[3]};
+                continue;
             }
+            float v[4] = {r[0] + bv.x, r[1] + bv.y, r[2] + bv.z, r[3] + bv.w};
+            if (flags & 2)
+#pragma unroll
+                for (int j = 0; j < 4; j++) v[j] = v[j] > 0.f ? v[j] : 0.f;
+            if (out) {
+                typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+                *reinterpret_cast<bf16x4*>(out + o) =
+                    bf16x4{(bf16)v[0], (bf16)v[1], (bf16)v[2], (bf16)v[3]};
+            }
+            if (out32) *reinterpret_cast<float4*>(out32 + o) = float4{v[0], v[1], v[2], v[3]};
+        }
     }
 }
 
 // Completes a split-K convolution: the ks partial sums added in split order
-// (the same bits on every run), then bias, ReLU and BF16.
+// (the same bits on every run), then bias, ReLU and BF16; four consecutive
+// channels per thread (n and cout are multiples of 4).
 __global__ void __launch_bounds__(256)
 k_conv_finish(const float* __restrict__ acc32, int ks, const float* __restrict__ bias,
               bf16* __restrict__ out, float* __restrict__ out32, int64_t n, int cout, int flags,
               const int* __restrict__ live, int64_t per_image)
 {
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t k = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
     if (k >= n) return;
     if (live && k >= (int64_t)*live * per_image) return;   // images the splits skipped
-    float v = acc32[k];
-    for (int z = 1; z < ks; z++) v = v + acc32[(int64_t)z * n + k];
-    v = v + ((flags & 1) ? bias[k % cout] : 0.f);
-    if (flags & 2) v = v > 0.f ? v : 0.f;
-    if (out) out[k] = (bf16)v;
-    if (out32) out32[k] = v;
+    float4 v = *reinterpret_cast<const float4*>(acc32 + k);
+    for (int z = 1; z < ks; z++) {
+        const float4 a = *reinterpret_cast<const float4*>(acc32 + (int64_t)z * n + k);
+        v.x = v.x + a.x;
+        v.y = v.y + a.y;
+        v.z = v.z + a.z;
+        v.w = v.w + a.w;
+    }
+    if (flags & 1) {
+        const float4 bv = *reinterpret_cast<const float4*>(bias + (int)(k % cout));
+        v.x = v.x + bv.x;
+        v.y = v.y + bv.y;
+        v.z = v.z + bv.z;
+        v.w = v.w + bv.w;
+    }
+    if (flags & 2) {
+        v.x = v.x > 0.f ? v.x : 0.f;
+        v.y = v.y > 0.f ? v.y : 0.f;
+        v.z = v.z > 0.f ? v.z : 0.f;
+        v.w = v.w > 0.f ? v.w : 0.f;
+    }
+    if (out) {
+        typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+        *reinterpret_cast<bf16x4*>(out + k) = bf16x4{(bf16)v.x, (bf16)v.y, (bf16)v.z, (bf16)v.w};
+    }
+    if (out32) *reinterpret_cast<float4*>(out32 + k) = v;
 }
 
 // w'[ci][8 - tap][co] = w[co][tap][ci]
@@ -612,7 +698,12 @@ __global__ void k_head(const bf16* __restrict__ x, int c, const float* __restric
     if (p >= npix) return;
     if (live && p >= (int64_t)*live * hw) return;
     float s = b[0];
-    for (int i = 0; i < c; i++) s += (float)x[p * c + i] * w[i];
+    const bf16* xp = x + p * c;              // c is a multiple of 8: 16-byte loads
+    for (int i = 0; i < c; i += 8) {
+        const bf16x8 q = load8(xp + i);
+#pragma unroll
+        for (int e = 0; e < 8; e++) s += (float)q[e] * w[i + e];
+    }
     y[p] = s;
 }
 
@@ -736,7 +827,7 @@ static void conv_tile(hipStream_t st, const ConvIn& src, const bf16* w, const fl
                2 * ks * nout <= src.acc_cap)
             ks *= 2;
     dim3 grid(tiles, cout / (32 * WN), ks);
-    const bool pool = src.mode == IN_POOL, pipe = nsl / ks >= 3;
+    const bool pool = src.mode == IN_POOL, pipe = nsl / ks >= 3 && nblocks * ks <= 512;
 #define STM_CONV_TILE(P, Q)                                                                    \
     k_conv3x3_tile<TW, TH, WN, P, Q><<<grid, 256, 0, st>>>(src, w, bias, out, out32, cout, nb, H, \
                                                            W, flags)
@@ -746,7 +837,7 @@ static void conv_tile(hipStream_t st, const ConvIn& src, const bf16* w, const fl
     else STM_CONV_TILE(false, false);
 #undef STM_CONV_TILE
     if (ks > 1)
-        k_conv_finish<<<nblk(nout), 256, 0, st>>>(src.acc32, ks, bias, out, out32, nout, cout,
+        k_conv_finish<<<nblk(nout / 4), 256, 0, st>>>(src.acc32, ks, bias, out, out32, nout, cout,
                                                   flags, src.live, (int64_t)H * W * cout);
 }
 

@@ -22,6 +22,12 @@ struct ConvIn {
     const int* live = nullptr;   // device count: images at or beyond it are skipped
     float* acc32 = nullptr;      // FP32 scratch for the partial sums of split K ...
     int64_t acc_cap = 0;         // ... holding this many floats (split s writes [s*n, s*n + n))
+    // the 1x1 head fused into a 32-channel convolution (no split K):
+    // head_y[pix] = head_b[0] + sum_c out[pix, c] * head_w[c], channels in
+    // order, over the bf16-rounded outputs (the operations of nn::head)
+    const float* head_w = nullptr;
+    const float* head_b = nullptr;
+    float* head_y = nullptr;
 };
 
 // 3x3 convolution, padding 1, NHWC bf16, fp32 accumulation on the matrix

@@ -351,6 +351,13 @@ k_conv3x3_tile(ConvIn src, const bf16* __restrict__ w, const float* __restrict__
     // n_base + 16 u + 4 (l / 16) + 0..3: one 8-byte store of bf16 (16 bytes
     // of fp32) per tile and channel half
     const int q4 = (lane >> 4) * 4;
+    // with the fused head (32 channels, WN == 1, no split) each wave also
+    // stages its bf16 outputs, one row of CS per pixel, where the patch was
+    constexpr int HROWS = MT * 16;
+    static_assert(WN != 1 || 4 * HROWS <= PH * PW, "head staging fits the patch buffer");
+    const bool head = WN == 1 && src.head_y != nullptr;      // uniform
+    bf16* hs = &patch[0][0] + wave * HROWS * CS;
+    if (head) __syncthreads();       // every wave's MFMAs are done with the patch
 #pragma unroll
     for (int u = 0; u < 2; u++) {
         const int n = n_base + u * 16 + q4;
@@ -372,12 +379,28 @@ k_conv3x3_tile(ConvIn src, const bf16* __restrict__ w, const float* __restrict__
             if (flags & 2)
 #pragma unroll
                 for (int j = 0; j < 4; j++) v[j] = v[j] > 0.f ? v[j] : 0.f;
-            if (out) {
-                typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
-                *reinterpret_cast<bf16x4*>(out + o) =
-                    bf16x4{(bf16)v[0], (bf16)v[1], (bf16)v[2], (bf16)v[3]};
-            }
+            typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+            const bf16x4 vb = {(bf16)v[0], (bf16)v[1], (bf16)v[2], (bf16)v[3]};
+            if (out) *reinterpret_cast<bf16x4*>(out + o) = vb;
             if (out32) *reinterpret_cast<float4*>(out32 + o) = float4{v[0], v[1], v[2], v[3]};
+            if (head) *reinterpret_cast<bf16x4*>(hs + (i * 16 + row) * CS + u * 16 + q4) = vb;
+        }
+    }
+    if (head) {
+        __syncthreads();
+        if (lane < HROWS) {
+            const int m = wm * HROWS + lane;
+            const int gh = th0 + m / TW, gw = tw0 + m % TW;
+            if (gh < H && gw < W) {
+                float sum = src.head_b[0];
+#pragma unroll
+                for (int c = 0; c < 32; c += 8) {
+                    const bf16x8 x = *reinterpret_cast<const bf16x8*>(hs + lane * CS + c);
+#pragma unroll
+                    for (int e = 0; e < 8; e++) sum += (float)x[e] * src.head_w[c + e];
+                }
+                src.head_y[((int64_t)b * H + gh) * W + gw] = sum;
+            }
         }
     }
 }
@@ -822,7 +845,7 @@ static void conv_tile(hipStream_t st, const ConvIn& src, const bf16* w, const fl
     const int nsl = (src.ca + src.cb) / 32;
     const int64_t nout = (int64_t)nb * H * W * cout;
     int ks = 1;
-    if (src.acc32 && nblocks < 256)
+    if (src.acc32 && nblocks < 256 && !src.head_y)
         while (ks < 4 && nblocks * ks < 1024 && nsl / (2 * ks) >= 2 &&
                2 * ks * nout <= src.acc_cap)
             ks *= 2;

@@ -255,9 +255,17 @@ void Net::forward_infer(int nb, const int* live) {
     joined(10, 14, 5, 15);
     plain(11);
     joined(12, 17, 2, 18);
-    plain(13);
-    nn::head(stream, acts[20].v, 32, p32 + head_w, p32 + head_b, y, npix(0), live,
-             (int64_t)h(0) * w(0));
+    if (conv_tiles_fit(h(0), w(0))) {   // the head in the last convolution's epilogue
+        ConvIn in{acts[convs[13].x].v, convs[13].cin, nn::IN_DIRECT, nullptr, 0};
+        in.head_w = p32 + head_w;
+        in.head_b = p32 + head_b;
+        in.head_y = y;
+        run(13, in);
+    } else {
+        plain(13);
+        nn::head(stream, acts[20].v, 32, p32 + head_w, p32 + head_b, y, npix(0), live,
+                 (int64_t)h(0) * w(0));
+    }
 }
 
 void Net::backward(int nb) {

@@ -10,6 +10,7 @@ RHS, pressure solve to the residual, velocity update, next step size).
     python bench.py --profile        # fixed sweeps + one 256 MiB copy, for rocprofv3
     python bench.py --workload channels           # 256 uneven channels, plain
     python bench.py --workload learned_cavities   # 32 cavities 256x256: plain vs learned
+    python bench.py --workload learned_cavities --no-graphs   # cycles launched, not replayed
     python bench.py --workload learned_channels   # 64 channels 512x256: plain vs learned
     python bench.py --workload learned_channels_500   # the same, 500 learned steps from rest
     python bench.py --workload learned_uneven     # the 256 uneven channels: plain vs learned
@@ -86,6 +87,9 @@ def parse():
                          "independent channels of uneven size, seed 0; learned_*: plain vs "
                          "learned solve")
     ap.add_argument("--channels", type=int, default=256)
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="learned_cavities: the learned ticks launch every cycle instead of "
+                         "replaying hipGraphs (for kernel traces)")
     ap.add_argument("--no-channels", action="store_true",
                     help="leave the 256-channel summary out of the default line")
     ap.add_argument("--no-learned", action="store_true",
@@ -390,7 +394,7 @@ def train_cavity_net(log=None):
                              log=log or (lambda *x: None))
 
 
-def learned_cavities(ticks=10, log=None, trained=None):
+def learned_cavities(ticks=10, log=None, trained=None, graphs=True):
     """32 random-obstacle lid cavities of 256 x 256 cells, Re 50-1000, eps
     1e-6: network trained on 64 seed-0 layouts, compared on seed-100 layouts."""
     from stroemung_amd import learned, workloads
@@ -398,7 +402,7 @@ def learned_cavities(ticks=10, log=None, trained=None):
     params, info = trained or train_cavity_net(log)
     test = workloads.random_cavities(32, n=256, seed=100, eps=1e-6)
     res = learned.compare(test, params, coarsen=4, cycles=32, sweeps=16, warm=20, ticks=ticks,
-                          step="momentum", log=log)
+                          step="momentum", graphs=graphs, log=log)
     res["training"] = info
     res["workload"] = ("32 lid cavities 256x256 with 1-4 random obstacles, Re 50-1000, "
                        "red-black SOR omega=1.8 to relative residual 1e-6, tau=0.5, fp64; "
@@ -645,7 +649,7 @@ def net_line(a):
 def learned_line(a):
     log = lambda *x: print(*x, file=sys.stderr, flush=True)  # noqa: E731
     if a.workload == "learned_cavities":
-        res = learned_cavities(a.steps, log)
+        res = learned_cavities(a.steps, log, graphs=not a.no_graphs)
     elif a.workload == "learned_uneven":
         res = learned_uneven(a.steps, log)
     else:

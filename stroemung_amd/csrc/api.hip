@@ -84,6 +84,8 @@ struct stm_ctx {
     double* d_tbpart = nullptr;    // blocked solver: residual partials, two launches
     TbTile* d_tb = nullptr;
     int ntb = 0, tb_capacity = 0, part_capacity = 0;
+    uint32_t* d_codes = nullptr;   // blocked solver: per-thread cell codes of every tile
+    size_t codes_capacity = 0;     // bytes
     std::vector<TbTile> h_tb;      // host copy of the full list
     // the list launches use: d_tb, or during a polled solve the tiles of the
     // channels still running (d_tbc, staged through pinned h_tbc)
@@ -427,6 +429,14 @@ int plan_tb(stm_ctx* ctx, int variant) {
     }
     HIPCHK(ctx, hipMemcpyAsync(ctx->d_tb, tl.data(), sizeof(TbTile) * tl.size(),
                                hipMemcpyHostToDevice, ctx->stream));
+    // the codes table (graphs holding the old one were dropped above)
+    const size_t cbytes = v.codes ? (size_t)tl.size() * v.nt * TB_CODE_BYTES : 0;
+    if (cbytes > ctx->codes_capacity) {
+        if (ctx->d_codes) HIPCHK(ctx, hipFree(ctx->d_codes));
+        ctx->d_codes = nullptr;
+        HIPCHK(ctx, hipMalloc(&ctx->d_codes, cbytes));
+        ctx->codes_capacity = cbytes;
+    }
     ctx->ntb = (int)tl.size();
     ctx->h_tb = tl;
     ctx->tb_cur = ctx->d_tb;
@@ -670,13 +680,17 @@ int compact_tiles(stm_ctx* ctx, const int32_t* active) {
     return STM_OK;
 }
 
-// Launch L of the blocked solver on the current variant.
+// Launch L of the blocked solver on the current variant.  Every sequence
+// of launches starts at L = 0, which computes the cell codes of every
+// running tile and stores them; launches L > 0 of the sequence load them
+// (variants whose codes fit the table).
 void launch_blocked(stm_ctx* ctx, int variant, int L, int mode) {
     const int cap = ctx->part_capacity;
     double* prev = ctx->d_tbpart + (size_t)((L + 1) & 1) * cap;
     double* cur = ctx->d_tbpart + (size_t)(L & 1) * cap;
+    if (TB_VARIANTS[variant].codes) mode |= L == 0 ? TB_CODES_PUT : TB_CODES_GET;
     launch_rb_tb(ctx->stream, variant, ctx->ntb_cur, ctx->d_cd, ctx->d_cs, ctx->tb_cur, ctx->flags,
-                 ctx->rhs, ctx->p, ctx->p2, prev, cur, ctx->d_active, L, mode);
+                 ctx->rhs, ctx->p, ctx->p2, prev, cur, ctx->d_active, ctx->d_codes, L, mode);
 }
 
 // Blocked red-black launches until no channel is running.  Launch L also
@@ -1192,7 +1206,7 @@ void stm_destroy(stm_ctx* ctx) {
     for (hipEvent_t e : ctx->free_events) hipEventDestroy(e);
     void* bufs[] = {ctx->u, ctx->v, ctx->p, ctx->f, ctx->g, ctx->rhs, ctx->scratch, ctx->p2,
                     ctx->flags, ctx->d_cd, ctx->d_cs, ctx->d_be, ctx->d_restore, ctx->d_tiles,
-                    ctx->d_part, ctx->d_active, ctx->d_tbpart, ctx->d_tb, ctx->d_tbc, ctx->d_lc, ctx->lres,
+                    ctx->d_part, ctx->d_active, ctx->d_tbpart, ctx->d_tb, ctx->d_codes, ctx->d_tbc, ctx->d_lc, ctx->lres,
                     ctx->lcf, ctx->lprev, ctx->lprev2, ctx->lsnap, ctx->lrc, ctx->lmc, ctx->d_cpart, ctx->d_map,
                     ctx->d_inv, ctx->d_live, ctx->d_rgba};
     for (void* b : bufs)

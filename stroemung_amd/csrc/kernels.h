@@ -52,16 +52,23 @@ struct TbVariant {
     int k;          // sweeps per launch
     int oxm, oym;   // largest output tile away from a channel's edges
     int ex, ey;     // loaded region; the halo is (ex - oxm) / 2 = 2k + 4
+    int nt;         // threads per workgroup
+    int codes;      // 1: keeps its per-thread cell codes in the codes table
 };
 extern const TbVariant TB_VARIANTS[];
 extern const int N_TB_VARIANTS;
-// mode bit TB_CHECK: form the residual after the sweeps and stop on it.
-enum : int { TB_CHECK = 1 };
+// mode bits: TB_CHECK, form the residual after the sweeps and stop on it;
+// TB_CODES_PUT / TB_CODES_GET, compute the cell codes from the flags and
+// store them in the codes table / load them from it (variants with codes).
+enum : int { TB_CHECK = 1, TB_CODES_PUT = 2, TB_CODES_GET = 4 };
+// codes table: 32 bytes per thread of every tile, at (tile idx * nt + thread)
+constexpr int TB_CODE_BYTES = 32;
 // Launch L of a blocked solve: reads the partials of launch L - 1 from
 // part_prev, writes its own to part_cur (one double per tile).
 void launch_rb_tb(hipStream_t, int variant, int ntb, const ChanDesc*, ChanState*, const TbTile*,
                   const uint8_t* flags, const double* rhs, double* pa, double* pb,
-                  const double* part_prev, double* part_cur, int32_t* active, int L, int mode);
+                  const double* part_prev, double* part_cur, int32_t* active,
+                  uint32_t* codes, int L, int mode);
 void launch_pbuf_fix(hipStream_t, int ntiles, int nchan, const ChanDesc*, ChanState*,
                      const Tile*, const double* pb, double* pa);
 

@@ -328,10 +328,10 @@ struct Blk {
         return acc;
     }
 
-    // The loaded cells and their flags (FL, two per (row, pair)) into LDS;
-    // returns W_GHOST / W_DIAG: the wave holds a ghost / a corner ghost
-    // (wave-uniform).  Ends with a barrier.
-    template <int NF>
+    // The loaded cells and (FLAGS) their flags (FL, two per (row, pair)) into
+    // LDS; with FLAGS returns W_GHOST / W_DIAG: the wave holds a ghost / a
+    // corner ghost (wave-uniform), else 0.  Ends with a barrier.
+    template <bool FLAGS, int NF>
     __device__ static __forceinline__ int init(double (*P)[PS], uint16_t* FL,
                                                const double (&pv)[R][2],
                                                const uint32_t (&fo)[NF], const Codes<R>& cc,
@@ -341,12 +341,15 @@ struct Blk {
         for (int r = 0; r < R; r++) {
 #pragma unroll
             for (int q = 0; q < 2; q++) P[col(r, q)][OFF + (lx0 + r) * HY + m] = pv[r][q];
-            FL[(lx0 + r) * HY + m] = (uint16_t)(fo[r >> 1] >> (16 * (r & 1)));
+            if (FLAGS) FL[(lx0 + r) * HY + m] = (uint16_t)(fo[r >> 1] >> (16 * (r & 1)));
         }
+        int w = 0;
+        if (FLAGS) {
 #pragma unroll
-        for (int k = 0; k < Codes<R>::NG; k++) any |= cc.g[k] != 0;
-        int w = __ballot(any) != 0 ? W_GHOST : 0;
-        w |= __ballot(cc.diag != 0) != 0 ? W_DIAG : 0;
+            for (int k = 0; k < Codes<R>::NG; k++) any |= cc.g[k] != 0;
+            w = __ballot(any) != 0 ? W_GHOST : 0;
+            w |= __ballot(cc.diag != 0) != 0 ? W_DIAG : 0;
+        }
         __syncthreads();
         return w;
     }
@@ -457,7 +460,7 @@ k_rb_strip(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
            const TbTile* __restrict__ tiles, const uint8_t* __restrict__ flags,
            const double* __restrict__ rhs, double* __restrict__ pa, double* __restrict__ pb,
            const double* __restrict__ part_prev, double* __restrict__ part_cur,
-           int32_t* __restrict__ active_out, int L, int mode)
+           int32_t* __restrict__ active_out, uint32_t* __restrict__ codes, int L, int mode)
 {
     constexpr int NT = NS * HY;
     constexpr int H = 2 * K + 4;
@@ -509,6 +512,15 @@ k_rb_strip(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
     const bool y0in = gy >= 0 && gy < ny, y1in = gy >= 0 && gy + 1 < ny;
     double2 pp[R], rh[R];
     uint32_t ff[R];
+    // a thread's codes (CACHE: 2R <= 16 cells, two 16-byte words), loaded
+    // unconditionally with the region and used when mode has TB_CODES_GET
+    constexpr bool CACHE = 2 * R <= 16;
+    uint4 k0 = {}, k1 = {};
+    if constexpr (CACHE) {
+        const uint4* kp = reinterpret_cast<const uint4*>(codes) + ((int64_t)t.idx * NT + tid) * 2;
+        k0 = kp[0];
+        k1 = kp[1];
+    }
 #pragma unroll
     for (int r = 0; r < R; r++) {
         const int64_t row = base + (int64_t)min(max(ex0 + lx0 + r, 0), nx - 1) * pitch;
@@ -553,118 +565,147 @@ k_rb_strip(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
         const double w = wave_sum(pacc);
         if ((tid & 63) == 0) wsum[tid >> 6] = w;
     }
-    // ---- the codes of the own cells, four cells at a time (the flag bytes
-    // of word k of fo are cells 4k .. 4k + 3; byte masks, then cell bits):
-    // updated = an interior fluid cell inside the region's border; a ghost =
-    // a non-fluid cell inside the border (its edge kind kept); the output
-    // tile; 0xff (outside the channel) is none of them
-    constexpr int NW = (2 * R + 3) / 4;
-    static_assert(2 * NW == R, "two rows per word");
-    auto rows = [](bool a, bool b) -> uint32_t {   // byte mask of rows 2k, 2k + 1
-        return (a ? 0x0000ffffu : 0u) | (b ? 0xffff0000u : 0u);
-    };
-    auto slots = [](bool a, bool b) -> uint32_t {  // byte mask of slots q = 0, 1
-        return (a ? 0x00ff00ffu : 0u) | (b ? 0xff00ff00u : 0u);
-    };
-    const uint32_t yin = slots(2 * m >= 1 && 2 * m <= EY - 2, 2 * m + 1 <= EY - 2);
-    const uint32_t yupd = slots(gy >= 1 && gy <= ny - 2, gy + 1 >= 1 && gy + 1 <= ny - 2);
-    const uint32_t yout = slots(gy >= t.y0 && gy < t.y0 + t.yn,
-                                gy + 1 >= t.y0 && gy + 1 < t.y0 + t.yn);
-    auto xin = [&](int r) { const int lx = lx0 + r; return lx >= 1 && lx <= EX - 2; };
-    auto xupd = [&](int r) { const int gx = ex0 + lx0 + r; return gx >= 1 && gx <= nx - 2; };
-    auto xout = [&](int r) { const int gx = ex0 + lx0 + r; return gx >= t.x0 && gx < t.x0 + t.xn; };
+    // ---- the cell codes: computed from the flags (and, with TB_CODES_PUT,
+    // stored for the later launches of the sequence, which load them with
+    // TB_CODES_GET: the flags and the tiles do not change inside a sequence
+    // of launches, and a tile whose channel has stopped stays stopped)
     Codes<R> cc;
-    cc.upd = cc.out = cc.diag = 0;
-    cc.vsub = cc.hsub = 0;
-    uint32_t keep = 0;   // fluid cells inside the border that are not updated
-    uint32_t ghm = 0;    // ghost cells
-    uint32_t upd7[NW];   // bit 7 of each updated cell's byte
-#pragma unroll
-    for (int k = 0; k < Codes<R>::NG; k++) cc.g[k] = 0;
-#pragma unroll
-    for (int k = 0; k < NW; k++) {
-        const uint32_t w = fo[k];
-        const uint32_t valid = ones(zero7(~w) ^ 0x80808080u);
-        const uint32_t fluid = ones((w << 3) & 0x80808080u);
-        const uint32_t rin = rows(xin(2 * k), xin(2 * k + 1)) & yin & valid;
-        const uint32_t ru = rows(xupd(2 * k), xupd(2 * k + 1)) & yupd;
-        const uint32_t u = rin & fluid & ru;
-        const uint32_t gh = rin & ~fluid;
-        const uint32_t o = valid & rows(xout(2 * k), xout(2 * k + 1)) & yout;
-        const uint32_t e4 = w & 0x0f0f0f0fu & gh;                       // ghosts' edge kinds
-        const uint32_t corner = (zero7(e4) ^ 0x80808080u) & ~(e4 << 7);   // kind 2, 4, 6 or 8
-        upd7[k] = u & 0x80808080u;
-        cc.upd |= nib7(u) << (4 * k);
-        cc.out |= nib7(o) << (4 * k);
-        cc.diag |= nib7(corner) << (4 * k);
-        keep |= nib7(rin & fluid & ~ru) << (4 * k);
-        ghm |= nib7(zero7(e4) ^ 0x80808080u) << (4 * k);
-        const uint32_t y = e4 | (e4 >> 4);
-        cc.g[k >> 1] |= ((y & 0xffu) | ((y >> 8) & 0xff00u)) << (16 * (k & 1));
-    }
-
-    int wm = (ex0 & 1) ? Blk<K, R, HY, NS, 1>::init(P, FL, pv, fo, cc, lx0, m)
-                       : Blk<K, R, HY, NS, 0>::init(P, FL, pv, fo, cc, lx0, m);
+    int wm;
     uint32_t vown = 0, cpv = 0;
     typename Codes<R>::Sub cph = 0;
-
-    // ---- the updated cells' neighbours that are straight ghosts of them,
-    // from the flags every thread has put in LDS (N / S across the pair
-    // boundary, E / W across the strip ends; the rest are the thread's own),
-    // four cells at a time: each direction's neighbour flags laid out as the
-    // cells' own (byte j of word k), matched against the edge kind that
-    // makes this cell their fluid neighbour
-    {
-        auto fls = [&](int lx, int mm) -> uint32_t {   // a pair's flags, 0xffff outside the region
-            const bool in = lx >= 0 && lx < EX && mm >= 0 && mm < HY;
-            const int k = min(max(lx, 0), EX - 1) * HY + min(max(mm, 0), HY - 1);
-            const uint32_t f = FL[k];
-            return in ? f : 0xffffu;
+    if (CACHE && (mode & TB_CODES_GET)) {
+        cc.upd = k0.x & 0xffffu;
+        cc.out = k0.x >> 16;
+        cc.diag = k0.y & 0xffffu;
+        cpv = k0.y >> 16;
+        cc.vsub = k0.z;
+        cc.hsub = k0.w;
+        cph = k1.x;
+        cc.g[0] = k1.y;
+        if constexpr (Codes<R>::NG > 1) cc.g[Codes<R>::NG > 1 ? 1 : 0] = k1.z;
+        vown = k1.w & 0xffu;
+        wm = (int)(k1.w >> 8);
+        if (ex0 & 1) Blk<K, R, HY, NS, 1>::template init<false>(P, FL, pv, fo, cc, lx0, m);
+        else Blk<K, R, HY, NS, 0>::template init<false>(P, FL, pv, fo, cc, lx0, m);
+    } else {
+        // ---- the codes of the own cells, four cells at a time (the flag bytes
+        // of word k of fo are cells 4k .. 4k + 3; byte masks, then cell bits):
+        // updated = an interior fluid cell inside the region's border; a ghost =
+        // a non-fluid cell inside the border (its edge kind kept); the output
+        // tile; 0xff (outside the channel) is none of them
+        constexpr int NW = (2 * R + 3) / 4;
+        static_assert(2 * NW == R, "two rows per word");
+        auto rows = [](bool a, bool b) -> uint32_t {   // byte mask of rows 2k, 2k + 1
+            return (a ? 0x0000ffffu : 0u) | (b ? 0xffff0000u : 0u);
         };
-        uint32_t fN[R], fS[R];
-#pragma unroll
-        for (int r = 0; r < R; r++) {
-            fN[r] = fls(lx0 + r, m - 1) >> 8;      // the S cell of the pair above
-            fS[r] = fls(lx0 + r, m + 1) & 0xffu;   // the N cell of the pair below
-        }
-        const uint32_t fE = fls(lx0 + R, m), fW = fls(lx0 - 1, m);
-        uint32_t nsub = 0, ssub = 0, esub = 0, wsub = 0;
-#pragma unroll
+        auto slots = [](bool a, bool b) -> uint32_t {  // byte mask of slots q = 0, 1
+            return (a ? 0x00ff00ffu : 0u) | (b ? 0xff00ff00u : 0u);
+        };
+        const uint32_t yin = slots(2 * m >= 1 && 2 * m <= EY - 2, 2 * m + 1 <= EY - 2);
+        const uint32_t yupd = slots(gy >= 1 && gy <= ny - 2, gy + 1 >= 1 && gy + 1 <= ny - 2);
+        const uint32_t yout = slots(gy >= t.y0 && gy < t.y0 + t.yn,
+                                    gy + 1 >= t.y0 && gy + 1 < t.y0 + t.yn);
+        auto xin = [&](int r) { const int lx = lx0 + r; return lx >= 1 && lx <= EX - 2; };
+        auto xupd = [&](int r) { const int gx = ex0 + lx0 + r; return gx >= 1 && gx <= nx - 2; };
+        auto xout = [&](int r) { const int gx = ex0 + lx0 + r; return gx >= t.x0 && gx < t.x0 + t.xn; };
+        cc.upd = cc.out = cc.diag = 0;
+        cc.vsub = cc.hsub = 0;
+        uint32_t keep = 0;   // fluid cells inside the border that are not updated
+        uint32_t ghm = 0;    // ghost cells
+        uint32_t upd7[NW];   // bit 7 of each updated cell's byte
+    #pragma unroll
+        for (int k = 0; k < Codes<R>::NG; k++) cc.g[k] = 0;
+    #pragma unroll
         for (int k = 0; k < NW; k++) {
             const uint32_t w = fo[k];
-            const uint32_t wn = ((w << 8) & 0xff00ff00u) | fN[2 * k] | (fN[2 * k + 1] << 16);
-            const uint32_t ws = ((w >> 8) & 0x00ff00ffu) | (fS[2 * k] << 8) | (fS[2 * k + 1] << 24);
-            const uint32_t we = (w >> 16) | ((k + 1 < NW ? fo[k + 1 < NW ? k + 1 : k] & 0xffffu : fE) << 16);
-            const uint32_t ww = (w << 16) | (k > 0 ? fo[k > 0 ? k - 1 : k] >> 16 : fW);
-            nsub |= nib7(straight7(wn, E_S) & upd7[k]) << (4 * k);
-            ssub |= nib7(straight7(ws, E_N) & upd7[k]) << (4 * k);
-            esub |= nib7(straight7(we, E_W) & upd7[k]) << (4 * k);
-            wsub |= nib7(straight7(ww, E_E) & upd7[k]) << (4 * k);
+            const uint32_t valid = ones(zero7(~w) ^ 0x80808080u);
+            const uint32_t fluid = ones((w << 3) & 0x80808080u);
+            const uint32_t rin = rows(xin(2 * k), xin(2 * k + 1)) & yin & valid;
+            const uint32_t ru = rows(xupd(2 * k), xupd(2 * k + 1)) & yupd;
+            const uint32_t u = rin & fluid & ru;
+            const uint32_t gh = rin & ~fluid;
+            const uint32_t o = valid & rows(xout(2 * k), xout(2 * k + 1)) & yout;
+            const uint32_t e4 = w & 0x0f0f0f0fu & gh;                       // ghosts' edge kinds
+            const uint32_t corner = (zero7(e4) ^ 0x80808080u) & ~(e4 << 7);   // kind 2, 4, 6 or 8
+            upd7[k] = u & 0x80808080u;
+            cc.upd |= nib7(u) << (4 * k);
+            cc.out |= nib7(o) << (4 * k);
+            cc.diag |= nib7(corner) << (4 * k);
+            keep |= nib7(rin & fluid & ~ru) << (4 * k);
+            ghm |= nib7(zero7(e4) ^ 0x80808080u) << (4 * k);
+            const uint32_t y = e4 | (e4 >> 4);
+            cc.g[k >> 1] |= ((y & 0xffu) | ((y >> 8) & 0xff00u)) << (16 * (k & 1));
         }
-        using Sub = typename Codes<R>::Sub;
-        cc.vsub = (Sub)nsub | (Sub)ssub << Codes<R>::SH;
-        cc.hsub = (Sub)esub | (Sub)wsub << Codes<R>::SH;
-        // the vertical substitutions from the own other cell (q = 1 reads N:
-        // own_n; q = 0 reads S: own_s) or across the pair boundary (cross);
-        // per lane when every updated cell of the slot has it (or none does)
-        constexpr uint32_t Q1 = 0xaaaaaaaau & ((2 * R < 32) ? (1u << 2 * R) - 1 : ~0u);
-        const uint32_t own_n = nsub & Q1, own_s = ssub & ~Q1;
-        const uint32_t cross = (nsub & ~Q1) | (ssub & Q1);
-        const uint32_t upd1 = cc.upd & Q1, upd0 = cc.upd & ~Q1;
-        const bool lane_ok = cross == 0 && (own_n == 0 || own_n == upd1) &&
-                             (own_s == 0 || own_s == upd0);
-        vown = (own_n != 0 ? 1u : 0u) | (own_s != 0 ? 2u : 0u);
-        // the ghosts whose one fluid neighbour is a cell of this thread (and
-        // updated): the other cell of the pair, the next or previous row
-        constexpr uint32_t CELLS = (2 * R < 32) ? (1u << 2 * R) - 1 : ~0u;
-        cpv = (own_n >> 1) | (own_s << 1);
-        const uint32_t ce = (wsub >> 2), cw = (esub << 2) & CELLS;
-        cph = (Sub)ce | (Sub)cw << Codes<R>::SH;
-        wm |= __ballot(ghm != (cpv | ce | cw)) == 0 ? W_OWN : 0;
-        const int vm = __ballot((nsub | ssub) != 0) == 0 ? 0 : (__ballot(!lane_ok) != 0 ? 2 : 1);
-        wm |= vm << W_VSHIFT;
-        wm |= __ballot((esub | wsub) != 0) != 0 ? W_HX : 0;
-        wm |= __ballot(keep != 0) != 0 ? W_SEL : 0;
+
+        wm = (ex0 & 1) ? Blk<K, R, HY, NS, 1>::template init<true>(P, FL, pv, fo, cc, lx0, m)
+                       : Blk<K, R, HY, NS, 0>::template init<true>(P, FL, pv, fo, cc, lx0, m);
+
+        // ---- the updated cells' neighbours that are straight ghosts of them,
+        // from the flags every thread has put in LDS (N / S across the pair
+        // boundary, E / W across the strip ends; the rest are the thread's own),
+        // four cells at a time: each direction's neighbour flags laid out as the
+        // cells' own (byte j of word k), matched against the edge kind that
+        // makes this cell their fluid neighbour
+        {
+            auto fls = [&](int lx, int mm) -> uint32_t {   // a pair's flags, 0xffff outside the region
+                const bool in = lx >= 0 && lx < EX && mm >= 0 && mm < HY;
+                const int k = min(max(lx, 0), EX - 1) * HY + min(max(mm, 0), HY - 1);
+                const uint32_t f = FL[k];
+                return in ? f : 0xffffu;
+            };
+            uint32_t fN[R], fS[R];
+    #pragma unroll
+            for (int r = 0; r < R; r++) {
+                fN[r] = fls(lx0 + r, m - 1) >> 8;      // the S cell of the pair above
+                fS[r] = fls(lx0 + r, m + 1) & 0xffu;   // the N cell of the pair below
+            }
+            const uint32_t fE = fls(lx0 + R, m), fW = fls(lx0 - 1, m);
+            uint32_t nsub = 0, ssub = 0, esub = 0, wsub = 0;
+    #pragma unroll
+            for (int k = 0; k < NW; k++) {
+                const uint32_t w = fo[k];
+                const uint32_t wn = ((w << 8) & 0xff00ff00u) | fN[2 * k] | (fN[2 * k + 1] << 16);
+                const uint32_t ws = ((w >> 8) & 0x00ff00ffu) | (fS[2 * k] << 8) | (fS[2 * k + 1] << 24);
+                const uint32_t we = (w >> 16) | ((k + 1 < NW ? fo[k + 1 < NW ? k + 1 : k] & 0xffffu : fE) << 16);
+                const uint32_t ww = (w << 16) | (k > 0 ? fo[k > 0 ? k - 1 : k] >> 16 : fW);
+                nsub |= nib7(straight7(wn, E_S) & upd7[k]) << (4 * k);
+                ssub |= nib7(straight7(ws, E_N) & upd7[k]) << (4 * k);
+                esub |= nib7(straight7(we, E_W) & upd7[k]) << (4 * k);
+                wsub |= nib7(straight7(ww, E_E) & upd7[k]) << (4 * k);
+            }
+            using Sub = typename Codes<R>::Sub;
+            cc.vsub = (Sub)nsub | (Sub)ssub << Codes<R>::SH;
+            cc.hsub = (Sub)esub | (Sub)wsub << Codes<R>::SH;
+            // the vertical substitutions from the own other cell (q = 1 reads N:
+            // own_n; q = 0 reads S: own_s) or across the pair boundary (cross);
+            // per lane when every updated cell of the slot has it (or none does)
+            constexpr uint32_t Q1 = 0xaaaaaaaau & ((2 * R < 32) ? (1u << 2 * R) - 1 : ~0u);
+            const uint32_t own_n = nsub & Q1, own_s = ssub & ~Q1;
+            const uint32_t cross = (nsub & ~Q1) | (ssub & Q1);
+            const uint32_t upd1 = cc.upd & Q1, upd0 = cc.upd & ~Q1;
+            const bool lane_ok = cross == 0 && (own_n == 0 || own_n == upd1) &&
+                                 (own_s == 0 || own_s == upd0);
+            vown = (own_n != 0 ? 1u : 0u) | (own_s != 0 ? 2u : 0u);
+            // the ghosts whose one fluid neighbour is a cell of this thread (and
+            // updated): the other cell of the pair, the next or previous row
+            constexpr uint32_t CELLS = (2 * R < 32) ? (1u << 2 * R) - 1 : ~0u;
+            cpv = (own_n >> 1) | (own_s << 1);
+            const uint32_t ce = (wsub >> 2), cw = (esub << 2) & CELLS;
+            cph = (Sub)ce | (Sub)cw << Codes<R>::SH;
+            wm |= __ballot(ghm != (cpv | ce | cw)) == 0 ? W_OWN : 0;
+            const int vm = __ballot((nsub | ssub) != 0) == 0 ? 0 : (__ballot(!lane_ok) != 0 ? 2 : 1);
+            wm |= vm << W_VSHIFT;
+            wm |= __ballot((esub | wsub) != 0) != 0 ? W_HX : 0;
+            wm |= __ballot(keep != 0) != 0 ? W_SEL : 0;
+        }
+        if (CACHE && (mode & TB_CODES_PUT)) {
+            uint4* kp = reinterpret_cast<uint4*>(codes) + ((int64_t)t.idx * NT + tid) * 2;
+            kp[0] = make_uint4(cc.upd | cc.out << 16, cc.diag | cpv << 16, (uint32_t)cc.vsub,
+                               (uint32_t)cc.hsub);
+            kp[1] = make_uint4((uint32_t)cph, cc.g[0],
+                               Codes<R>::NG > 1 ? cc.g[Codes<R>::NG > 1 ? 1 : 0] : 0u,
+                               vown | (uint32_t)wm << 8);
+        }
     }
 
     // the black corner ghosts for the first red half (before the same barrier)
@@ -775,7 +816,7 @@ __global__ void k_pbuf_reset(int nchan, ChanState* __restrict__ cs) {
     V(9, 6, 6, 48, 16)
 
 #define V_ENTRY(i, K, R, HY, NS) \
-    {K, NS * R - 2 * (2 * K + 4), 2 * HY - 2 * (2 * K + 4), NS * R, 2 * HY},
+    {K, NS * R - 2 * (2 * K + 4), 2 * HY - 2 * (2 * K + 4), NS * R, 2 * HY, NS * HY, 2 * R <= 16},
 const TbVariant TB_VARIANTS[] = {STM_TB_VARIANTS(V_ENTRY)};
 #undef V_ENTRY
 const int N_TB_VARIANTS = sizeof(TB_VARIANTS) / sizeof(TB_VARIANTS[0]);
@@ -783,12 +824,12 @@ const int N_TB_VARIANTS = sizeof(TB_VARIANTS) / sizeof(TB_VARIANTS[0]);
 void launch_rb_tb(hipStream_t st, int variant, int ntb, const ChanDesc* cd, ChanState* cs,
                   const TbTile* tiles, const uint8_t* flags, const double* rhs, double* pa,
                   double* pb, const double* part_prev, double* part_cur, int32_t* active,
-                  int L, int mode) {
+                  uint32_t* codes, int L, int mode) {
 #define V_CASE(i, K, R, HY, NS)                                                              \
     case i:                                                                                  \
         k_rb_strip<K, R, HY, NS><<<ntb, NS * HY, 0, st>>>(cd, cs, tiles, flags, rhs, pa, pb, \
-                                                          part_prev, part_cur, active, L,    \
-                                                          mode);                             \
+                                                          part_prev, part_cur, active,       \
+                                                          codes, L, mode);                   \
         break;
     switch (variant) {
         STM_TB_VARIANTS(V_CASE)

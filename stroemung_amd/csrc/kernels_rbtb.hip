@@ -748,15 +748,20 @@ k_rb_strip(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
                   : Blk<K, R, HY, NS, 0>::run(P, pv, rr, cc, vown, cpv, cph, lx0, m, nsw, check, wm, d);
 
     // ---- write the output tile.  Output cells that are neither updated nor
-    // ghosts keep their value in every sweep: they are taken from the input
-    // (the sweeps may have stored into them, see Blk::update).
+    // ghosts keep their value in every sweep of the sequence (the sweeps may
+    // have stored into their registers, see Blk::update): launch 0 copies
+    // them from its input buffer to its output buffer, after which both
+    // buffers hold them, and the later launches leave them out.
+    uint32_t wout = cc.out;
     {
         uint32_t rest = 0;
 #pragma unroll
         for (int i = 0; i < 2 * R; i++)
             rest |= (cc.edge(i) == 0 ? 1u : 0u) << i;
         rest &= cc.out & ~cc.upd;
-        if (__ballot(rest != 0) != 0) {
+        if (L > 0) {
+            wout &= ~rest;
+        } else if (__ballot(rest != 0) != 0) {
 #pragma unroll
             for (int r = 0; r < R; r++) {
                 const int64_t g = base + (int64_t)(ex0 + lx0 + r) * pitch + gy;
@@ -768,7 +773,7 @@ k_rb_strip(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
 #pragma unroll
     for (int r = 0; r < R; r++) {
         const int64_t g = base + (int64_t)(ex0 + lx0 + r) * pitch + gy;
-        const bool o0 = (cc.out >> (2 * r)) & 1u, o1 = (cc.out >> (2 * r + 1)) & 1u;
+        const bool o0 = (wout >> (2 * r)) & 1u, o1 = (wout >> (2 * r + 1)) & 1u;
         if (o0 && o1) *reinterpret_cast<double2*>(pout + g) = make_double2(pv[r][0], pv[r][1]);
         else if (o0) pout[g] = pv[r][0];
         else if (o1) pout[g + 1] = pv[r][1];

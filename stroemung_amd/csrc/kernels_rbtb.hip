@@ -512,7 +512,8 @@ k_rb_strip(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
     const bool y0in = gy >= 0 && gy < ny, y1in = gy >= 0 && gy + 1 < ny;
     double2 pp[R], rh[R];
     uint32_t ff[R];
-    // a thread's codes (CACHE: 2R <= 16 cells, two 16-byte words), loaded
+    // a thread's codes (CACHE: 2R <= 16 cells, two 16-byte words: upd | out,
+    // diag | cpv, vsub, hsub; cph, g[0], g[1], vown | wm | rest), loaded
     // unconditionally with the region and used when mode has TB_CODES_GET
     constexpr bool CACHE = 2 * R <= 16;
     uint4 k0 = {}, k1 = {};
@@ -521,12 +522,18 @@ k_rb_strip(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
         k0 = kp[0];
         k1 = kp[1];
     }
+    const bool get = CACHE && (mode & TB_CODES_GET);   // uniform: the codes come from the table
 #pragma unroll
     for (int r = 0; r < R; r++) {
         const int64_t row = base + (int64_t)min(max(ex0 + lx0 + r, 0), nx - 1) * pitch;
         pp[r] = *reinterpret_cast<const double2*>(pin + row + gyc);
         rh[r] = *reinterpret_cast<const double2*>(rhs + row + gyc);
-        ff[r] = *reinterpret_cast<const uint16_t*>(flags + row + gyc);
+    }
+    // the flags only feed the codes
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int64_t row = base + (int64_t)min(max(ex0 + lx0 + r, 0), nx - 1) * pitch;
+        ff[r] = get ? 0u : (uint32_t)*reinterpret_cast<const uint16_t*>(flags + row + gyc);
     }
     // every load is issued before any of them is used: the wave waits once
     // for the whole region, row by row, instead of once per batch the
@@ -573,7 +580,8 @@ k_rb_strip(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
     int wm;
     uint32_t vown = 0, cpv = 0;
     typename Codes<R>::Sub cph = 0;
-    if (CACHE && (mode & TB_CODES_GET)) {
+    uint32_t rest = 0;   // output cells that keep their value (see the write-back)
+    if (get) {
         cc.upd = k0.x & 0xffffu;
         cc.out = k0.x >> 16;
         cc.diag = k0.y & 0xffffu;
@@ -584,7 +592,8 @@ k_rb_strip(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
         cc.g[0] = k1.y;
         if constexpr (Codes<R>::NG > 1) cc.g[Codes<R>::NG > 1 ? 1 : 0] = k1.z;
         vown = k1.w & 0xffu;
-        wm = (int)(k1.w >> 8);
+        wm = (int)((k1.w >> 8) & 0xffu);
+        rest = k1.w >> 16;
         if (ex0 & 1) Blk<K, R, HY, NS, 1>::template init<false>(P, FL, pv, fo, cc, lx0, m);
         else Blk<K, R, HY, NS, 0>::template init<false>(P, FL, pv, fo, cc, lx0, m);
     } else {
@@ -698,13 +707,17 @@ k_rb_strip(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
             wm |= __ballot((esub | wsub) != 0) != 0 ? W_HX : 0;
             wm |= __ballot(keep != 0) != 0 ? W_SEL : 0;
         }
+#pragma unroll
+        for (int i = 0; i < 2 * R; i++)
+            rest |= (cc.edge(i) == 0 ? 1u : 0u) << i;
+        rest &= cc.out & ~cc.upd;
         if (CACHE && (mode & TB_CODES_PUT)) {
             uint4* kp = reinterpret_cast<uint4*>(codes) + ((int64_t)t.idx * NT + tid) * 2;
             kp[0] = make_uint4(cc.upd | cc.out << 16, cc.diag | cpv << 16, (uint32_t)cc.vsub,
                                (uint32_t)cc.hsub);
             kp[1] = make_uint4((uint32_t)cph, cc.g[0],
                                Codes<R>::NG > 1 ? cc.g[Codes<R>::NG > 1 ? 1 : 0] : 0u,
-                               vown | (uint32_t)wm << 8);
+                               vown | (uint32_t)wm << 8 | rest << 16);
         }
     }
 
@@ -754,11 +767,6 @@ k_rb_strip(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
     // buffers hold them, and the later launches leave them out.
     uint32_t wout = cc.out;
     {
-        uint32_t rest = 0;
-#pragma unroll
-        for (int i = 0; i < 2 * R; i++)
-            rest |= (cc.edge(i) == 0 ? 1u : 0u) << i;
-        rest &= cc.out & ~cc.upd;
         if (L > 0) {
             wout &= ~rest;
         } else if (__ballot(rest != 0) != 0) {

@@ -420,8 +420,8 @@ struct Blk {
                                                  Codes<R>& cc, uint32_t vown, uint32_t cpv,
                                                  typename Codes<R>::Sub cph, int lx0, int m,
                                                  int nsw, bool check, int wm,
-                                                 const ChanDesc& d) {
-        const double omw = d.omw, mid = d.mid, rdx2 = d.rdx2, rdy2 = d.rdy2;
+                                                 double omw, double mid, double rdx2,
+                                                 double rdy2) {
         const bool ghost = wm & W_GHOST;
         const int vm = (wm >> W_VSHIFT) & 3;
         const bool hx = wm & W_HX;
@@ -482,6 +482,13 @@ k_rb_strip(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
     // read here, tested only after the loads: no load waits for it
     const int stopped = s.stop_launch;
     const double mean = s.rhs_mean;
+    // the channel's constants for the stop decision and the sweeps, read
+    // with its state: each would otherwise be a memory round trip after a
+    // barrier (the compiler does not move a load above the stop test)
+    const ChanDesc& d = cd[c];
+    const int budget = s.budget, d_itermax = d.itermax;
+    const double tol2 = s.tol2;
+    const double omw = d.omw, mid = d.mid, rdx2 = d.rdx2, rdy2 = d.rdy2;
     const int tid = threadIdx.x;
     const bool check = mode & TB_CHECK;
     const double* __restrict__ pin = (L & 1) ? pb : pa;
@@ -561,7 +568,6 @@ k_rb_strip(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
     // a channel that has stopped: nothing to do (uniform; the loads above
     // are in bounds and their results unused)
     if (stopped >= 0) return;
-    const ChanDesc& d = cd[c];
 
     // ---- every thread adds its share of the partials in a fixed order,
     // waves add theirs, and after the barrier below every thread adds the wave sums
@@ -727,14 +733,14 @@ k_rb_strip(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
     __syncthreads();
 
     // ---- stop decision on the previous launch (identical in every tile)
-    const int itermax = (s.budget > 0 && s.budget < d.itermax) ? s.budget : d.itermax;
+    const int itermax = (budget > 0 && budget < d_itermax) ? budget : d_itermax;
     const int64_t sweeps = (int64_t)L * K;
     const int done = sweeps < itermax ? (int)sweeps : itermax;
     if (L > 0) {
         double res2 = 0.0;
 #pragma unroll
         for (int w = 0; w < NT / 64; w++) res2 = res2 + wsum[w];
-        const bool conv = check && res2 <= s.tol2;
+        const bool conv = check && res2 <= tol2;
         // a residual that is not a number never meets the test: stop there
         // (the tick then reports it) instead of sweeping to itermax
         const bool nan = check && res2 != res2;
@@ -757,8 +763,10 @@ k_rb_strip(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
     const int nsw = itermax - done < K ? itermax - done : K;
 
     const double acc =
-        (ex0 & 1) ? Blk<K, R, HY, NS, 1>::run(P, pv, rr, cc, vown, cpv, cph, lx0, m, nsw, check, wm, d)
-                  : Blk<K, R, HY, NS, 0>::run(P, pv, rr, cc, vown, cpv, cph, lx0, m, nsw, check, wm, d);
+        (ex0 & 1) ? Blk<K, R, HY, NS, 1>::run(P, pv, rr, cc, vown, cpv, cph, lx0, m, nsw, check, wm,
+                                              omw, mid, rdx2, rdy2)
+                  : Blk<K, R, HY, NS, 0>::run(P, pv, rr, cc, vown, cpv, cph, lx0, m, nsw, check, wm,
+                                              omw, mid, rdx2, rdy2);
 
     // ---- write the output tile.  Output cells that are neither updated nor
     // ghosts keep their value in every sweep of the sequence (the sweeps may

@@ -569,6 +569,7 @@ __global__ void k_chunk_sum(const float* __restrict__ part, int chunks, int64_t 
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float s = part[i];
+#pragma unroll 8
     for (int c = 1; c < chunks; c++) s += part[(int64_t)c * n + i];
     dst[i] += s;
 }
@@ -602,23 +603,32 @@ __global__ void k_relu_bwd(int64_t n, const bf16* __restrict__ y, bf16* __restri
     if (k < n && !((float)y[k] > 0.f)) g[k] = (bf16)0.f;
 }
 
-// part[block][c] = sum over the block's pixels of g[pix][c]
-// c divides 256: thread t sums channel t % c over pixels t / c, t / c + 256 / c,
-// ... of the block's chunk (consecutive threads read consecutive channels),
-// then the 256 / c partials of each channel are added in LDS.
+// part[block][c] = sum over the block's pixels of g[pix][c].  c is 32, 64,
+// 128 or 256: thread t takes the 8 channels of vector v = t % (c / 8) (one
+// 16-byte load per pixel) for pixels j, j + lanes, ... of the block's chunk
+// (j = t / (c / 8), lanes = 256 / (c / 8)), each channel added in pixel
+// order; then the lanes' partials of each channel are added in LDS in lane
+// order.  The loads of several pixels are in flight at once.
 __global__ void __launch_bounds__(256)
 k_bias_grad(const bf16* __restrict__ g, int c, int npix, int per, float* __restrict__ part)
 {
-    __shared__ float red[256];
-    const int ch = threadIdx.x % c, lanes = 256 / c;
+    __shared__ float red[256 * 8];
+    const int nv = c / 8, v = threadIdx.x % nv, j = threadIdx.x / nv, lanes = 256 / nv;
     const int p0 = blockIdx.x * per, p1 = min(npix, p0 + per);
-    float s = 0.f;
-    for (int p = p0 + threadIdx.x / c; p < p1; p += lanes) s += (float)g[(int64_t)p * c + ch];
-    red[threadIdx.x] = s;
+    float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll 4
+    for (int p = p0 + j; p < p1; p += lanes) {
+        const bf16x8 x = load8(g + (int64_t)p * c + v * 8);
+#pragma unroll
+        for (int e = 0; e < 8; e++) s[e] += (float)x[e];
+    }
+#pragma unroll
+    for (int e = 0; e < 8; e++) red[threadIdx.x * 8 + e] = s[e];
     __syncthreads();
     if (threadIdx.x < c) {
+        const int cv = threadIdx.x / 8, ce = threadIdx.x % 8;
         float t = 0.f;
-        for (int k = threadIdx.x; k < 256; k += c) t += red[k];
+        for (int k = 0; k < lanes; k++) t += red[(k * nv + cv) * 8 + ce];
         part[(int64_t)blockIdx.x * c + threadIdx.x] = t;
     }
 }

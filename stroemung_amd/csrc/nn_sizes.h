@@ -14,11 +14,12 @@ namespace nn {
 // 256-thread blocks over n items
 inline int64_t blocks256(int64_t n) { return (n + 255) / 256; }
 
-// weight gradient: pixel chunks of 4096 << k pixels, at most 64 full chunks
-// until a chunk reaches 1 << 20 pixels
+// weight gradient: pixel chunks of 1024 << k pixels, at most 256 full
+// chunks until a chunk reaches 1 << 20 pixels (several workgroups per CU on
+// every layer of the small frames, so their load latencies overlap)
 inline int dw_pixels_per_chunk(int64_t npix) {
-    int per = 4096;
-    while (npix / per > 64 && per < (1 << 20)) per *= 2;
+    int per = 1024;
+    while (npix / per > 256 && per < (1 << 20)) per *= 2;
     return per;
 }
 inline int64_t dw_chunks(int64_t npix) {
@@ -38,11 +39,11 @@ inline int64_t bias_blocks(int64_t npix) {
 }
 
 // Partials (floats) of conv3x3_dw and bias_grad for any pixel count up to
-// npix.  The chunk counts are not monotone in the pixel count: at most 65
+// npix.  The chunk counts are not monotone in the pixel count: at most 257
 // chunks until the chunk size stops growing, at most 257 bias blocks.
 inline int64_t grad_partials(int cin, int cout, int64_t npix) {
     const int64_t chunks =
-        std::min((npix + 4095) / 4096, std::max<int64_t>(65, (npix + (1 << 20) - 1) >> 20));
+        std::min((npix + 1023) / 1024, std::max<int64_t>(257, (npix + (1 << 20) - 1) >> 20));
     const int64_t dw = chunks > 1 ? chunks * cout * 9 * cin : 0;
     const int64_t db = std::min<int64_t>((npix + 127) / 128, 257) * cout;
     return std::max(dw, db);

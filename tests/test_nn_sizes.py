@@ -45,7 +45,7 @@ static int fits(int64_t cap, int64_t npix) {
 
 int main(int argc, char** argv) {
     const int64_t dense = atoll(argv[1]);
-    const int64_t caps[] = {64, 4095, 4096, 4097, 266239, 266240, 524288, 1 << 20,
+    const int64_t caps[] = {64, 1023, 1024, 1025, 4096, 262145, 266239, 266240, 524288, 1 << 20,
                             3 << 20, 64LL << 20, (68LL << 20) + 5, 200LL << 20};
     std::mt19937_64 rng(7);
     for (int64_t cap : caps) {
@@ -90,11 +90,11 @@ def test_partials_of_every_smaller_batch_fit_the_bound(tmp_path):
     assert r.returncode == 0 and r.stdout.strip().endswith("ok"), r.stdout + r.stderr
 
 
-def test_a_bound_of_64_chunks_is_caught(tmp_path):
-    """The check is not vacuous: 64 instead of 65 chunks (the count reaches
-    65 just below 65 chunk sizes) fails it."""
+def test_a_bound_of_256_chunks_is_caught(tmp_path):
+    """The check is not vacuous: 256 instead of 257 chunks (the count reaches
+    257 just above 256 chunk sizes) fails it."""
     text = open(os.path.join(CSRC, "nn_sizes.h")).read()
-    bad = text.replace("std::max<int64_t>(65,", "std::max<int64_t>(64,")
+    bad = text.replace("std::max<int64_t>(257,", "std::max<int64_t>(256,")
     assert bad != text
     (tmp_path / "nn_sizes.h").write_text(bad)
     exe = _build(tmp_path, tmp_path)

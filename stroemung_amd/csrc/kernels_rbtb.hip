@@ -75,10 +75,12 @@ __device__ __forceinline__ uint32_t ghost_bits(uint32_t edge) {
 
 // The forms a wave runs (wave-uniform bits): it holds a ghost cell, a corner
 // ghost, a fluid cell that is not updated (SEL); its vertical substitutions
-// (2 bits at W_VSHIFT: none, per lane, per cell); per-cell E / W ones (HX);
-// every ghost of the wave is straight and its fluid neighbour is a cell of
-// the same thread (OWN).
-enum : int { W_GHOST = 1, W_DIAG = 2, W_SEL = 4, W_VSHIFT = 3, W_HX = 32, W_OWN = 64 };
+// (2 bits at W_VSHIFT: none, per lane, per cell); per-cell E ones (HX) and
+// W ones (HW: a wall on the low x side has only W, one on the high side
+// only E); every ghost of the wave is straight and its fluid neighbour is a
+// cell of the same thread (OWN).
+enum : int { W_GHOST = 1, W_DIAG = 2, W_SEL = 4, W_VSHIFT = 3, W_HX = 32, W_OWN = 64,
+             W_HW = 128 };
 
 // What a thread knows of its 2R own cells, cell i = 2r + q, packed so the
 // codes take a few registers: bit i of `upd` = updated by SOR, of `out` = in
@@ -229,7 +231,7 @@ struct Blk {
     // bit 0: q = 1 reads N, bit 1: q = 0 reads S); VM 2 = per-cell bits
     // (cc.vsub); HX = per-cell E / W bits (cc.hsub).  The form is chosen
     // per wave.
-    template <int c, bool SEL, int VM, bool HX>
+    template <int c, bool SEL, int VM, int HX>
     __device__ static __forceinline__ void update(double (*P)[PS], double (&pv)[R][2],
                                                   const double (&rr)[R][2], const Codes<R>& cc,
                                                   uint32_t vown, int lx0, int m, double omw,
@@ -252,10 +254,8 @@ struct Blk {
                 pn = ((cc.vsub >> i) & 1u) ? pc : pn;
                 ps = ((cc.vsub >> (Codes<R>::SH + i)) & 1u) ? pc : ps;
             }
-            if constexpr (HX) {
-                pe = ((cc.hsub >> i) & 1u) ? pc : pe;
-                pw = ((cc.hsub >> (Codes<R>::SH + i)) & 1u) ? pc : pw;
-            }
+            if constexpr ((HX & 1) != 0) pe = ((cc.hsub >> i) & 1u) ? pc : pe;
+            if constexpr ((HX & 2) != 0) pw = ((cc.hsub >> (Codes<R>::SH + i)) & 1u) ? pc : pw;
             const double tt = fma(pe + pw, rdx2, (ps + pn) * rdy2);
             const double nv = fma(omw, pc, mid * (tt - rr[r][q]));
             pv[r][q] = SEL ? (((cc.upd >> i) & 1u) ? nv : pc) : nv;
@@ -388,7 +388,7 @@ struct Blk {
     // The sweeps in one form of the update (each form has its own loop, so
     // no form's registers meet another's inside it; the forms without corner
     // ghosts have no branch in the loop).
-    template <bool SEL, int VM, bool HX, bool DIAG>
+    template <bool SEL, int VM, int HX, bool DIAG>
     __device__ static __forceinline__ void sweeps(double (*P)[PS], double (&pv)[R][2],
                                                   const double (&rr)[R][2], Codes<R>& cc,
                                                   uint32_t vown, int lx0, int m, int nsw,
@@ -424,15 +424,23 @@ struct Blk {
                                                  double rdy2) {
         const bool ghost = wm & W_GHOST;
         const int vm = (wm >> W_VSHIFT) & 3;
-        const bool hx = wm & W_HX;
+        const int hx = ((wm & W_HX) ? 1 : 0) | ((wm & W_HW) ? 2 : 0);
 #define STM_SW(S, V, X, D) sweeps<S, V, X, D>(P, pv, rr, cc, vown, lx0, m, nsw, omw, mid, rdx2, rdy2)
+#define STM_SWX(V)                                                           \
+    do {                                                                     \
+        if (hx == 0) STM_SW(false, V, 0, false);                             \
+        else if (hx == 1) STM_SW(false, V, 1, false);                        \
+        else if (hx == 2) STM_SW(false, V, 2, false);                        \
+        else STM_SW(false, V, 3, false);                                     \
+    } while (0)
         // a wave with a corner ghost or a fluid cell that is not updated runs
         // the general form: the select, per-cell substitutions, the corner
         // refresh
-        if (wm & (W_SEL | W_DIAG)) STM_SW(true, 2, true, true);
-        else if (vm == 0) { if (hx) STM_SW(false, 0, true, false); else STM_SW(false, 0, false, false); }
-        else if (vm == 1) { if (hx) STM_SW(false, 1, true, false); else STM_SW(false, 1, false, false); }
-        else { if (hx) STM_SW(false, 2, true, false); else STM_SW(false, 2, false, false); }
+        if (wm & (W_SEL | W_DIAG)) STM_SW(true, 2, 3, true);
+        else if (vm == 0) STM_SWX(0);
+        else if (vm == 1) STM_SWX(1);
+        else STM_SWX(2);
+#undef STM_SWX
 #undef STM_SW
         // every ghost from the final values, for the output and the residual
         if (ghost) {
@@ -710,7 +718,8 @@ k_rb_strip(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
             wm |= __ballot(ghm != (cpv | ce | cw)) == 0 ? W_OWN : 0;
             const int vm = __ballot((nsub | ssub) != 0) == 0 ? 0 : (__ballot(!lane_ok) != 0 ? 2 : 1);
             wm |= vm << W_VSHIFT;
-            wm |= __ballot((esub | wsub) != 0) != 0 ? W_HX : 0;
+            wm |= __ballot(esub != 0) != 0 ? W_HX : 0;
+            wm |= __ballot(wsub != 0) != 0 ? W_HW : 0;
             wm |= __ballot(keep != 0) != 0 ? W_SEL : 0;
         }
 #pragma unroll

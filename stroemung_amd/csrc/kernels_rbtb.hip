@@ -777,6 +777,11 @@ k_rb_strip(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
                   : Blk<K, R, HY, NS, 0>::run(P, pv, rr, cc, vown, cpv, cph, lx0, m, nsw, check, wm,
                                               omw, mid, rdx2, rdy2);
 
+    // ---- this tile's residual partial, before the stores: the barrier of
+    // the sum then waits for the residual, not for every wave's write-back
+    const double b = block_sum_to_first<NT>(acc, wsum);
+    if (tid == 0) part_cur[t.idx] = b;
+
     // ---- write the output tile.  Output cells that are neither updated nor
     // ghosts keep their value in every sweep of the sequence (the sweeps may
     // have stored into their registers, see Blk::update): launch 0 copies
@@ -803,8 +808,6 @@ k_rb_strip(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
         else if (o0) pout[g] = pv[r][0];
         else if (o1) pout[g + 1] = pv[r][1];
     }
-    const double b = block_sum_to_first<NT>(acc, wsum);
-    if (tid == 0) part_cur[t.idx] = b;
 }
 
 // After a blocked solve: channels whose latest p is in the second buffer

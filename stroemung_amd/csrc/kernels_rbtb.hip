@@ -528,7 +528,8 @@ k_rb_strip(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
     double2 pp[R], rh[R];
     uint32_t ff[R];
     // a thread's codes (CACHE: 2R <= 16 cells, two 16-byte words: upd | out,
-    // diag | cpv, vsub, hsub; cph, g[0], g[1], vown | wm | rest), loaded
+    // diag | cpv, vsub, hsub; cph, g[0], g[1], vown | wm | rest | the
+    // workgroup's corner bit), loaded
     // unconditionally with the region and used when mode has TB_CODES_GET
     constexpr bool CACHE = 2 * R <= 16;
     uint4 k0 = {}, k1 = {};
@@ -595,6 +596,7 @@ k_rb_strip(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
     uint32_t vown = 0, cpv = 0;
     typename Codes<R>::Sub cph = 0;
     uint32_t rest = 0;   // output cells that keep their value (see the write-back)
+    bool bdiag = true;   // some wave of the workgroup holds a corner ghost
     if (get) {
         cc.upd = k0.x & 0xffffu;
         cc.out = k0.x >> 16;
@@ -607,7 +609,8 @@ k_rb_strip(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
         if constexpr (Codes<R>::NG > 1) cc.g[Codes<R>::NG > 1 ? 1 : 0] = k1.z;
         vown = k1.w & 0xffu;
         wm = (int)((k1.w >> 8) & 0xffu);
-        rest = k1.w >> 16;
+        rest = (k1.w >> 16) & 0x7fffu;
+        bdiag = (k1.w >> 31) != 0;
         if (ex0 & 1) Blk<K, R, HY, NS, 1>::template init<false>(P, FL, pv, fo, cc, lx0, m);
         else Blk<K, R, HY, NS, 0>::template init<false>(P, FL, pv, fo, cc, lx0, m);
     } else {
@@ -727,19 +730,25 @@ k_rb_strip(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
             rest |= (cc.edge(i) == 0 ? 1u : 0u) << i;
         rest &= cc.out & ~cc.upd;
         if (CACHE && (mode & TB_CODES_PUT)) {
+            static_assert(!CACHE || 2 * R <= 15, "rest and the workgroup bit share a word");
+            bdiag = __syncthreads_or((wm & W_DIAG) != 0) != 0;
             uint4* kp = reinterpret_cast<uint4*>(codes) + ((int64_t)t.idx * NT + tid) * 2;
             kp[0] = make_uint4(cc.upd | cc.out << 16, cc.diag | cpv << 16, (uint32_t)cc.vsub,
                                (uint32_t)cc.hsub);
             kp[1] = make_uint4((uint32_t)cph, cc.g[0],
                                Codes<R>::NG > 1 ? cc.g[Codes<R>::NG > 1 ? 1 : 0] : 0u,
-                               vown | (uint32_t)wm << 8 | rest << 16);
+                               vown | (uint32_t)wm << 8 | rest << 16 | (bdiag ? 1u << 31 : 0u));
         }
     }
 
     // the black corner ghosts for the first red half (before the same barrier)
-    if (ex0 & 1) Blk<K, R, HY, NS, 1>::start(P, pv, cc, lx0, m, wm);
-    else Blk<K, R, HY, NS, 0>::start(P, pv, cc, lx0, m, wm);
-    __syncthreads();
+    // (a workgroup without corner ghosts has nothing to refresh here, and
+    // init's barrier already orders the first sweep's reads)
+    if (bdiag) {
+        if (ex0 & 1) Blk<K, R, HY, NS, 1>::start(P, pv, cc, lx0, m, wm);
+        else Blk<K, R, HY, NS, 0>::start(P, pv, cc, lx0, m, wm);
+        __syncthreads();
+    }
 
     // ---- stop decision on the previous launch (identical in every tile)
     const int itermax = (budget > 0 && budget < d_itermax) ? budget : d_itermax;

@@ -421,7 +421,7 @@ struct Blk {
                                                  typename Codes<R>::Sub cph, int lx0, int m,
                                                  int nsw, bool check, int wm,
                                                  double omw, double mid, double rdx2,
-                                                 double rdy2) {
+                                                 double rdy2, bool lds_refresh) {
         const bool ghost = wm & W_GHOST;
         const int vm = (wm >> W_VSHIFT) & 3;
         const int hx = ((wm & W_HX) ? 1 : 0) | ((wm & W_HW) ? 2 : 0);
@@ -453,7 +453,10 @@ struct Blk {
         }
         double acc = 0.0;
         if (check) {
-            __syncthreads();
+            // the residual reads neighbours through LDS: a workgroup none of
+            // whose waves refreshes ghosts there finds them ordered by the
+            // last sweep's barrier
+            if (lds_refresh) __syncthreads();
             acc = resid(P, pv, rr, cc, lx0, m, rdx2, rdy2);
         }
         return acc;
@@ -529,7 +532,7 @@ k_rb_strip(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
     uint32_t ff[R];
     // a thread's codes (CACHE: 2R <= 16 cells, two 16-byte words: upd | out,
     // diag | cpv, vsub, hsub; cph, g[0], g[1], vown | wm | rest | the
-    // workgroup's corner bit), loaded
+    // workgroup's LDS-refresh and corner bits), loaded
     // unconditionally with the region and used when mode has TB_CODES_GET
     constexpr bool CACHE = 2 * R <= 16;
     uint4 k0 = {}, k1 = {};
@@ -597,6 +600,7 @@ k_rb_strip(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
     typename Codes<R>::Sub cph = 0;
     uint32_t rest = 0;   // output cells that keep their value (see the write-back)
     bool bdiag = true;   // some wave of the workgroup holds a corner ghost
+    bool blds = true;    // some wave refreshes its ghosts through LDS after the sweeps
     if (get) {
         cc.upd = k0.x & 0xffffu;
         cc.out = k0.x >> 16;
@@ -609,8 +613,9 @@ k_rb_strip(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
         if constexpr (Codes<R>::NG > 1) cc.g[Codes<R>::NG > 1 ? 1 : 0] = k1.z;
         vown = k1.w & 0xffu;
         wm = (int)((k1.w >> 8) & 0xffu);
-        rest = (k1.w >> 16) & 0x7fffu;
+        rest = (k1.w >> 16) & 0x3fffu;
         bdiag = (k1.w >> 31) != 0;
+        blds = ((k1.w >> 30) & 1u) != 0;
         if (ex0 & 1) Blk<K, R, HY, NS, 1>::template init<false>(P, FL, pv, fo, cc, lx0, m);
         else Blk<K, R, HY, NS, 0>::template init<false>(P, FL, pv, fo, cc, lx0, m);
     } else {
@@ -730,14 +735,16 @@ k_rb_strip(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
             rest |= (cc.edge(i) == 0 ? 1u : 0u) << i;
         rest &= cc.out & ~cc.upd;
         if (CACHE && (mode & TB_CODES_PUT)) {
-            static_assert(!CACHE || 2 * R <= 15, "rest and the workgroup bit share a word");
+            static_assert(!CACHE || 2 * R <= 14, "rest and the workgroup bits share a word");
             bdiag = __syncthreads_or((wm & W_DIAG) != 0) != 0;
+            blds = __syncthreads_or((wm & W_GHOST) != 0 && (wm & W_OWN) == 0) != 0;
             uint4* kp = reinterpret_cast<uint4*>(codes) + ((int64_t)t.idx * NT + tid) * 2;
             kp[0] = make_uint4(cc.upd | cc.out << 16, cc.diag | cpv << 16, (uint32_t)cc.vsub,
                                (uint32_t)cc.hsub);
             kp[1] = make_uint4((uint32_t)cph, cc.g[0],
                                Codes<R>::NG > 1 ? cc.g[Codes<R>::NG > 1 ? 1 : 0] : 0u,
-                               vown | (uint32_t)wm << 8 | rest << 16 | (bdiag ? 1u << 31 : 0u));
+                               vown | (uint32_t)wm << 8 | rest << 16 | (blds ? 1u << 30 : 0u) |
+                                   (bdiag ? 1u << 31 : 0u));
         }
     }
 
@@ -782,9 +789,9 @@ k_rb_strip(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
 
     const double acc =
         (ex0 & 1) ? Blk<K, R, HY, NS, 1>::run(P, pv, rr, cc, vown, cpv, cph, lx0, m, nsw, check, wm,
-                                              omw, mid, rdx2, rdy2)
+                                              omw, mid, rdx2, rdy2, blds)
                   : Blk<K, R, HY, NS, 0>::run(P, pv, rr, cc, vown, cpv, cph, lx0, m, nsw, check, wm,
-                                              omw, mid, rdx2, rdy2);
+                                              omw, mid, rdx2, rdy2, blds);
 
     // ---- this tile's residual partial, before the stores: the barrier of
     // the sum then waits for the residual, not for every wave's write-back

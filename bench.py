@@ -15,6 +15,7 @@ RHS, pressure solve to the residual, velocity update, next step size).
     python bench.py --workload learned_channels_500   # the same, 500 learned steps from rest
     python bench.py --workload learned_uneven     # the 256 uneven channels: plain vs learned
     python bench.py --workload sweeps --n 4096    # the blocked kernel alone (HBM-resident at 4096)
+    python bench.py --workload sync               # relaunch vs graph vs resident-kernel waits
 
 Prints one JSON line.  `value` = interior cells * K / wall seconds of the K
 timed ticks, inputs resident in HBM.  The roofline entry is for the dominant
@@ -81,11 +82,12 @@ def parse():
                          "timings the tuner sees)")
     ap.add_argument("--workload", choices=("cavity", "channels", "learned_cavities",
                                            "learned_channels", "learned_channels_500",
-                                           "learned_uneven", "net", "sweeps"),
+                                           "learned_uneven", "net", "sweeps", "sync"),
                     default="cavity",
                     help="cavity: the 1024x1024 lid cavity (default); channels: 256 "
                          "independent channels of uneven size, seed 0; learned_*: plain vs "
-                         "learned solve")
+                         "learned solve; sync: launch, graph and resident-kernel wait "
+                         "costs")
     ap.add_argument("--channels", type=int, default=256)
     ap.add_argument("--no-graphs", action="store_true",
                     help="learned_cavities: the learned ticks launch every cycle instead of "
@@ -292,6 +294,32 @@ def sweeps_run(a):
                      "halo_factor": loaded / unique, "effective_sweep_gbs": effective,
                      "avg_launch_us": avg_s * 1e6, "working_set_bytes": working_set,
                      "cache_resident": working_set < MALL_BYTES},
+    }
+    print(json.dumps(out), flush=True)
+
+
+def sync_run(a):
+    """What keeping the blocked solver resident between its K-sweep steps
+    would cost, against relaunching it: 1000 steps of the blocked kernel's
+    launch shape on the 1026 x 1026 grid (256 workgroups of 768 threads, each
+    writing a 64 x 64 FP64 tile), as separate launches, replayed from a
+    hipGraph, and inside one resident kernel that waits at a grid-wide
+    barrier or for its four neighbours' flags; HIP events on the library's
+    stream."""
+    from stroemung_amd.simulation import Context
+
+    steps = 1000
+    ctx = Context(0)
+    us = ctx.sync_benchmark(steps)
+    ctx.close()
+    out = {
+        "metric": "cost per step of the blocked kernel's launch shape",
+        "value": us["launch"], "unit": "us", "n_gpus": 1, "steps": steps, "warmup": 10,
+        "ms_per_step": us["launch"] * 1e-3, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "256 workgroups x 768 threads writing 64x64 FP64 tiles, "
+                               "%d steps" % steps},
+        "us_per_step": us,
     }
     print(json.dumps(out), flush=True)
 
@@ -690,6 +718,9 @@ def main():
         return
     if a.workload == "sweeps":
         sweeps_run(a)
+        return
+    if a.workload == "sync":
+        sync_run(a)
         return
     if a.workload == "learned_channels_500":
         learned_500_line(a)

@@ -203,6 +203,19 @@ int stm_solver_bytes(stm_ctx *ctx, double *unique_bytes, double *loaded_bytes);
 /* Streaming device-to-device copy of `bytes` (read + write), `reps` times,
  * on the context's stream: the measured bandwidth ceiling in GB/s. */
 int stm_stream_copy(stm_ctx *ctx, uint64_t bytes, uint32_t reps, double *gbs);
+/* What keeping the blocked solver resident between its K-sweep steps would
+ * cost, against relaunching it: `steps` (10 .. 100000) steps of 256
+ * workgroups of 768 threads (the blocked kernel's shape on the 1026 x 1026
+ * grid), each writing a 64 x 64 FP64 tile, on the context's stream.
+ * Microseconds per step into us[0..7]: us[0] separate launches; us[1] the
+ * same launches replayed from a hipGraph; us[2] one resident kernel with a
+ * grid-wide barrier per step; us[3] one resident kernel whose workgroups
+ * wait for their four neighbours' flags (release / acquire at agent scope);
+ * us[4], us[5] the two waits without the tile write; us[6], us[7] separate
+ * and graph-replayed launches of a kernel that holds every workgroup 10 us.
+ * STM_ERR_STATE when the device cannot hold the 256 workgroups at once or a
+ * neighbour wait reached its poll limit (every wait is bounded). */
+int stm_sync_benchmark(stm_ctx *ctx, uint32_t steps, double *us);
 /* Time every `sample`-th launch of each kernel with HIP events on the
  * context's stream (0 disables). */
 int stm_profile(stm_ctx *ctx, uint32_t sample);

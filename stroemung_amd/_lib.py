@@ -109,6 +109,7 @@ SIGNATURES = {
     "stm_synchronize": (C.c_int, [_CTX]),
     "stm_solver_bytes": (C.c_int, [_CTX, _D, _D]),
     "stm_stream_copy": (C.c_int, [_CTX, C.c_uint64, C.c_uint32, _D]),
+    "stm_sync_benchmark": (C.c_int, [_CTX, C.c_uint32, _D]),
     "stm_profile": (C.c_int, [_CTX, C.c_uint32]),
     "stm_kernel_stats": (C.c_int, [_CTX, C.c_char_p, _D, C.POINTER(C.c_uint64),
                                    C.POINTER(C.c_uint64)]),

@@ -16,7 +16,8 @@ LIB = os.path.join(LIBDIR, "libstroemung_hip.so")
 BUILD = os.path.join(HERE, "_build")
 
 SOURCES = ["api.hip", "kernels_step.hip", "kernels_pressure.hip", "kernels_rbtb.hip",
-           "kernels_learn.hip", "nn_kernels.hip", "nn_net.hip", "kernels_view.hip"]
+           "kernels_learn.hip", "nn_kernels.hip", "nn_net.hip", "kernels_view.hip",
+           "kernels_sync.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

@@ -1699,6 +1699,106 @@ int stm_stream_copy(stm_ctx* ctx, uint64_t bytes, uint32_t reps, double* gbs) {
     return check_launch(ctx);
 }
 
+int stm_sync_benchmark(stm_ctx* ctx, uint32_t steps, double* us) {
+    if (!ctx || !us || steps < 10 || steps > 100000)
+        return fail(ctx, STM_ERR_INVALID_ARG, "invalid arguments");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    constexpr int SIDE = 16, NWG = SIDE * SIDE, TILE = 64 * 64, GRAPH_NODES = 200;
+    constexpr unsigned SPIN = 20000;   // flag polls per wait before giving up
+    {
+        const hipError_t e = sync_coresident(NWG, ctx->device);
+        if (e == hipErrorCooperativeLaunchTooLarge)
+            return fail(ctx, STM_ERR_STATE, "the device cannot hold %d workgroups of %d threads "
+                        "at once", NWG, SYNC_THREADS);
+        HIPCHK(ctx, e);
+    }
+    int khz = 0;
+    HIPCHK(ctx, hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device));
+    const int64_t hold = (int64_t)khz / 100;   // 10 us of wall-clock ticks
+
+    double* out = nullptr;
+    unsigned* flags = nullptr;   // NWG flags, then the error word
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    auto release = [&]() {
+        if (exec) hipGraphExecDestroy(exec);
+        if (graph) hipGraphDestroy(graph);
+        if (e0) hipEventDestroy(e0);
+        if (e1) hipEventDestroy(e1);
+        if (out) hipFree(out);
+        if (flags) hipFree(flags);
+    };
+    hipError_t e = hipMalloc(&out, sizeof(double) * NWG * TILE);
+    if (e == hipSuccess) e = hipMalloc(&flags, sizeof(unsigned) * (NWG + 1));
+    if (e == hipSuccess) e = hipMemsetAsync(flags, 0, sizeof(unsigned) * (NWG + 1), ctx->stream);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    // microseconds per step of the work queued by `work` (timed between two events)
+    auto timed = [&](auto work, double per) -> double {
+        if (e != hipSuccess) return 0.0;
+        e = hipEventRecord(e0, ctx->stream);
+        if (e == hipSuccess) e = work();
+        if (e == hipSuccess) e = hipEventRecord(e1, ctx->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        return ms * 1e3 / per;
+    };
+    const int reps = (int)((steps + GRAPH_NODES - 1) / GRAPH_NODES);
+    for (int kind = 0; kind < 2 && e == hipSuccess; kind++) {   // tile write, 10-us hold
+        auto one = [&](int i) {
+            if (kind == 0) launch_sync_tile(ctx->stream, NWG, out, i);
+            else launch_sync_hold(ctx->stream, NWG, out, hold);
+        };
+        for (int i = 0; i < 10; i++) one(i);
+        const double direct = timed([&]() {
+            for (uint32_t i = 0; i < steps; i++) one((int)i);
+            return hipGetLastError();
+        }, steps);
+        if (e == hipSuccess) e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            for (int i = 0; i < GRAPH_NODES; i++) one(i);
+            e = hipStreamEndCapture(ctx->stream, &graph);
+        }
+        if (e == hipSuccess) e = hipGraphInstantiateWithFlags(&exec, graph, 0);
+        if (e == hipSuccess) e = hipGraphLaunch(exec, ctx->stream);
+        const double replay = timed([&]() {
+            hipError_t r = hipSuccess;
+            for (int i = 0; i < reps && r == hipSuccess; i++) r = hipGraphLaunch(exec, ctx->stream);
+            return r;
+        }, (double)reps * GRAPH_NODES);
+        if (exec) hipGraphExecDestroy(exec);
+        if (graph) hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        us[kind == 0 ? 0 : 6] = direct;
+        us[kind == 0 ? 1 : 7] = replay;
+    }
+    for (int write = 1; write >= 0 && e == hipSuccess; write--) {
+        e = launch_sync_grid(ctx->stream, NWG, out, 10, write);   // the first launch sets up
+        us[write ? 2 : 4] = timed([&]() {
+            return launch_sync_grid(ctx->stream, NWG, out, (int)steps, write);
+        }, steps);
+        for (int pass = 0; pass < 2 && e == hipSuccess; pass++) {
+            e = hipMemsetAsync(flags, 0, sizeof(unsigned) * NWG, ctx->stream);
+            const int n = pass == 0 ? 10 : (int)steps;
+            const double t = timed([&]() {
+                return launch_sync_neighbours(ctx->stream, SIDE, out, flags, n, write, SPIN,
+                                              flags + NWG);
+            }, n);
+            if (pass == 1) us[write ? 3 : 5] = t;
+        }
+    }
+    unsigned spun = 0;
+    if (e == hipSuccess)
+        e = hipMemcpy(&spun, flags + NWG, sizeof(unsigned), hipMemcpyDeviceToHost);
+    release();
+    if (e != hipSuccess) return fail(ctx, STM_ERR_HIP, "sync benchmark: %s", hipGetErrorString(e));
+    if (spun) return fail(ctx, STM_ERR_STATE, "a neighbour wait reached its poll limit");
+    return check_launch(ctx);
+}
+
 int stm_solver_bytes(stm_ctx* ctx, double* unique_bytes, double* loaded_bytes) {
     int rc = ready(ctx);
     if (rc) return rc;

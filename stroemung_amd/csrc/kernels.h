@@ -122,4 +122,14 @@ void launch_learn_target(hipStream_t, int ntiles, int nch, const ChanDesc*, cons
                          double* part, int H, int W, int f, float* dt, int slot0, int dcap,
                          int which);
 
+// kernels_sync.hip: the cost of a resident blocked solver's waits (measurement)
+constexpr int SYNC_THREADS = 768;
+void launch_sync_tile(hipStream_t, int nwg, double* out, int step);
+void launch_sync_hold(hipStream_t, int nwg, double* out, int64_t ticks);
+// hipSuccess when nwg workgroups of both waiting kernels fit on the device at once
+hipError_t sync_coresident(int nwg, int device);
+hipError_t launch_sync_grid(hipStream_t, int nwg, double* out, int steps, int write);
+hipError_t launch_sync_neighbours(hipStream_t, int side, double* out, unsigned* flags,
+                                  int steps, int write, unsigned spin_limit, unsigned* err);
+
 }  // namespace stm

@@ -169,6 +169,17 @@ class Context:
         self._check(lib().stm_stream_copy(self._ctx, nbytes, reps, C.byref(g)))
         return g.value
 
+    def sync_benchmark(self, steps=1000):
+        """Microseconds per step of the blocked kernel's launch shape: separate
+        launches, graph replay, a resident kernel with a grid-wide barrier or
+        with neighbour flags, each with and without its tile write, and a
+        10-us kernel launched and replayed (keys below)."""
+        us = (C.c_double * 8)()
+        self._check(lib().stm_sync_benchmark(self._ctx, steps, us))
+        keys = ("launch", "graph", "grid_barrier", "neighbour_flags", "grid_barrier_no_write",
+                "neighbour_flags_no_write", "launch_10us", "graph_10us")
+        return dict(zip(keys, list(us)))
+
     def autotune(self, launches=8):
         """Times `launches` blocked launches of each variant on the current
         fields (restored afterwards); keeps the fastest per sweep.  Returns

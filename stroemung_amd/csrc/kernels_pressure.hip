@@ -140,9 +140,59 @@ __device__ __forceinline__ double ref_residual_sq(const double* __restrict__ p,
     return r * r;
 }
 
+// The sequential sum from 0.0 of the interior terms t[(x, y)] >= 0, x-major
+// (x outer, y inner), by the first wave of the workgroup: the wave loads 64
+// consecutive terms of a row at once (eight such chunks per batch) and adds
+// them lane by lane into an accumulator every lane holds, so the additions
+// are the sequential ones in the same order, bit for bit.  Lanes past the end
+// of a row add +0.0, which leaves the sum unchanged: it starts at +0.0 and
+// adds squares, so it is never -0.0 (a NaN or infinity stays what it is).
+__device__ double ordered_row_sum(const double* __restrict__ t, int64_t base, int64_t P, int nxi,
+                                  int nyi) {
+    const int lane = threadIdx.x & 63;
+    double acc = 0.0;
+    if (nxi <= 0 || nyi <= 0) return acc;
+    const int64_t chunks = (int64_t)nxi * ((nyi + 63) / 64);
+    int x = 1, y0 = 0;   // the next chunk to load: row x, terms 1 + y0 ..
+    // that chunk: every lane loads (an address clamped into the channel, so
+    // the load is unconditional and one chunk can stay in flight), and `in`
+    // says whether the lane's term is part of the sum (not past the row or
+    // the last row)
+    auto take = [&](bool& in) -> double {
+        in = x <= nxi && y0 + lane < nyi;
+        const double v = t[base + (int64_t)min(x, nxi) * P + 1 + min(y0 + lane, nyi - 1)];
+        y0 += 64;
+        if (y0 >= nyi) {
+            y0 = 0;
+            x++;
+        }
+        return v;
+    };
+    auto fold = [&](double v, bool in) {
+        const int64_t b = __double_as_longlong(in ? v : 0.0);
+        const int lo = (int)(b & 0xffffffffll), hi = (int)(b >> 32);
+#pragma unroll
+        for (int j = 0; j < 64; j++)
+            acc = acc + __hiloint2double(__builtin_amdgcn_readlane(hi, j),
+                                         __builtin_amdgcn_readlane(lo, j));
+    };
+    // eight chunks loaded together, then added in order: one load latency per
+    // 512 terms (chunks past the last row add zeros)
+    constexpr int U = 8;
+    for (int64_t k = 0; k < chunks; k += U) {
+        double v[U];
+        bool in[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) v[u] = take(in[u]);
+#pragma unroll
+        for (int u = 0; u < U; u++) fold(v[u], in[u]);
+    }
+    return acc;
+}
+
 // src/simulation.rs:216-227: sum r^2 over every interior cell, x-major,
 // sequentially from 0.0, divided by the fluid count.  The squares are formed
-// in parallel; one thread then folds them in the reference's order.
+// in parallel; the first wave then folds them in the reference's order.
 __device__ double ref_norm_squared(const ChanDesc& d, const double* __restrict__ p,
                                    const double* __restrict__ rhs, double* __restrict__ sq,
                                    double* sh_out) {
@@ -154,13 +204,9 @@ __device__ double ref_norm_squared(const ChanDesc& d, const double* __restrict__
         sq[i] = ref_residual_sq(p, rhs, i, P, d.delx2, d.dely2);
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        double acc = 0.0;
-        for (int x = 1; x <= nxi; x++) {
-            const double* row = sq + d.off + x * P;
-            for (int y = 1; y <= nyi; y++) acc = acc + row[y];
-        }
-        *sh_out = acc / d.fluid_cells;
+    if (threadIdx.x < 64) {
+        const double acc = ordered_row_sum(sq, d.off, P, nxi, nyi);
+        if (threadIdx.x == 0) *sh_out = acc / d.fluid_cells;
     }
     __syncthreads();
     return *sh_out;

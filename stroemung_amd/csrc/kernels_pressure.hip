@@ -147,7 +147,7 @@ __device__ __forceinline__ double ref_residual_sq(const double* __restrict__ p,
 // are the sequential ones in the same order, bit for bit.  Lanes past the end
 // of a row add +0.0, which leaves the sum unchanged: it starts at +0.0 and
 // adds squares, so it is never -0.0 (a NaN or infinity stays what it is).
-__device__ double ordered_row_sum(const double* __restrict__ t, int64_t base, int64_t P, int nxi,
+__device__ __forceinline__ double ordered_row_sum(const double* __restrict__ t, int64_t base, int64_t P, int nxi,
                                   int nyi) {
     const int lane = threadIdx.x & 63;
     double acc = 0.0;
@@ -230,25 +230,127 @@ k_initial_norm(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
     }
 }
 
+// src/simulation.rs:266-270: the SOR update of one fluid cell from its x + 1,
+// x - 1, y + 1 and y - 1 neighbours, in the reference's association.
+__device__ __forceinline__ double lex_update(const ChanDesc& d, double pc, double pxp, double pxm,
+                                             double pyp, double pym, double r) {
+    return (d.omw * pc) + (d.mid * ((((pxp + pxm) / d.delx2) + ((pyp + pym) / d.dely2)) - r));
+}
+
+// Channels of at most this many cells (ring included) solve with p, rhs, the
+// flags and the norm's terms in LDS (115 KB).
+constexpr int LEX_LDS_CELLS = 4608;
+
 // src/simulation.rs:239-285 for channels with solver == SOLVER_LEX.
 // Lexicographic Gauss-Seidel order (x outer, y inner, in place) makes cell
 // (x, y) read the new W and N values and the old E and S values; sweeping the
 // anti-diagonals x + y = const in increasing order, with a barrier between
 // them, reproduces exactly those reads, so the result is bit-identical.
+//
+// A channel that fits LEX_LDS_CELLS is staged in LDS once, iterated there and
+// written back: the same operations on the same values in the same order.
+// When no anti-diagonal is longer than 64 cells (the shorter side is at most
+// 64), the first wave sweeps alone, one lane per cell of a diagonal: a wave's
+// LDS accesses complete in order, so it needs no workgroup barrier between
+// diagonals (the wavefront-scope fence keeps the compiler from reordering).
 __global__ void __launch_bounds__(1024)
 k_lex_solve(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
             const BEntry* __restrict__ be, const uint8_t* __restrict__ flags,
             const double* __restrict__ rhs, double* __restrict__ p, double* __restrict__ scratch)
 {
     __shared__ double sh_norm;
+    __shared__ double sp[LEX_LDS_CELLS], srhs[LEX_LDS_CELLS], ssq[LEX_LDS_CELLS];
+    __shared__ uint8_t sfl[LEX_LDS_CELLS];
     const ChanDesc d = cd[blockIdx.x];
     if (d.solver != SOLVER_LEX) return;
     ChanState& s = cs[blockIdx.x];
     const int nxi = d.nx - 2, nyi = d.ny - 2;
     const int64_t P = d.pitch;
     const double init = s.init_norm2;
+    const int tid = threadIdx.x;
     double norm2 = 0.0;
     int iters = d.itermax, exhausted = 1;
+    const int ny = d.ny, ncell = d.nx * d.ny;
+    if (ncell <= LEX_LDS_CELLS) {
+        for (int l = tid; l < ncell; l += blockDim.x) {
+            const int x = l / ny, y = l - (l / ny) * ny;
+            const int64_t g = d.off + x * P + y;
+            sp[l] = p[g];
+            srhs[l] = rhs[g];
+            sfl[l] = flags[g];
+        }
+        __syncthreads();
+        const bool one_wave = (nxi < nyi ? nxi : nyi) <= 64;
+        for (int it = 0; it < d.itermax; it++) {
+            // copy_pressure_to_boundaries, src/grid/mod.rs:343-412
+            for (int k = tid; k < d.nb; k += blockDim.x) {
+                const BEntry e = be[d.boff + k];
+                if (e.edge != E_NONE) {
+                    const int64_t o = e.gidx - d.off;
+                    const int l = (int)(o / P) * ny + (int)(o % P);
+                    sp[l] = ghost_value(sp, l, ny, e.edge);
+                }
+            }
+            __syncthreads();
+            if (one_wave) {
+                if (tid < 64) {
+                    for (int dd = 2; dd <= nxi + nyi; dd++) {
+                        const int xlo = dd - nyi > 1 ? dd - nyi : 1;
+                        const int xhi = dd - 1 < nxi ? dd - 1 : nxi;
+                        const int x = xlo + tid;
+                        if (x <= xhi) {
+                            const int l = x * ny + (dd - x);
+                            if (flag_fluid(sfl[l]))
+                                sp[l] = lex_update(d, sp[l], sp[l + ny], sp[l - ny], sp[l + 1],
+                                                   sp[l - 1], srhs[l]);
+                        }
+                        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                    }
+                }
+                __syncthreads();
+            } else {
+                for (int dd = 2; dd <= nxi + nyi; dd++) {
+                    const int xlo = dd - nyi > 1 ? dd - nyi : 1;
+                    const int xhi = dd - 1 < nxi ? dd - 1 : nxi;
+                    for (int x = xlo + tid; x <= xhi; x += blockDim.x) {
+                        const int l = x * ny + (dd - x);
+                        if (flag_fluid(sfl[l]))
+                            sp[l] = lex_update(d, sp[l], sp[l + ny], sp[l - ny], sp[l + 1],
+                                               sp[l - 1], srhs[l]);
+                    }
+                    __syncthreads();
+                }
+            }
+            // src/simulation.rs:216-227, as ref_norm_squared, on the LDS copy
+            for (int k = tid; k < nxi * nyi; k += blockDim.x) {
+                const int l = (1 + k / nyi) * ny + 1 + k % nyi;
+                ssq[l] = ref_residual_sq(sp, srhs, l, ny, d.delx2, d.dely2);
+            }
+            __syncthreads();
+            if (tid < 64) {
+                const double acc = ordered_row_sum(ssq, 0, ny, nxi, nyi);
+                if (tid == 0) sh_norm = acc / d.fluid_cells;
+            }
+            __syncthreads();
+            norm2 = sh_norm;
+            if ((norm2 < init) || (norm2 < d.eps2)) {
+                iters = it + 1;
+                exhausted = 0;
+                break;
+            }
+        }
+        for (int l = tid; l < ncell; l += blockDim.x) {
+            const int x = l / ny, y = l - (l / ny) * ny;
+            p[d.off + x * P + y] = sp[l];
+        }
+        if (tid == 0) {
+            s.sor_iters = (uint32_t)iters;
+            s.norm2 = norm2;
+            s.exhausted = exhausted;
+            s.active = 0;
+        }
+        return;
+    }
     for (int it = 0; it < d.itermax; it++) {
         // copy_pressure_to_boundaries, src/grid/mod.rs:343-412
         for (int k = threadIdx.x; k < d.nb; k += blockDim.x) {
@@ -263,10 +365,7 @@ k_lex_solve(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
                 const int y = dd - x;
                 const int64_t i = d.off + x * P + y;
                 if (!flag_fluid(flags[i])) continue;
-                const double pc = p[i];
-                p[i] = (d.omw * pc)
-                     + (d.mid * ((((p[i + P] + p[i - P]) / d.delx2)
-                                  + ((p[i + 1] + p[i - 1]) / d.dely2)) - rhs[i]));
+                p[i] = lex_update(d, p[i], p[i + P], p[i - P], p[i + 1], p[i - 1], rhs[i]);
             }
             __syncthreads();
         }

@@ -120,10 +120,15 @@ def test_c1_square_obstacle_lex_100_ticks_bit_exact():
     assert st["time"] == osim.s.time
 
 
-@pytest.mark.parametrize("seed", [1, 2])
-def test_mixed_boundaries_lex_ticks_bit_exact(seed):
-    say("test: mixed boundary types, lexicographic ticks, seed", seed)
-    ct, vel = mixed_grid(seed, 37, 29)
+# Grid sizes (ring included) for each form of the lexicographic kernel: in
+# LDS swept by one wave (at most 4 608 cells, shorter interior side <= 64), in
+# LDS swept by the workgroup (shorter side > 64), and in global memory (more
+# than 4 608 cells) with long and with short diagonals.
+@pytest.mark.parametrize("seed,nx,ny", [(1, 37, 29), (2, 37, 29), (3, 67, 68), (4, 92, 72),
+                                        (5, 302, 22)])
+def test_mixed_boundaries_lex_ticks_bit_exact(seed, nx, ny):
+    say("test: mixed boundary types, lexicographic ticks, seed", seed, nx, ny)
+    ct, vel = mixed_grid(seed, nx, ny)
     kw = dict(C1, max_iterations=30, sor_absolute_epsilon=1e-9)
     ctx, ch, osim = _pair(ct, vel, **kw)
     ctx.construct()

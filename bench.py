@@ -16,6 +16,7 @@ RHS, pressure solve to the residual, velocity update, next step size).
     python bench.py --workload learned_uneven     # the 256 uneven channels: plain vs learned
     python bench.py --workload sweeps --n 4096    # the blocked kernel alone (HBM-resident at 4096)
     python bench.py --workload sync               # relaunch vs graph vs resident-kernel waits
+    python bench.py --workload c1                 # square obstacle, reference lexicographic solve
 
 Prints one JSON line.  `value` = interior cells * K / wall seconds of the K
 timed ticks, inputs resident in HBM.  The roofline entry is for the dominant
@@ -82,7 +83,7 @@ def parse():
                          "timings the tuner sees)")
     ap.add_argument("--workload", choices=("cavity", "channels", "learned_cavities",
                                            "learned_channels", "learned_channels_500",
-                                           "learned_uneven", "net", "sweeps", "sync"),
+                                           "learned_uneven", "net", "sweeps", "sync", "c1"),
                     default="cavity",
                     help="cavity: the 1024x1024 lid cavity (default); channels: 256 "
                          "independent channels of uneven size, seed 0; learned_*: plain vs "
@@ -321,6 +322,87 @@ def sync_run(a):
                                "%d steps" % steps},
         "us_per_step": us,
     }
+    print(json.dumps(out), flush=True)
+
+
+def c1_cpu_baseline(steps, got):
+    """Oracle (plain-C port of the reference, one core) on the square-obstacle
+    channel for the same `steps` lexicographic ticks; its final u, v and p
+    are also the check that the GPU's `got` fields are bit-identical."""
+    import numpy as np
+
+    from oracle import pyoracle as O
+    from stroemung_amd import presets
+
+    O.set_threads(1)
+    ct, vel = presets.square_obstacle()
+    sim = O.Sim(ct, vel, solver=O.SOLVER_LEX, **C1_PARAMS).construct()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.tick()
+    el = time.perf_counter() - t0
+    same = all(np.array_equal(np.asarray(got[k]).view(np.uint64),
+                              np.asarray(getattr(sim, k)).view(np.uint64)) for k in ("u", "v", "p"))
+    interior = (ct.shape[0] - 2) * (ct.shape[1] - 2)
+    return {"value": interior * steps / el, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+            "host": host_info(), "ms_per_step": el * 1e3 / steps,
+            "sample": "the whole workload: %d lexicographic ticks of the oracle" % steps}, same
+
+
+# the square-obstacle channel with the reference's settings: 100 x 20 interior
+# cells, xlength 10, ylength 2, Re 100, gamma 0.9, omega 1.7, eps 1e-3, 100
+# iterations, fixed step 0.005
+C1_PARAMS = dict(delx=0.1, dely=0.1, delt=0.005, gamma=0.9, reynolds=100.0,
+                 sor_absolute_epsilon=1e-3, max_iterations=100, omega=1.7)
+
+
+def c1_run(a):
+    """The reference's own CPU-runnable case: flow past a square obstacle,
+    100 x 20 cells, 100 ticks of the reference's lexicographic solve (the
+    same bits as the reference).  One channel on the GPU, then 256 copies
+    advanced together (one workgroup each), with the oracle's one-core time on
+    the same ticks beside them.  The solve is a serial chain (each cell waits
+    for its W and N neighbours, then the residual is summed in order), so one
+    channel is latency-bound; the GPU's gain is in many channels at once."""
+    from stroemung_amd import presets, workloads
+    from stroemung_amd.simulation import Context
+
+    steps, batch = 100, 256
+    ct, vel = presets.square_obstacle()
+    interior = (ct.shape[0] - 2) * (ct.shape[1] - 2)
+    prm = workloads.square_channel_params()
+    timings = {}
+    got = None
+    for n in (1, batch):
+        ctx = Context(0)
+        for _ in range(n):
+            ctx.add_grid(prm, ct, vel)
+        ctx.construct()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ctx.tick()
+        ctx.synchronize()
+        timings[n] = time.perf_counter() - t0
+        if n == 1:
+            got = ctx.download(0)
+        ctx.close()
+    value = interior * steps / timings[1]
+    out = {
+        "metric": "cell-updates/s at fixed Poisson residual, reference lexicographic solve",
+        "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": steps, "warmup": 0,
+        "ms_per_step": timings[1] * 1e3 / steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "flow past a square obstacle, 100x20 cells, Re 100, omega 1.7, "
+                               "eps 1e-3, 100 iterations per tick, fixed step 0.005, %d ticks, "
+                               "lexicographic SOR (reference algorithm)" % steps},
+        "batched": {"channels": batch, "ms_per_step": timings[batch] * 1e3 / steps,
+                    "cell_updates_per_s": batch * interior * steps / timings[batch]},
+    }
+    if not a.no_cpu:
+        cpu, same = c1_cpu_baseline(steps, got)
+        out["cpu_baseline"] = cpu
+        out["bit_identical_to_oracle"] = same
     print(json.dumps(out), flush=True)
 
 
@@ -721,6 +803,9 @@ def main():
         return
     if a.workload == "sync":
         sync_run(a)
+        return
+    if a.workload == "c1":
+        c1_run(a)
         return
     if a.workload == "learned_channels_500":
         learned_500_line(a)

@@ -140,24 +140,43 @@ __device__ __forceinline__ double ref_residual_sq(const double* __restrict__ p,
     return r * r;
 }
 
+// acc + s[0] + s[1] + ... + s[n - 1], added left to right, from LDS: every
+// lane of the wave reads the same address (a broadcast, no bank conflict),
+// sixteen reads in flight, then their additions, one dependent add per term.
+__device__ __forceinline__ double lds_ordered_sum(double acc, const double* s, int n) {
+    int k = 0;
+    for (; k + 16 <= n; k += 16) {
+        double v[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++) v[u] = s[k + u];
+#pragma unroll
+        for (int u = 0; u < 16; u++) acc = acc + v[u];
+    }
+    for (; k < n; k++) acc = acc + s[k];
+    return acc;
+}
+
 // The sequential sum from 0.0 of the interior terms t[(x, y)] >= 0, x-major
 // (x outer, y inner), by the first wave of the workgroup: the wave loads 64
-// consecutive terms of a row at once (eight such chunks per batch) and adds
-// them lane by lane into an accumulator every lane holds, so the additions
-// are the sequential ones in the same order, bit for bit.  Lanes past the end
-// of a row add +0.0, which leaves the sum unchanged: it starts at +0.0 and
-// adds squares, so it is never -0.0 (a NaN or infinity stays what it is).
-__device__ __forceinline__ double ordered_row_sum(const double* __restrict__ t, int64_t base, int64_t P, int nxi,
-                                  int nyi) {
+// consecutive terms of a row at once, eight such chunks per batch, puts the
+// batch in `stage` (512 doubles of LDS) in that order and adds it term by
+// term, so the additions are the sequential ones in the same order, bit for
+// bit.  Lanes past the end of a row add +0.0, which leaves the sum unchanged:
+// it starts at +0.0 and adds squares, so it is never -0.0 (a NaN or infinity
+// stays what it is).  A wave's LDS accesses complete in order, so the batch
+// it stored is what it reads back (the wavefront-scope fences keep the
+// compiler from moving them).
+__device__ __forceinline__ double ordered_row_sum(const double* __restrict__ t, int64_t base,
+                                                  int64_t P, int nxi, int nyi, double* stage) {
     const int lane = threadIdx.x & 63;
     double acc = 0.0;
     if (nxi <= 0 || nyi <= 0) return acc;
     const int64_t chunks = (int64_t)nxi * ((nyi + 63) / 64);
     int x = 1, y0 = 0;   // the next chunk to load: row x, terms 1 + y0 ..
     // that chunk: every lane loads (an address clamped into the channel, so
-    // the load is unconditional and one chunk can stay in flight), and `in`
-    // says whether the lane's term is part of the sum (not past the row or
-    // the last row)
+    // the loads are unconditional and all of a batch are in flight together),
+    // and `in` says whether the lane's term is part of the sum (not past the
+    // row or the last row)
     auto take = [&](bool& in) -> double {
         in = x <= nxi && y0 + lane < nyi;
         const double v = t[base + (int64_t)min(x, nxi) * P + 1 + min(y0 + lane, nyi - 1)];
@@ -168,16 +187,6 @@ __device__ __forceinline__ double ordered_row_sum(const double* __restrict__ t, 
         }
         return v;
     };
-    auto fold = [&](double v, bool in) {
-        const int64_t b = __double_as_longlong(in ? v : 0.0);
-        const int lo = (int)(b & 0xffffffffll), hi = (int)(b >> 32);
-#pragma unroll
-        for (int j = 0; j < 64; j++)
-            acc = acc + __hiloint2double(__builtin_amdgcn_readlane(hi, j),
-                                         __builtin_amdgcn_readlane(lo, j));
-    };
-    // eight chunks loaded together, then added in order: one load latency per
-    // 512 terms (chunks past the last row add zeros)
     constexpr int U = 8;
     for (int64_t k = 0; k < chunks; k += U) {
         double v[U];
@@ -185,7 +194,10 @@ __device__ __forceinline__ double ordered_row_sum(const double* __restrict__ t, 
 #pragma unroll
         for (int u = 0; u < U; u++) v[u] = take(in[u]);
 #pragma unroll
-        for (int u = 0; u < U; u++) fold(v[u], in[u]);
+        for (int u = 0; u < U; u++) stage[u * 64 + lane] = in[u] ? v[u] : 0.0;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        acc = lds_ordered_sum(acc, stage, U * 64);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     }
     return acc;
 }
@@ -204,8 +216,9 @@ __device__ double ref_norm_squared(const ChanDesc& d, const double* __restrict__
         sq[i] = ref_residual_sq(p, rhs, i, P, d.delx2, d.dely2);
     }
     __syncthreads();
+    __shared__ double stage[512];
     if (threadIdx.x < 64) {
-        const double acc = ordered_row_sum(sq, d.off, P, nxi, nyi);
+        const double acc = ordered_row_sum(sq, d.off, P, nxi, nyi, stage);
         if (threadIdx.x == 0) *sh_out = acc / d.fluid_cells;
     }
     __syncthreads();
@@ -299,10 +312,14 @@ k_lex_solve(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
                         const int xhi = dd - 1 < nxi ? dd - 1 : nxi;
                         const int x = xlo + tid;
                         if (x <= xhi) {
+                            // the flag and the values are read together (an
+                            // interior cell's neighbours are all in the
+                            // channel); only a fluid cell stores its update
                             const int l = x * ny + (dd - x);
-                            if (flag_fluid(sfl[l]))
-                                sp[l] = lex_update(d, sp[l], sp[l + ny], sp[l - ny], sp[l + 1],
-                                                   sp[l - 1], srhs[l]);
+                            const uint8_t f = sfl[l];
+                            const double nv = lex_update(d, sp[l], sp[l + ny], sp[l - ny],
+                                                         sp[l + 1], sp[l - 1], srhs[l]);
+                            if (flag_fluid(f)) sp[l] = nv;
                         }
                         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
                     }
@@ -314,21 +331,24 @@ k_lex_solve(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
                     const int xhi = dd - 1 < nxi ? dd - 1 : nxi;
                     for (int x = xlo + tid; x <= xhi; x += blockDim.x) {
                         const int l = x * ny + (dd - x);
-                        if (flag_fluid(sfl[l]))
-                            sp[l] = lex_update(d, sp[l], sp[l + ny], sp[l - ny], sp[l + 1],
-                                               sp[l - 1], srhs[l]);
+                        const uint8_t f = sfl[l];
+                        const double nv = lex_update(d, sp[l], sp[l + ny], sp[l - ny], sp[l + 1],
+                                                     sp[l - 1], srhs[l]);
+                        if (flag_fluid(f)) sp[l] = nv;
                     }
                     __syncthreads();
                 }
             }
-            // src/simulation.rs:216-227, as ref_norm_squared, on the LDS copy
+            // src/simulation.rs:216-227, as ref_norm_squared, on the LDS copy:
+            // the terms in the reference's order (x-major over the interior),
+            // then added in that order
             for (int k = tid; k < nxi * nyi; k += blockDim.x) {
                 const int l = (1 + k / nyi) * ny + 1 + k % nyi;
-                ssq[l] = ref_residual_sq(sp, srhs, l, ny, d.delx2, d.dely2);
+                ssq[k] = ref_residual_sq(sp, srhs, l, ny, d.delx2, d.dely2);
             }
             __syncthreads();
             if (tid < 64) {
-                const double acc = ordered_row_sum(ssq, 0, ny, nxi, nyi);
+                const double acc = lds_ordered_sum(0.0, ssq, nxi * nyi);
                 if (tid == 0) sh_norm = acc / d.fluid_cells;
             }
             __syncthreads();
@@ -364,8 +384,12 @@ k_lex_solve(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
             for (int x = xlo + (int)threadIdx.x; x <= xhi; x += blockDim.x) {
                 const int y = dd - x;
                 const int64_t i = d.off + x * P + y;
-                if (!flag_fluid(flags[i])) continue;
-                p[i] = lex_update(d, p[i], p[i + P], p[i - P], p[i + 1], p[i - 1], rhs[i]);
+                // the flag and the values are read together; only a fluid
+                // cell stores its update
+                const uint8_t f = flags[i];
+                const double nv = lex_update(d, p[i], p[i + P], p[i - P], p[i + 1], p[i - 1],
+                                             rhs[i]);
+                if (flag_fluid(f)) p[i] = nv;
             }
             __syncthreads();
         }

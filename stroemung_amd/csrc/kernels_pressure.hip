@@ -254,6 +254,14 @@ __device__ __forceinline__ double lex_update(const ChanDesc& d, double pc, doubl
 // flags and the norm's terms in LDS (115 KB).
 constexpr int LEX_LDS_CELLS = 4608;
 
+// Lane i of the wave gets lane i - 1's v (lane 0 gets 0.0): a DPP wave shift.
+__device__ __forceinline__ double lane_shift_up(double v) {
+    const int64_t b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), 0x138, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x138, 0xf, 0xf, false);
+    return __longlong_as_double(((int64_t)hi << 32) | (int64_t)(uint32_t)lo);
+}
+
 // src/simulation.rs:239-285 for channels with solver == SOLVER_LEX.
 // Lexicographic Gauss-Seidel order (x outer, y inner, in place) makes cell
 // (x, y) read the new W and N values and the old E and S values; sweeping the
@@ -263,9 +271,15 @@ constexpr int LEX_LDS_CELLS = 4608;
 // A channel that fits LEX_LDS_CELLS is staged in LDS once, iterated there and
 // written back: the same operations on the same values in the same order.
 // When no anti-diagonal is longer than 64 cells (the shorter side is at most
-// 64), the first wave sweeps alone, one lane per cell of a diagonal: a wave's
-// LDS accesses complete in order, so it needs no workgroup barrier between
-// diagonals (the wavefront-scope fence keeps the compiler from reordering).
+// 64), the first wave sweeps alone: a wave's LDS accesses complete in order,
+// so it needs no workgroup barrier between diagonals (the wavefront-scope
+// fence keeps the compiler from reordering).  With at most 64 interior rows
+// each lane owns a row y and takes its cells x = 1, 2, .. on successive
+// diagonals: the W neighbour is the lane's own previous result, the N
+// neighbour the previous result of the lane below (a DPP shift), and E, S,
+// rhs and the flag -- values no earlier diagonal of the sweep writes -- are
+// read from LDS one diagonal ahead.  Otherwise (at most 64 interior columns)
+// a lane takes one cell of each diagonal, all reads from LDS.
 __global__ void __launch_bounds__(1024)
 k_lex_solve(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
             const BEntry* __restrict__ be, const uint8_t* __restrict__ flags,
@@ -294,6 +308,7 @@ k_lex_solve(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
         }
         __syncthreads();
         const bool one_wave = (nxi < nyi ? nxi : nyi) <= 64;
+        const bool row_lanes = nyi <= 64;
         for (int it = 0; it < d.itermax; it++) {
             // copy_pressure_to_boundaries, src/grid/mod.rs:343-412
             for (int k = tid; k < d.nb; k += blockDim.x) {
@@ -305,7 +320,53 @@ k_lex_solve(const ChanDesc* __restrict__ cd, ChanState* __restrict__ cs,
                 }
             }
             __syncthreads();
-            if (one_wave) {
+            if (row_lanes) {
+                if (tid < 64) {
+                    // lane tid owns row y (rows past nyi idle, their addresses
+                    // clamped to row nyi); cur = the cell x the lane takes next,
+                    // with its old value pc, E = (x + 1, y), S = (x, y + 1), rhs,
+                    // flag, and for lane 0 the ghost N = (x, 0)
+                    const int y = tid + 1;
+                    const bool row = y <= nyi;
+                    const int yc = row ? y : nyi;
+                    double w = sp[yc];                                       // (0, y)
+                    double pc = sp[ny + yc];                                 // (1, y)
+                    double e = sp[min(2, d.nx - 1) * ny + yc];               // (2, y)
+                    double sv = sp[ny + yc + 1];                             // (1, y + 1)
+                    double r = srhs[ny + yc];
+                    uint8_t f = sfl[ny + yc];
+                    double ng = sp[ny];                                      // (1, 0)
+                    double mine = 0.0;   // the lane's latest result
+                    for (int dd = 2; dd <= nxi + nyi; dd++) {
+                        const double below = lane_shift_up(mine);
+                        const int x = dd - y;
+                        if (row && x >= 1 && x <= nxi) {
+                            // the next cell's values, read ahead (clamped past nxi)
+                            const int xn = x + 1;
+                            const int ln = xn * ny + yc;
+                            const double e_n = sp[min(xn + 1, d.nx - 1) * ny + yc];
+                            const double s_n = sp[ln + 1];
+                            const double r_n = srhs[ln];
+                            const uint8_t f_n = sfl[ln];
+                            const double g_n = sp[xn * ny];
+                            const double nb = tid == 0 ? ng : below;
+                            const double nv = lex_update(d, pc, e, w, sv, nb, r);
+                            const double res = flag_fluid(f) ? nv : pc;
+                            if (flag_fluid(f)) sp[x * ny + yc] = nv;
+                            w = res;
+                            mine = res;
+                            pc = e;
+                            e = e_n;
+                            sv = s_n;
+                            r = r_n;
+                            f = f_n;
+                            ng = g_n;
+                        }
+                        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                    }
+                }
+                __syncthreads();
+            } else if (one_wave) {
                 if (tid < 64) {
                     for (int dd = 2; dd <= nxi + nyi; dd++) {
                         const int xlo = dd - nyi > 1 ? dd - nyi : 1;

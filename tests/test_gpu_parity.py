@@ -121,11 +121,12 @@ def test_c1_square_obstacle_lex_100_ticks_bit_exact():
 
 
 # Grid sizes (ring included) for each form of the lexicographic kernel: in
-# LDS swept by one wave (at most 4 608 cells, shorter interior side <= 64), in
-# LDS swept by the workgroup (shorter side > 64), and in global memory (more
-# than 4 608 cells) with long and with short diagonals.
-@pytest.mark.parametrize("seed,nx,ny", [(1, 37, 29), (2, 37, 29), (3, 67, 68), (4, 92, 72),
-                                        (5, 302, 22)])
+# LDS (at most 4 608 cells) swept by one wave with a lane per row (at most 64
+# interior rows), by one wave with a lane per cell of a diagonal (at most 64
+# interior columns), or by the workgroup (both sides over 64); and in global
+# memory (more than 4 608 cells) with long and with short diagonals.
+@pytest.mark.parametrize("seed,nx,ny", [(1, 37, 29), (2, 37, 29), (6, 30, 125), (3, 67, 68),
+                                        (4, 92, 72), (5, 302, 22)])
 def test_mixed_boundaries_lex_ticks_bit_exact(seed, nx, ny):
     say("test: mixed boundary types, lexicographic ticks, seed", seed, nx, ny)
     ct, vel = mixed_grid(seed, nx, ny)

@@ -165,8 +165,14 @@ def test_net_gradients_match_reference():
         cos, rel = _cmp(g[off:off + size], gr[off:off + size])
         print("%-10s cos %.5f rel %.3e" % (name, cos, rel))
         # BF16 storage of ~20 activations and of their gradients: ReLU and
-        # pooling decisions flip and the flips propagate, hence a loose bound
-        assert cos > 0.98 and rel < 0.2, name
+        # pooling decisions flip and the flips propagate, most at the 8x8
+        # bottleneck (conv6.w: cos 0.9938, rel 0.112 -- the fixed-order kernels
+        # give the same figures on every run); the head, after the last BF16
+        # rounding, is within 1.2e-3.  The exact-arithmetic comparison below
+        # holds the kernels to 1e-3.
+        assert cos > 0.99 and rel < 0.13, name
+        if name.startswith("head"):
+            assert rel < 5e-3, name
     # the padding of the first layer's input channels gets no gradient
     w0 = g[0:32 * 9 * 32].reshape(32, 9, 32)
     assert np.all(w0[:, :, 2:] == 0.0)
